@@ -1,0 +1,225 @@
+// device_common.hpp -- shared device code: EOS with floors, HLLC face flux,
+// double-double arithmetic and small reduction helpers.
+//
+// Everything here is compiled with -ffp-contract=off: each expression is
+// evaluated in the written order with one rounding per operation, which is
+// how the reference C solver (built by gcc -std=c11, no FMA contraction)
+// evaluates it.  Division and sqrt are the IEEE correctly rounded forms.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#define HYD_RHO_FLOOR 1.0e-5  // hydro_solver_3d.h:10
+#define HYD_P_FLOOR 1.0e-6    // hydro_solver_3d.h:11
+
+namespace hyd {
+
+// hydro_solver_3d.c:11-12: ternaries; a NaN first argument yields the second.
+__device__ __forceinline__ double tmin(double a, double b) { return (a < b) ? a : b; }
+__device__ __forceinline__ double tmax(double a, double b) { return (a > b) ? a : b; }
+
+// Conserved state of one cell plus its primitives (P.rho is recomputed from
+// d by the floor when needed, so it is not stored).
+struct Cell {
+    double d, mx, my, mz, e;  // rho, rho u, rho v, rho w, E
+    double u, v, w, p, c;     // velocities, pressure (floored), sound speed
+};
+
+__device__ __forceinline__ double rho_floor(double d) { return tmax(d, HYD_RHO_FLOOR); }
+
+// cons_to_prim_3d, hydro_solver_3d.c:162-175; sound_speed_3d :191-193.
+__device__ __forceinline__ void prims(Cell& q, double gamma) {
+    double r = rho_floor(q.d);
+    q.u = q.mx / r;
+    q.v = q.my / r;
+    q.w = q.mz / r;
+    double ke = 0.5 * r * (q.u * q.u + q.v * q.v + q.w * q.w);
+    double p = (gamma - 1.0) * (q.e - ke);
+    q.p = tmax(p, HYD_P_FLOOR);
+    q.c = sqrt(gamma * q.p / r);
+}
+
+// Floors and the cons->prim->cons round trip of the update,
+// hydro_solver_3d.c:765-768 with prim_to_cons_3d :177-189.
+__device__ __forceinline__ void floor_roundtrip(double& d, double& mx, double& my, double& mz,
+                                                double& e, double gamma) {
+    d = tmax(d, HYD_RHO_FLOOR);
+    double r = tmax(d, HYD_RHO_FLOOR);
+    double u = mx / r, v = my / r, w = mz / r;
+    double ke = 0.5 * r * (u * u + v * v + w * w);
+    double p = (gamma - 1.0) * (e - ke);
+    p = tmax(p, HYD_P_FLOOR);
+    p = tmax(p, HYD_P_FLOOR);
+    d = r;
+    mx = r * u;
+    my = r * v;
+    mz = r * w;
+    double ke2 = 0.5 * r * (u * u + v * v + w * w);
+    e = p / (gamma - 1.0) + ke2;
+}
+
+// Reflect the velocity component normal to a face through cons->prim->cons,
+// hydro_solver_3d.c:466-469 (x), :500-503 (y), :529-532 (z).
+template <int DIR>
+__device__ __forceinline__ void reflect(double& d, double& mx, double& my, double& mz, double& e,
+                                        double gamma) {
+    double r = tmax(d, HYD_RHO_FLOOR);
+    double u = mx / r, v = my / r, w = mz / r;
+    double ke = 0.5 * r * (u * u + v * v + w * w);
+    double p = (gamma - 1.0) * (e - ke);
+    p = tmax(p, HYD_P_FLOOR);
+    if (DIR == 0) u = -u;
+    if (DIR == 1) v = -v;
+    if (DIR == 2) w = -w;
+    d = r;
+    mx = r * u;
+    my = r * v;
+    mz = r * w;
+    double ke2 = 0.5 * r * (u * u + v * v + w * w);
+    e = p / (gamma - 1.0) + ke2;
+}
+
+// Largest signal speed of a cell as compute_dt_3d sees it
+// (hydro_solver_3d.c:428-440): max(|u|,|v|,|w|) + c.  Adding c is monotone
+// under rounding, so this equals the max of the three sums bit for bit.
+__device__ __forceinline__ double signal_speed(double d, double mx, double my, double mz,
+                                               double e, double gamma) {
+    double r = rho_floor(d);
+    double u = mx / r, v = my / r, w = mz / r;
+    double ke = 0.5 * r * (u * u + v * v + w * w);
+    double p = tmax((gamma - 1.0) * (e - ke), HYD_P_FLOOR);
+    double c = sqrt(gamma * p / r);
+    double a = fabs(u), b = fabs(v), z = fabs(w);
+    double m = tmax(tmax(a, b), z);
+    return m + c;
+}
+
+// HLLC flux through the face between L and R along DIR (0 x, 1 y, 2 z),
+// hllc_riemann_solver_{x,y,z}_3d, hydro_solver_3d.c:239-416.  The branches of
+// the reference become selects: the side (L or R) is chosen first, then the
+// one physical flux and star state of that side are formed, so every lane of
+// a wavefront runs the same instructions and each result is bit-identical to
+// the branch the reference takes.
+template <int DIR>
+__device__ __forceinline__ void hllc(const Cell& L, const Cell& R, double gamma, double F[5]) {
+    const double rL = rho_floor(L.d), rR = rho_floor(R.d);
+    const double uL = DIR == 0 ? L.u : (DIR == 1 ? L.v : L.w);
+    const double uR = DIR == 0 ? R.u : (DIR == 1 ? R.v : R.w);
+    const double SL = tmin(uL - L.c, uR - R.c);
+    const double SR = tmax(uL + L.c, uR + R.c);
+
+    const double num = R.p - L.p + rL * uL * (SL - uL) - rR * uR * (SR - uR);
+    const double den = rL * (SL - uL) - rR * (SR - uR);
+    const double Ss = num / den;
+
+    const bool sup_l = (SL >= 0.0);
+    const bool sup_r = !sup_l && (SR <= 0.0);
+    const bool left = sup_l || (!sup_r && (Ss >= 0.0));
+
+    const double kd = left ? L.d : R.d;
+    const double kmx = left ? L.mx : R.mx;
+    const double kmy = left ? L.my : R.my;
+    const double kmz = left ? L.mz : R.mz;
+    const double ke = left ? L.e : R.e;
+    const double ku = left ? L.u : R.u;
+    const double kv = left ? L.v : R.v;
+    const double kw = left ? L.w : R.w;
+    const double kp = left ? L.p : R.p;
+    const double kr = left ? rL : rR;
+    const double kn = left ? uL : uR;  // normal velocity of the chosen side
+    const double S = left ? SL : SR;
+
+    // compute_flux_{x,y,z}_3d, hydro_solver_3d.c:199-233
+    double f0, f1, f2, f3, f4;
+    if (DIR == 0) {
+        f0 = kmx; f1 = kmx * ku + kp; f2 = kmy * ku; f3 = kmz * ku;
+    } else if (DIR == 1) {
+        f0 = kmy; f1 = kmx * kv; f2 = kmy * kv + kp; f3 = kmz * kv;
+    } else {
+        f0 = kmz; f1 = kmx * kw; f2 = kmy * kw; f3 = kmz * kw + kp;
+    }
+    f4 = (ke + kp) * kn;
+
+    // star state of the chosen side, hydro_solver_3d.c:267-273 / :283-289
+    const double factor = (S - kn) / (S - Ss);
+    const double sd = kr * factor;
+    const double smx = sd * (DIR == 0 ? Ss : ku);
+    const double smy = sd * (DIR == 1 ? Ss : kv);
+    const double smz = sd * (DIR == 2 ? Ss : kw);
+    const double se = sd * (ke / kr + (Ss - kn) * (Ss + kp / (kr * (S - kn))));
+
+    const bool sup = sup_l || sup_r;
+    F[0] = sup ? f0 : f0 + S * (sd - kd);
+    F[1] = sup ? f1 : f1 + S * (smx - kmx);
+    F[2] = sup ? f2 : f2 + S * (smy - kmy);
+    F[3] = sup ? f3 : f3 + S * (smz - kmz);
+    F[4] = sup ? f4 : f4 + S * (se - ke);
+}
+
+// ---------------------------------------------------------------------------
+// double-double sums (compensated): two_sum is Knuth's error-free addition;
+// dd_add is the accurate pair addition of Joldes, Muller and Popescu (2017,
+// Algorithm 6).  dd_add is commutative bit for bit, so an xor butterfly over
+// lanes reproduces a pairwise tree over consecutive leaves.
+struct DD {
+    double hi, lo;
+};
+
+__device__ __forceinline__ DD two_sum(double a, double b) {
+    DD r;
+    r.hi = a + b;
+    double bb = r.hi - a;
+    r.lo = (a - (r.hi - bb)) + (b - bb);
+    return r;
+}
+__device__ __forceinline__ DD fast_two_sum(double a, double b) {
+    DD r;
+    r.hi = a + b;
+    r.lo = b - (r.hi - a);
+    return r;
+}
+__device__ __forceinline__ DD dd_add(DD x, DD y) {
+    DD s = two_sum(x.hi, y.hi);
+    DD t = two_sum(x.lo, y.lo);
+    double c = s.lo + t.hi;
+    DD v = fast_two_sum(s.hi, c);
+    double w = t.lo + v.lo;
+    return fast_two_sum(v.hi, w);
+}
+
+__device__ __forceinline__ double shfl_xor_d(double v, int m) {
+    return __shfl_xor(v, m, 64);
+}
+
+// Sum of a double-double over the 64 lanes of a wavefront as a pairwise tree
+// over lane order (offsets 1, 2, 4, ... 32); every lane ends with the sum.
+__device__ __forceinline__ DD wave_dd_tree(DD v) {
+#pragma unroll
+    for (int o = 1; o < 64; o <<= 1) {
+        DD p;
+        p.hi = shfl_xor_d(v.hi, o);
+        p.lo = shfl_xor_d(v.lo, o);
+        v = dd_add(v, p);
+    }
+    return v;
+}
+
+// Max of a non-negative double over a wavefront, as ordered bit patterns.
+__device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        unsigned long long p = __shfl_xor(v, o, 64);
+        v = p > v ? p : v;
+    }
+    return v;
+}
+
+__device__ __forceinline__ unsigned long long dbits(double x) {
+    return (unsigned long long)__double_as_longlong(x);
+}
+__device__ __forceinline__ double bitsd(unsigned long long b) {
+    return __longlong_as_double((long long)b);
+}
+
+}  // namespace hyd

@@ -1,0 +1,433 @@
+// hydro_kernels.hip -- the Godunov step on the GPU.
+//
+// One step of godunov_step_3d (hydro_solver_3d.c:637-811) is two kernels:
+//
+//  hydro_step_kernel  reads the state planes U_old once, forms every HLLC face
+//      flux on chip, applies the conservative update with floors to interior
+//      cells and the gravity source to all cells, and writes U_pre: interior
+//      cells into U_new, boundary cells into a small face buffer.  It also
+//      reduces the signal speed of the updated interior for the next dt.
+//  hydro_bc_kernel    writes the final boundary shell of U_new from U_pre in
+//      closed form (periodic copies, then x, y, z boundary phases of
+//      hydro_solver_3d.c:456-552 and :779-807) and reduces its signal speed.
+//
+// Layout: five SoA planes of doubles, flat index (i*ny + j)*nz + k, so k is
+// the contiguous axis (IDX3D, hydro_solver_3d.h:12).
+//
+// hydro_step_kernel tiling.  A block owns an 8 (j) x 64 (k) column of cells
+// and marches along x over a chunk of planes.  Ten wavefronts: waves 1..8 own
+// rows j0..j0+7, one lane per k; wave 0 owns row j0-1 and wave 9 row j0+8,
+// the y halo.  At each plane:
+//   x faces: the cell of plane i+1 is carried in registers, F_x is formed by
+//            the owning lane (no LDS), F_x(i-1) is kept from the last plane.
+//   y faces: every row writes its cells (conserved + primitives) to LDS; row
+//            r forms F_y(r, r+1) against row r+1 and passes it through LDS to
+//            row r+1.
+//   z faces: lane l forms F_z(l, l+1) against LDS; lane l+1 receives it by a
+//            lane shuffle.  The z halo cells k0-1 / k0+64 of rows 0..7 are
+//            loaded by lanes 0..7 of the halo waves.
+// Each face is formed once per block (plus the halo faces: 9 y rows for 8
+// cell rows, 65 z faces for 64 cells), each cell's primitives once.
+#include "device_common.hpp"
+#include "hydro_kernels.hpp"
+
+namespace hyd {
+
+namespace {
+constexpr int TK = 64;            // lanes (k) per row
+constexpr int TJ = 8;             // cell rows (j) per block
+constexpr int NW = TJ + 2;        // wavefronts per block
+constexpr int NT = NW * 64;       // threads per block
+constexpr int LR = TJ + 2;        // LDS rows: j0-1 .. j0+8
+constexpr int LC = TK + 2;        // LDS cols: k0-1 .. k0+64
+constexpr int NF = 10;            // LDS fields per cell: d mx my mz e u v w p c
+}  // namespace
+
+__device__ __forceinline__ double dt_from_speed(unsigned long long bits, double dx, double dy,
+                                                double dz, double cfl) {
+    // compute_dt_3d, hydro_solver_3d.c:423 and :444-449
+    double ms = tmax(1.0e-10, bitsd(bits));
+    double tx = dx / ms, ty = dy / ms, tz = dz / ms;
+    return cfl * tmin(tx, tmin(ty, tz));
+}
+
+struct Raw {
+    double d, mx, my, mz, e;
+};
+
+__device__ __forceinline__ void load_raw(const HydroStepArgs& a, long idx, bool valid, Raw& r) {
+    if (valid) {
+        r.d = a.uin[0][idx];
+        r.mx = a.uin[1][idx];
+        r.my = a.uin[2][idx];
+        r.mz = a.uin[3][idx];
+        r.e = a.uin[4][idx];
+    } else {  // outside the grid: a quiet placeholder whose fluxes are never used
+        r.d = 1.0; r.mx = 0.0; r.my = 0.0; r.mz = 0.0; r.e = 1.0;
+    }
+}
+
+__device__ __forceinline__ Cell to_cell(const Raw& r, double gamma) {
+    Cell q;
+    q.d = r.d; q.mx = r.mx; q.my = r.my; q.mz = r.mz; q.e = r.e;
+    prims(q, gamma);
+    return q;
+}
+
+typedef double LdsCells[NF][LR][LC];
+
+__device__ __forceinline__ void lds_put(LdsCells& s, int r, int c, const Cell& q) {
+    s[0][r][c] = q.d; s[1][r][c] = q.mx; s[2][r][c] = q.my; s[3][r][c] = q.mz;
+    s[4][r][c] = q.e; s[5][r][c] = q.u; s[6][r][c] = q.v; s[7][r][c] = q.w;
+    s[8][r][c] = q.p; s[9][r][c] = q.c;
+}
+__device__ __forceinline__ Cell lds_get(const LdsCells& s, int r, int c) {
+    Cell q;
+    q.d = s[0][r][c]; q.mx = s[1][r][c]; q.my = s[2][r][c]; q.mz = s[3][r][c];
+    q.e = s[4][r][c]; q.u = s[5][r][c]; q.v = s[6][r][c]; q.w = s[7][r][c];
+    q.p = s[8][r][c]; q.c = s[9][r][c];
+    return q;
+}
+
+__device__ __forceinline__ long face_index(int nx, int ny, int nz, int i, int j, int k, int f) {
+    // Boundary cell -> slot in the face buffer: x faces take cells with i on a
+    // boundary, y faces the rest with j on a boundary, z faces the remainder.
+    if (i == 0 || i == nx - 1) {
+        int side = (i != 0);
+        return ((long)(side * 5 + f) * ny + j) * nz + k;
+    }
+    long off = 10L * ny * nz;
+    if (j == 0 || j == ny - 1) {
+        int side = (j != 0);
+        return off + ((long)(side * 5 + f) * nx + i) * nz + k;
+    }
+    off += 10L * nx * nz;
+    int side = (k != 0);
+    return off + ((long)(side * 5 + f) * nx + i) * ny + j;
+}
+
+// LDS coordinates: cell (row, col) with row in -1..8, col in -1..64 lives at
+// [row+1][col+1].  s_fy[r+1] holds F_y(r, r+1) for r = -1..7.
+__global__ void __launch_bounds__(NT, 2) hydro_step_kernel(HydroStepArgs a) {
+    __shared__ double s_cell[NF][LR][LC];
+    __shared__ double s_fy[5][TJ + 1][TK];
+    __shared__ double s_fzl[5][TJ];  // F_z(k0-1, k0) for rows 0..7
+
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const int wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+
+    // Block -> tile.  Consecutive blocks land on different XCDs; regroup so
+    // that each XCD walks a contiguous range of tiles (shared halos stay in
+    // that XCD's L2).
+    int b = blockIdx.x;
+    const int nb = gridDim.x;
+    if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
+    const int kt = b % a.ntk;
+    const int jt = (b / a.ntk) % a.ntj;
+    const int xc = b / (a.ntk * a.ntj);
+    const int j0 = jt * TJ, k0 = kt * TK;
+    const int ib = xc * a.cx;
+    const int ie = min(nx, ib + a.cx);
+
+    const int row = wave - 1;  // -1 .. 8
+    const int j = j0 + row;
+    const int k = k0 + lane;
+    const bool valid = (j >= 0) && (j < ny) && (k < nz);
+    const bool compute = (wave >= 1) && (wave <= TJ);
+
+    // z halo cells: wave 0 lanes 0..7 load (j0+lane, k0-1), wave NW-1 lanes
+    // 0..7 load (j0+lane, k0+64).
+    const bool has_halo = (wave == 0 || wave == NW - 1) && (lane < TJ);
+    const int hj = j0 + lane;
+    const int hk = (wave == 0) ? k0 - 1 : k0 + TK;
+    const int hcol = (wave == 0) ? 0 : LC - 1;
+    const bool hvalid = has_halo && (hj < ny) && (hk >= 0) && (hk < nz);
+
+    const long sx = (long)ny * nz;
+    const long cidx = ((long)ib * ny + j) * nz + k;  // own cell at plane ib
+    const long hidx = ((long)ib * ny + hj) * nz + hk;
+
+    const double dt = dt_from_speed(*a.ms_in, a.dx, a.dy, a.dz, a.cfl);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *a.dt_out = dt;
+        *a.ms_clear = 0ull;
+    }
+    const double ax = dt / a.dx, ay = dt / a.dy, az = dt / a.dz;
+    const double gamma = a.gamma;
+
+    double fxl[5] = {0, 0, 0, 0, 0};
+    unsigned long long smax = 0ull;
+
+    {  // prologue: plane ib into LDS, F_x(ib-1) when plane ib is updated
+        Raw r;
+        load_raw(a, cidx, valid, r);
+        const Cell c = to_cell(r, gamma);
+        lds_put(s_cell, row + 1, lane + 1, c);
+        if (has_halo) {
+            Raw h;
+            load_raw(a, hidx, hvalid, h);
+            lds_put(s_cell, lane + 1, hcol, to_cell(h, gamma));
+        }
+        if (ib > 0 && compute) {
+            Raw pr;
+            load_raw(a, cidx - sx, valid, pr);
+            hllc<0>(to_cell(pr, gamma), c, gamma, fxl);
+        }
+    }
+
+    for (int i = ib; i < ie; ++i) {
+        const long off = (long)(i - ib) * sx;
+        const bool has_next = (i + 1 < nx);
+        Raw rn, rh;
+        if (has_next) {  // prefetch plane i+1
+            load_raw(a, cidx + off + sx, valid, rn);
+            if (has_halo) load_raw(a, hidx + off + sx, hvalid, rh);
+        }
+        __syncthreads();  // B1: plane i complete in LDS
+
+        const Cell cur = lds_get(s_cell, row + 1, lane + 1);
+        if (wave <= TJ) {  // rows -1..7: F_y(row, row+1)
+            double fy[5];
+            hllc<1>(cur, lds_get(s_cell, row + 2, lane + 1), gamma, fy);
+#pragma unroll
+            for (int f = 0; f < 5; ++f) s_fy[f][row + 1][lane] = fy[f];
+        }
+        double fz[5] = {0, 0, 0, 0, 0};
+        if (compute) hllc<2>(cur, lds_get(s_cell, row + 1, lane + 2), gamma, fz);
+        if (wave == 0 && lane < TJ) {  // F_z(k0-1, k0) of row `lane`
+            double fh[5];
+            hllc<2>(lds_get(s_cell, lane + 1, 0), lds_get(s_cell, lane + 1, 1), gamma, fh);
+#pragma unroll
+            for (int f = 0; f < 5; ++f) s_fzl[f][lane] = fh[f];
+        }
+        double fzl[5];
+#pragma unroll
+        for (int f = 0; f < 5; ++f) fzl[f] = __shfl_up(fz[f], 1, 64);
+
+        double tx[5] = {0, 0, 0, 0, 0};
+        Cell nxt;
+        if (has_next) {
+            nxt = to_cell(rn, gamma);
+            if (compute) {
+                double fxr[5];
+                hllc<0>(cur, nxt, gamma, fxr);
+#pragma unroll
+                for (int f = 0; f < 5; ++f) {
+                    tx[f] = -ax * (fxr[f] - fxl[f]);
+                    fxl[f] = fxr[f];
+                }
+            }
+        }
+        __syncthreads();  // B2: all reads of plane i done
+
+        if (has_next) {
+            lds_put(s_cell, row + 1, lane + 1, nxt);
+            if (has_halo) lds_put(s_cell, lane + 1, hcol, to_cell(rh, gamma));
+        }
+        __syncthreads();  // B3: fluxes visible
+
+        if (compute && valid) {
+            if (lane == 0) {
+#pragma unroll
+                for (int f = 0; f < 5; ++f) fzl[f] = s_fzl[f][row];
+            }
+            double q[5] = {cur.d, cur.mx, cur.my, cur.mz, cur.e};
+            const bool interior = (i >= 1) && (i <= nx - 2) && (j >= 1) && (j <= ny - 2) &&
+                                  (k >= 1) && (k <= nz - 2);
+            if (interior) {
+                // hydro_solver_3d.c:738-762: x, then y, then z, left to right
+#pragma unroll
+                for (int f = 0; f < 5; ++f) {
+                    double t = tx[f] - ay * (s_fy[f][row + 1][lane] - s_fy[f][row][lane]);
+                    t = t - az * (fz[f] - fzl[f]);
+                    q[f] = q[f] + t;
+                }
+                floor_roundtrip(q[0], q[1], q[2], q[3], q[4], gamma);
+            }
+            const long idx = cidx + off;
+            if (a.gravity) {
+                // apply_gravity_source_terms, hydro_solver_3d.c:606-627
+                const double gx = a.g[0][idx], gy = a.g[1][idx], gz = a.g[2][idx];
+                const double rho = q[0];
+                const double u = q[1] / rho, v = q[2] / rho, w = q[3] / rho;
+                q[1] = q[1] + dt * rho * gx;
+                q[2] = q[2] + dt * rho * gy;
+                q[3] = q[3] + dt * rho * gz;
+                const double work = rho * (u * gx + v * gy + w * gz);
+                q[4] = q[4] + dt * work;
+            }
+            if (interior) {
+#pragma unroll
+                for (int f = 0; f < 5; ++f) a.uout[f][idx] = q[f];
+                unsigned long long sb = dbits(signal_speed(q[0], q[1], q[2], q[3], q[4], gamma));
+                smax = sb > smax ? sb : smax;
+            } else {
+#pragma unroll
+                for (int f = 0; f < 5; ++f) a.face[face_index(nx, ny, nz, i, j, k, f)] = q[f];
+            }
+        }
+    }
+
+    smax = wave_max_u64(smax);
+    if (lane == 0 && smax != 0ull) atomicMax(a.ms_out, smax);
+}
+
+// ---------------------------------------------------------------------------
+// Boundary shell.  For a boundary cell c the reference's sequence
+//   periodic x, periodic y, periodic z, then x, y, z boundary phases
+// resolves to: take U_pre at the source cell s (per axis: an outflow side
+// moves the coordinate one cell inward, a periodic pair maps 0 to n-1), then
+// apply the reflections of the axes on which c sits at a reflective side,
+// x first.  Sources are read from the face buffer when they are boundary
+// cells and from U_new (final interior values) otherwise, so the in-place
+// writes below never feed another thread's read.
+
+template <int DIR>
+__device__ __forceinline__ void maybe_reflect(bool on, double q[5], double gamma) {
+    if (on) reflect<DIR>(q[0], q[1], q[2], q[3], q[4], gamma);
+}
+
+__global__ void __launch_bounds__(256) hydro_bc_kernel(HydroBCArgs a) {
+    const int nx = a.nx, ny = a.ny, nz = a.nz;
+    const long nX = 2L * ny * nz, nY = 2L * (nx - 2) * nz, nZ = 2L * (nx - 2) * (ny - 2);
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long smax = 0ull;
+    if (t < nX + nY + nZ) {
+        int i, j, k;
+        if (t < nX) {
+            long side = t / ((long)ny * nz), rem = t % ((long)ny * nz);
+            i = side ? nx - 1 : 0; j = (int)(rem / nz); k = (int)(rem % nz);
+        } else if (t < nX + nY) {
+            long u = t - nX, per = (long)(nx - 2) * nz;
+            long side = u / per, rem = u % per;
+            i = 1 + (int)(rem / nz); j = side ? ny - 1 : 0; k = (int)(rem % nz);
+        } else {
+            long u = t - nX - nY, per = (long)(nx - 2) * (ny - 2);
+            long side = u / per, rem = u % per;
+            i = 1 + (int)(rem / (ny - 2)); j = 1 + (int)(rem % (ny - 2)); k = side ? nz - 1 : 0;
+        }
+        const int n3[3] = {nx, ny, nz};
+        const int c3[3] = {i, j, k};
+        int s3[3];
+        bool refl[3];
+#pragma unroll
+        for (int d = 0; d < 3; ++d) {
+            const int lo = a.bc[2 * d], hi = a.bc[2 * d + 1];
+            const bool both_per = (lo == 0) && (hi == 0);  // BC_PERIODIC == 0
+            int s = c3[d];
+            bool r = false;
+            if (c3[d] == 0) {
+                if (lo == 2) s = 1;                // outflow
+                else if (both_per) s = n3[d] - 1;  // periodic copy
+                r = (lo == 1);
+            } else if (c3[d] == n3[d] - 1) {
+                if (hi == 2) s = n3[d] - 2;
+                r = (hi == 1);
+            }
+            s3[d] = s;
+            refl[d] = r;
+        }
+        double q[5];
+        const bool src_boundary = (s3[0] == 0 || s3[0] == nx - 1 || s3[1] == 0 ||
+                                   s3[1] == ny - 1 || s3[2] == 0 || s3[2] == nz - 1);
+        if (src_boundary) {
+#pragma unroll
+            for (int f = 0; f < 5; ++f) q[f] = a.face[face_index(nx, ny, nz, s3[0], s3[1], s3[2], f)];
+        } else {
+            const long sidx = ((long)s3[0] * ny + s3[1]) * nz + s3[2];
+#pragma unroll
+            for (int f = 0; f < 5; ++f) q[f] = a.u[f][sidx];
+        }
+        maybe_reflect<0>(refl[0], q, a.gamma);
+        maybe_reflect<1>(refl[1], q, a.gamma);
+        maybe_reflect<2>(refl[2], q, a.gamma);
+        const long idx = ((long)i * ny + j) * nz + k;
+#pragma unroll
+        for (int f = 0; f < 5; ++f) a.u[f][idx] = q[f];
+        smax = dbits(signal_speed(q[0], q[1], q[2], q[3], q[4], a.gamma));
+    }
+    smax = wave_max_u64(smax);
+    if ((threadIdx.x & 63) == 0 && smax != 0ull) atomicMax(a.ms_out, smax);
+}
+
+// Signal speed of a whole state (after an upload).
+__global__ void __launch_bounds__(256) hydro_maxspeed_kernel(const double* __restrict__ d,
+                                                             const double* __restrict__ mx,
+                                                             const double* __restrict__ my,
+                                                             const double* __restrict__ mz,
+                                                             const double* __restrict__ e,
+                                                             long n, double gamma,
+                                                             unsigned long long* out) {
+    unsigned long long smax = 0ull;
+    for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+         c += (long)gridDim.x * blockDim.x) {
+        unsigned long long sb = dbits(signal_speed(d[c], mx[c], my[c], mz[c], e[c], gamma));
+        smax = sb > smax ? sb : smax;
+    }
+    smax = wave_max_u64(smax);
+    if ((threadIdx.x & 63) == 0 && smax != 0ull) atomicMax(out, smax);
+}
+
+// AoS (host layout, 5 doubles per cell) <-> SoA planes.
+__global__ void aos_to_soa_kernel(const double* __restrict__ aos, double* d, double* mx,
+                                  double* my, double* mz, double* e, long n) {
+    for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+         c += (long)gridDim.x * blockDim.x) {
+        d[c] = aos[5 * c]; mx[c] = aos[5 * c + 1]; my[c] = aos[5 * c + 2];
+        mz[c] = aos[5 * c + 3]; e[c] = aos[5 * c + 4];
+    }
+}
+__global__ void soa_to_aos_kernel(double* __restrict__ aos, const double* d, const double* mx,
+                                  const double* my, const double* mz, const double* e, long n) {
+    for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < n;
+         c += (long)gridDim.x * blockDim.x) {
+        aos[5 * c] = d[c]; aos[5 * c + 1] = mx[c]; aos[5 * c + 2] = my[c];
+        aos[5 * c + 3] = mz[c]; aos[5 * c + 4] = e[c];
+    }
+}
+
+// ---------------------------------------------------------------- launchers
+hipError_t launch_hydro_step(const HydroStepArgs& a, hipStream_t st) {
+    const int ntj = (a.ny + TJ - 1) / TJ, ntk = (a.nz + TK - 1) / TK;
+    const int nxc = (a.nx + a.cx - 1) / a.cx;
+    HydroStepArgs b = a;
+    b.ntj = ntj; b.ntk = ntk; b.nxc = nxc;
+    const long nblk = (long)ntj * ntk * nxc;
+    hipLaunchKernelGGL(hydro_step_kernel, dim3((unsigned)nblk), dim3(NT), 0, st, b);
+    return hipGetLastError();
+}
+
+hipError_t launch_hydro_bc(const HydroBCArgs& a, hipStream_t st) {
+    const long tot = 2L * a.ny * a.nz + 2L * (a.nx - 2) * a.nz + 2L * (a.nx - 2) * (a.ny - 2);
+    const unsigned nb = (unsigned)((tot + 255) / 256);
+    hipLaunchKernelGGL(hydro_bc_kernel, dim3(nb), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_hydro_maxspeed(double* const u[5], long n, double gamma, unsigned long long* out,
+                                 hipStream_t st) {
+    long nb = (n + 255) / 256;
+    if (nb > 8192) nb = 8192;
+    hipLaunchKernelGGL(hydro_maxspeed_kernel, dim3((unsigned)nb), dim3(256), 0, st, u[0], u[1],
+                       u[2], u[3], u[4], n, gamma, out);
+    return hipGetLastError();
+}
+
+hipError_t launch_aos_to_soa(const double* aos, double* const u[5], long n, hipStream_t st) {
+    long nb = (n + 255) / 256;
+    if (nb > 8192) nb = 8192;
+    hipLaunchKernelGGL(aos_to_soa_kernel, dim3((unsigned)nb), dim3(256), 0, st, aos, u[0], u[1],
+                       u[2], u[3], u[4], n);
+    return hipGetLastError();
+}
+
+hipError_t launch_soa_to_aos(double* aos, double* const u[5], long n, hipStream_t st) {
+    long nb = (n + 255) / 256;
+    if (nb > 8192) nb = 8192;
+    hipLaunchKernelGGL(soa_to_aos_kernel, dim3((unsigned)nb), dim3(256), 0, st, aos, u[0], u[1],
+                       u[2], u[3], u[4], n);
+    return hipGetLastError();
+}
+
+}  // namespace hyd

@@ -1,0 +1,45 @@
+// hydro_kernels.hpp -- launch interface of the hydro step kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+namespace hyd {
+
+struct HydroStepArgs {
+    const double* uin[5];   // state planes being advanced
+    double* uout[5];        // advanced state (interior cells written here)
+    const double* g[3];     // gravitational acceleration planes (gravity != 0)
+    double* face;           // boundary cells of U_pre (see face_index)
+    const unsigned long long* ms_in;  // signal speed of uin (ordered bits)
+    unsigned long long* ms_out;       // signal speed of uout, accumulated
+    unsigned long long* ms_clear;     // slot reset for the step after next
+    double* dt_out;         // dt used by this step
+    int nx, ny, nz;
+    int cx;                 // planes per block along x
+    int ntj, ntk, nxc;      // filled by the launcher
+    double dx, dy, dz, cfl, gamma;
+    int gravity;
+};
+
+struct HydroBCArgs {
+    double* u[5];           // state being finalised in place
+    const double* face;     // U_pre of the boundary cells
+    unsigned long long* ms_out;
+    int nx, ny, nz;
+    int bc[6];              // xmin xmax ymin ymax zmin zmax (BCType values)
+    double gamma;
+};
+
+// face buffer size in doubles for an nx x ny x nz grid
+inline long face_buffer_len(int nx, int ny, int nz) {
+    return 10L * ny * nz + 10L * nx * nz + 10L * nx * ny;
+}
+
+hipError_t launch_hydro_step(const HydroStepArgs& a, hipStream_t st);
+hipError_t launch_hydro_bc(const HydroBCArgs& a, hipStream_t st);
+hipError_t launch_hydro_maxspeed(double* const u[5], long n, double gamma,
+                                 unsigned long long* out, hipStream_t st);
+hipError_t launch_aos_to_soa(const double* aos, double* const u[5], long n, hipStream_t st);
+hipError_t launch_soa_to_aos(double* aos, double* const u[5], long n, hipStream_t st);
+
+}  // namespace hyd
