@@ -1,0 +1,541 @@
+// gravity_api.hip -- the gravity_solver_3d.h API on the GPU.
+//
+// Lifecycle, parameters and diagnostics follow gravity_solver_3d.c:49-155;
+// the solve follows gravity_solver_3d.c:510-747 and :753-911 with the
+// potential, density and acceleration resident on the device.  The device
+// potential is authoritative: the host arrays phi/gx/gy/gz are refreshed
+// when an API call that promises them returns, and gravity_sync_to_device
+// pushes a caller-edited phi back.
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include <algorithm>
+
+#include "../../include/hydro_mi355x.h"
+#include "../../include/input_deck.h"
+#include "host_common.hpp"
+
+using hyd::GridDims;
+using hyd::SorCoef;
+using hyd::SorCtl;
+
+hipStream_t hyd_stream();
+
+namespace {
+
+constexpr double kPi = 3.14159265358979323846;  // M_PI of glibc
+constexpr int kW = hyd::MP_LMAX_MAX + 1;
+
+double fact(int n) {
+    double f = 1.0;
+    for (int i = 2; i <= n; ++i) f *= (double)i;
+    return f;
+}
+
+template <class T>
+int dev_alloc(T** p, long count) {
+    HYD_CHECK(hipMalloc((void**)p, sizeof(T) * (size_t)(count > 0 ? count : 1)));
+    HYD_CHECK(hipMemsetAsync(*p, 0, sizeof(T) * (size_t)(count > 0 ? count : 1), hyd_stream()));
+    return 0;
+}
+
+void dev_free(GravDevice* d) {
+    if (!d) return;
+    void* ptrs[] = {d->phi_r, d->phi_b, d->rhs_r, d->rhs_b, d->gx,  d->gy, d->gz,
+                    d->rho_tmp, d->nat_tmp, d->mom, d->mom_scratch, d->wlm, d->ctl, d->acc,
+                    d->e_r, d->e_b, d->r_r, d->r_b};
+    for (void* p : ptrs)
+        if (p) (void)hipFree(p);
+    if (d->ctl_host) (void)hipHostFree(d->ctl_host);
+    if (d->acc_host) (void)hipHostFree(d->acc_host);
+    if (d->ev0) (void)hipEventDestroy(d->ev0);
+    if (d->ev1) (void)hipEventDestroy(d->ev1);
+    delete d;
+}
+
+int dev_create(GravitySolver3D* s) {
+    GravDevice* d = new GravDevice();
+    s->dev = d;
+    GridDims& g = d->g;
+    g.nx = s->nx; g.ny = s->ny; g.nz = s->nz; g.nzh = (s->nz + 1) / 2;
+    g.xmin = s->xmin; g.xmax = s->xmax; g.ymin = s->ymin; g.ymax = s->ymax;
+    g.zmin = s->zmin; g.zmax = s->zmax;
+    g.dx = s->dx; g.dy = s->dy; g.dz = s->dz;
+    d->n = (long)s->nx * s->ny * s->nz;
+    d->nsplit = (long)s->nx * s->ny * g.nzh;
+    d->stream = hyd_stream();
+    if (!d->stream) return -2;
+    if (dev_alloc(&d->phi_r, d->nsplit) || dev_alloc(&d->phi_b, d->nsplit) ||
+        dev_alloc(&d->rhs_r, d->nsplit) || dev_alloc(&d->rhs_b, d->nsplit) ||
+        dev_alloc(&d->gx, d->n) || dev_alloc(&d->gy, d->n) || dev_alloc(&d->gz, d->n) ||
+        dev_alloc(&d->mom, 13 + hyd::mp_n_hi(hyd::MP_LMAX_MAX)) || dev_alloc(&d->wlm, kW * kW) ||
+        dev_alloc(&d->ctl, 1) || dev_alloc(&d->acc, 2))
+        return -2;
+    d->mom_scratch_len = hyd::moments_scratch_len(d->n, hyd::MP_LMAX_MAX);
+    if (dev_alloc(&d->mom_scratch, d->mom_scratch_len)) return -2;
+    HYD_CHECK(hipHostMalloc((void**)&d->ctl_host, sizeof(SorCtl), hipHostMallocDefault));
+    HYD_CHECK(hipHostMalloc((void**)&d->acc_host, 2 * sizeof(double), hipHostMallocDefault));
+    HYD_CHECK(hipEventCreate(&d->ev0));
+    HYD_CHECK(hipEventCreate(&d->ev1));
+    // weight of the (l,m) term of the addition theorem for 1/|r - r'|
+    double w[kW * kW];
+    for (int l = 0; l < kW; ++l)
+        for (int m = 0; m < kW; ++m) {
+            double v = 0.0;
+            if (m <= l) {
+                v = fact(l - m) / fact(l + m);
+                if (m > 0) v = 2.0 * v;
+            }
+            w[l * kW + m] = v;
+        }
+    HYD_CHECK(hipMemcpyAsync(d->wlm, w, sizeof w, hipMemcpyHostToDevice, d->stream));
+    HYD_CHECK(hipStreamSynchronize(d->stream));
+    return 0;
+}
+
+SorCoef coefficients(const GravitySolver3D* s) {
+    // gravity_solver_3d.c:519-545
+    SorCoef c;
+    const double dx2 = s->dx * s->dx, dy2 = s->dy * s->dy, dz2 = s->dz * s->dz;
+    c.dx2i = 1.0 / dx2;
+    c.dy2i = 1.0 / dy2;
+    c.dz2i = 1.0 / dz2;
+    c.factor = 2.0 * (c.dx2i + c.dy2i + c.dz2i);
+    c.omega = s->omega;
+    return c;
+}
+
+double bitsd(unsigned long long b) {
+    double v;
+    memcpy(&v, &b, sizeof v);
+    return v;
+}
+
+// Next batch of iterations from the observed convergence rate: enough to
+// reach the tolerance if the rate holds, so that one host round trip usually
+// ends the solve; extra launches after convergence exit at once.
+int next_batch(double r1, double r2, double tol, int prev) {
+    if (!(r1 > 0.0) || !(r2 > 0.0)) return std::min(2 * prev, 1024);
+    const double rate = r1 / r2;
+    if (!(rate < 1.0) || !(rate > 0.0)) return std::min(2 * prev, 1024);
+    const double est = log(tol / r1) / log(rate);
+    int b = (int)ceil(est * 1.02) + 2;
+    return std::max(4, std::min(b, 4096));
+}
+
+int rel_residual_dev(GravitySolver3D* s, float* rr, float* rb, double* rel) {
+    GravDevice* d = s->dev;
+    SorCoef c = coefficients(s);
+    HYD_CHECK(hipMemsetAsync(d->acc, 0, 2 * sizeof(double), d->stream));
+    HYD_CHECK(hyd::launch_residual(d->g, c, d->phi_r, d->phi_b, d->rhs_r, d->rhs_b, rr, rb, d->acc,
+                                   d->stream));
+    HYD_CHECK(hipMemcpyAsync(d->acc_host, d->acc, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                             d->stream));
+    HYD_CHECK(hipStreamSynchronize(d->stream));
+    const double rr2 = d->acc_host[0], ff = d->acc_host[1];
+    *rel = ff > 0.0 ? sqrt(rr2 / ff) : sqrt(rr2);
+    return 0;
+}
+
+int apply_boundary(GravitySolver3D* s) {
+    GravDevice* d = s->dev;
+    const int bc = (s->bc_type == GRAV_BC_PERIODIC) ? 0 : (s->bc_type == GRAV_BC_ISOLATED ? 1 : 2);
+    HYD_CHECK(hyd::launch_sor_boundary(d->g, bc, s->G, s->multipole_lmax, d->mom, d->wlm, d->phi_r,
+                                       d->phi_b, d->stream));
+    return 0;
+}
+
+// Red-black SOR in FP64 with the reference's iteration structure
+// (gravity_solver_3d.c:566-746): red, black, boundary, check.  The boundary
+// values depend only on the density, so they are written once, after the
+// first iteration, exactly where the reference first writes them.
+int sor_fp64(GravitySolver3D* s) {
+    GravDevice* d = s->dev;
+    const SorCoef c = coefficients(s);
+    const bool rel = (s->criterion == SOR_CRIT_REL_RESID);
+    const double tol = s->tolerance;
+    HYD_CHECK(hyd::launch_sor_reset(d->ctl, d->stream));
+    if (s->max_iterations <= 0) {
+        s->last_iterations = 0;
+        s->last_residual = 0.0;
+        return -1;
+    }
+    int it = 0, batch = 8;
+    double rel_prev = -1.0;
+    int it_prev = 0;
+    while (true) {
+        const int nb = std::min(batch, s->max_iterations - it);
+        for (int q = 0; q < nb; ++q, ++it) {
+            // in the relative-residual mode the sweeps never stop on their own
+            const double stol = rel ? -1.0 : tol;
+            HYD_CHECK(hyd::launch_sor_sweep(d->g, c, 0, it, stol, d->phi_r, d->phi_b, d->rhs_r,
+                                            d->rhs_b, d->ctl, d->stream));
+            HYD_CHECK(hyd::launch_sor_sweep(d->g, c, 1, it, stol, d->phi_r, d->phi_b, d->rhs_r,
+                                            d->rhs_b, d->ctl, d->stream));
+            if (it == 0 && apply_boundary(s)) return -2;
+        }
+        if (rel) {
+            double r;
+            if (rel_residual_dev(s, nullptr, nullptr, &r)) return -2;
+            s->last_rel_residual = r;
+            if (r <= tol) {
+                s->last_iterations = it;
+                s->last_residual = r;
+                return 0;
+            }
+            if (it >= s->max_iterations) {
+                s->last_iterations = it;
+                s->last_residual = r;
+                return -1;
+            }
+            int b = batch;
+            if (rel_prev > 0.0 && r < rel_prev) {
+                const double rate = pow(r / rel_prev, 1.0 / (it - it_prev));
+                b = (int)ceil(log(tol / r) / log(rate) * 1.02) + 1;
+            } else {
+                b = 2 * batch;
+            }
+            rel_prev = r;
+            it_prev = it;
+            batch = std::max(8, std::min(b, 4096));
+            continue;
+        }
+        HYD_CHECK(hipMemcpyAsync(d->ctl_host, d->ctl, sizeof(SorCtl), hipMemcpyDeviceToHost,
+                                 d->stream));
+        HYD_CHECK(hipStreamSynchronize(d->stream));
+        const SorCtl& h = *d->ctl_host;
+        if (h.converged) {
+            s->last_iterations = h.iters;
+            s->last_residual = h.last_res;
+            return 0;
+        }
+        const double r1 = bitsd(h.res[(it - 1) % 3]);
+        if (it >= s->max_iterations) {
+            s->last_iterations = it;
+            s->last_residual = r1;
+            if (r1 < tol) return 0;  // converged on the last allowed iteration
+            return -1;
+        }
+        const double r2 = it >= 2 ? bitsd(h.res[(it - 2) % 3]) : 0.0;
+        batch = next_batch(r1, r2, tol, batch);
+    }
+}
+
+int ensure_dc_buffers(GravDevice* d) {
+    if (d->e_r) return 0;
+    if (dev_alloc(&d->e_r, d->nsplit) || dev_alloc(&d->e_b, d->nsplit) ||
+        dev_alloc(&d->r_r, d->nsplit) || dev_alloc(&d->r_b, d->nsplit))
+        return -2;
+    return 0;
+}
+
+// Defect correction: outer FP64 residual r = f - L phi, inner FP32
+// red-black sweeps on L e = r with e = 0 on the boundary, phi += e in FP64.
+// Stops on the relative residual ||r|| / ||f|| <= tolerance.
+int sor_fp32_dc(GravitySolver3D* s) {
+    GravDevice* d = s->dev;
+    if (ensure_dc_buffers(d)) return -2;
+    const SorCoef c = coefficients(s);
+    if (apply_boundary(s)) return -2;
+    double rel;
+    if (rel_residual_dev(s, d->r_r, d->r_b, &rel)) return -2;
+    int sweeps = 0, k = 64;
+    const double tol = s->tolerance;
+    for (int outer = 0; outer < 64 && rel > tol && sweeps < s->max_iterations; ++outer) {
+        const int nk = std::min(k, s->max_iterations - sweeps);
+        for (int q = 0; q < nk; ++q) {
+            HYD_CHECK(hyd::launch_sor_sweep_f32(d->g, c, 0, d->e_r, d->e_b, d->r_r, d->r_b,
+                                                d->stream));
+            HYD_CHECK(hyd::launch_sor_sweep_f32(d->g, c, 1, d->e_r, d->e_b, d->r_r, d->r_b,
+                                                d->stream));
+        }
+        sweeps += nk;
+        HYD_CHECK(hyd::launch_correct(d->g, d->phi_r, d->phi_b, d->e_r, d->e_b, d->stream));
+        double rnew;
+        if (rel_residual_dev(s, d->r_r, d->r_b, &rnew)) return -2;
+        // per-sweep contraction of this round -> sweeps for the next round,
+        // aiming at a 1e-4 reduction (well inside FP32 reach) or the target
+        const double red = rnew / rel;
+        if (red > 0.0 && red < 1.0) {
+            const double rate = pow(red, 1.0 / nk);
+            const double want = std::max(1e-4, 0.5 * tol / rnew);
+            k = (int)ceil(log(want) / log(rate));
+            k = std::max(8, std::min(k, 8192));
+        } else {
+            k = std::min(2 * k, 8192);
+        }
+        rel = rnew;
+    }
+    s->last_iterations = sweeps;
+    s->last_residual = rel;
+    s->last_rel_residual = rel;
+    return rel <= tol ? 0 : -1;
+}
+
+int potential_device(GravitySolver3D* s, const double* d_rho) {
+    GravDevice* d = s->dev;
+    HYD_CHECK(hipEventRecord(d->ev0, d->stream));
+    const double four_pi_G = 4.0 * kPi * s->G;  // gravity_solver_3d.c:527
+    HYD_CHECK(hyd::launch_natural_to_split(d->g, d_rho, four_pi_G, d->rhs_r, d->rhs_b, d->stream));
+    if (s->bc_type == GRAV_BC_ISOLATED)
+        HYD_CHECK(hyd::launch_moments(d->g, d_rho, s->multipole_lmax, d->mom_scratch, d->mom,
+                                      d->stream));
+    int st = (s->poisson_mode == POISSON_FP32_DC) ? sor_fp32_dc(s) : sor_fp64(s);
+    HYD_CHECK(hipEventRecord(d->ev1, d->stream));
+    HYD_CHECK(hipEventSynchronize(d->ev1));
+    float ms = 0.0f;
+    HYD_CHECK(hipEventElapsedTime(&ms, d->ev0, d->ev1));
+    s->last_solve_ms = ms;
+    s->last_status = st;
+    if (st == -1)  // gravity_solver_3d.c:740-742
+        fprintf(stderr, "Warning: Gravity solver did not converge in %d iterations.\n",
+                s->last_iterations);
+    return st;
+}
+
+int accel_device(GravitySolver3D* s) {
+    GravDevice* d = s->dev;
+    HYD_CHECK(hyd::launch_acceleration(d->g, d->phi_r, d->phi_b, d->gx, d->gy, d->gz, d->stream));
+    d->g_valid = true;
+    return 0;
+}
+
+int download_phi(GravitySolver3D* s) {
+    GravDevice* d = s->dev;
+    if (!d->nat_tmp && dev_alloc(&d->nat_tmp, d->n)) return -2;
+    HYD_CHECK(hyd::launch_split_to_natural(d->g, d->phi_r, d->phi_b, d->nat_tmp, d->stream));
+    HYD_CHECK(hipMemcpyAsync(s->phi, d->nat_tmp, sizeof(double) * d->n, hipMemcpyDeviceToHost,
+                             d->stream));
+    HYD_CHECK(hipStreamSynchronize(d->stream));
+    return 0;
+}
+
+int download_g(GravitySolver3D* s) {
+    GravDevice* d = s->dev;
+    const size_t b = sizeof(double) * d->n;
+    HYD_CHECK(hipMemcpyAsync(s->gx, d->gx, b, hipMemcpyDeviceToHost, d->stream));
+    HYD_CHECK(hipMemcpyAsync(s->gy, d->gy, b, hipMemcpyDeviceToHost, d->stream));
+    HYD_CHECK(hipMemcpyAsync(s->gz, d->gz, b, hipMemcpyDeviceToHost, d->stream));
+    HYD_CHECK(hipStreamSynchronize(d->stream));
+    return 0;
+}
+
+int upload_rho(GravitySolver3D* s, const double* rho) {
+    GravDevice* d = s->dev;
+    if (!d->rho_tmp && dev_alloc(&d->rho_tmp, d->n)) return -2;
+    HYD_CHECK(hipMemcpyAsync(d->rho_tmp, rho, sizeof(double) * d->n, hipMemcpyHostToDevice,
+                             d->stream));
+    return 0;
+}
+
+}  // namespace
+
+// ------------------------------------------------------------------ C ABI --
+extern "C" {
+
+GravitySolver3D* gravity_solver_create(int nx, int ny, int nz, double xmin, double xmax,
+                                       double ymin, double ymax, double zmin, double zmax,
+                                       double G) {
+    if (nx < 3 || ny < 3 || nz < 3) {
+        fprintf(stderr,
+                "Error: gravity solver needs nx, ny, nz >= 3 on the device (got %d x %d x %d).\n",
+                nx, ny, nz);
+        return NULL;
+    }
+    GravitySolver3D* s = (GravitySolver3D*)calloc(1, sizeof(GravitySolver3D));
+    if (!s) {
+        fprintf(stderr, "Error: Failed to allocate gravity solver structure.\n");
+        return NULL;
+    }
+    // gravity_solver_3d.c:60-86
+    s->nx = nx; s->ny = ny; s->nz = nz;
+    s->xmin = xmin; s->xmax = xmax; s->ymin = ymin; s->ymax = ymax; s->zmin = zmin; s->zmax = zmax;
+    s->dx = (xmax - xmin) / nx;
+    s->dy = (ymax - ymin) / ny;
+    s->dz = (zmax - zmin) / nz;
+    s->G = G;
+    s->omega = 1.5;
+    s->max_iterations = 10000;
+    s->tolerance = 1.0e-6;
+    s->bc_type = GRAV_BC_ISOLATED;
+    s->multipole_lmax = 2;
+    s->poisson_mode = POISSON_FP64_SOR;
+    s->criterion = SOR_CRIT_MAX_UPDATE;
+    const long n = (long)nx * ny * nz;
+    s->phi = (double*)calloc(n, sizeof(double));
+    s->rho = NULL;
+    s->gx = (double*)calloc(n, sizeof(double));
+    s->gy = (double*)calloc(n, sizeof(double));
+    s->gz = (double*)calloc(n, sizeof(double));
+    if (!s->phi || !s->gx || !s->gy || !s->gz || dev_create(s) != 0) {
+        fprintf(stderr, "Error: Failed to allocate gravity solver arrays.\n");
+        gravity_solver_destroy(s);
+        return NULL;
+    }
+    return s;
+}
+
+void gravity_solver_destroy(GravitySolver3D* s) {
+    if (!s) return;
+    free(s->phi);
+    free(s->gx);
+    free(s->gy);
+    free(s->gz);
+    dev_free(s->dev);
+    free(s);
+}
+
+int gravity_solve_potential(GravitySolver3D* s, const double* rho_field) {
+    s->rho = (double*)rho_field;  // gravity_solver_3d.c:531
+    if (upload_rho(s, rho_field)) return -2;
+    int st = potential_device(s, s->dev->rho_tmp);
+    if (st < -1) return st;
+    if (download_phi(s)) return -2;
+    return st;
+}
+
+void gravity_compute_acceleration(GravitySolver3D* s) {
+    if (accel_device(s) == 0) (void)download_g(s);
+}
+
+void gravity_set_solver_params(GravitySolver3D* s, double omega, int max_iter, double tolerance) {
+    s->omega = omega;
+    s->max_iterations = max_iter;
+    s->tolerance = tolerance;
+}
+
+void gravity_set_boundary_type(GravitySolver3D* s, GravBCType bc_type) { s->bc_type = bc_type; }
+
+void gravity_get_diagnostics(const GravitySolver3D* s, int* iterations, double* residual) {
+    if (iterations) *iterations = s->last_iterations;
+    if (residual) *residual = s->last_residual;
+}
+
+int gravity_solve(GravitySolver3D* s, const double* rho_field) {
+    s->rho = (double*)rho_field;
+    if (upload_rho(s, rho_field)) return -2;
+    int st = potential_device(s, s->dev->rho_tmp);
+    if (st == 0 && accel_device(s)) return -2;  // gravity_solver_3d.c:905-910
+    if (st < -1) return st;
+    if (download_phi(s) || download_g(s)) return -2;
+    return st;
+}
+
+void gravity_write_potential(const GravitySolver3D* s, const char* filename) {
+    // gravity_solver_3d.c:913-943, from the host arrays
+    FILE* fp = fopen(filename, "w");
+    if (!fp) {
+        fprintf(stderr, "Error: Could not open file %s for writing potential.\n", filename);
+        return;
+    }
+    fprintf(fp, "x,y,z,phi,rho,gx,gy,gz\n");
+    for (int i = 0; i < s->nx; i++)
+        for (int j = 0; j < s->ny; j++)
+            for (int k = 0; k < s->nz; k++) {
+                const long idx = IDX3D_GRAV(i, j, k, (long)s->ny, s->nz);
+                const double x = s->xmin + (i + 0.5) * s->dx;
+                const double y = s->ymin + (j + 0.5) * s->dy;
+                const double z = s->zmin + (k + 0.5) * s->dz;
+                const double r = s->rho ? s->rho[idx] : 0.0;
+                fprintf(fp, "%.6f,%.6f,%.6f,%.6e,%.6e,%.6e,%.6e,%.6e\n", x, y, z, s->phi[idx], r,
+                        s->gx[idx], s->gy[idx], s->gz[idx]);
+            }
+    fclose(fp);
+    printf("Wrote gravitational potential to %s\n", filename);
+}
+
+// ------------------------------------------------------------- extensions --
+int gravity_set_multipole_lmax(GravitySolver3D* s, int lmax) {
+    if (lmax < 2 || lmax > hyd::MP_LMAX_MAX) {
+        fprintf(stderr, "gravity_set_multipole_lmax: lmax must be in [2, %d]\n", hyd::MP_LMAX_MAX);
+        return -1;
+    }
+    s->multipole_lmax = lmax;
+    return 0;
+}
+
+int gravity_set_poisson_mode(GravitySolver3D* s, int mode) {
+    if (mode != POISSON_FP64_SOR && mode != POISSON_FP32_DC) return -1;
+    s->poisson_mode = mode;
+    return 0;
+}
+
+int gravity_set_criterion(GravitySolver3D* s, int criterion) {
+    if (criterion != SOR_CRIT_MAX_UPDATE && criterion != SOR_CRIT_REL_RESID) return -1;
+    s->criterion = criterion;
+    return 0;
+}
+
+int gravity_sync_to_device(GravitySolver3D* s) {
+    GravDevice* d = s->dev;
+    if (!d->nat_tmp && dev_alloc(&d->nat_tmp, d->n)) return -2;
+    HYD_CHECK(hipMemcpyAsync(d->nat_tmp, s->phi, sizeof(double) * d->n, hipMemcpyHostToDevice,
+                             d->stream));
+    HYD_CHECK(hyd::launch_natural_to_split(d->g, d->nat_tmp, 1.0, d->phi_r, d->phi_b, d->stream));
+    HYD_CHECK(hipStreamSynchronize(d->stream));
+    return 0;
+}
+
+int gravity_sync_to_host(GravitySolver3D* s) {
+    if (download_phi(s)) return -2;
+    return download_g(s);
+}
+
+int gravity_moments(GravitySolver3D* s, const double* rho, double* out13, double* out_hi) {
+    GravDevice* d = s->dev;
+    if (upload_rho(s, rho)) return -2;
+    HYD_CHECK(hyd::launch_moments(d->g, d->rho_tmp, s->multipole_lmax, d->mom_scratch, d->mom,
+                                  d->stream));
+    const int nhi = hyd::mp_n_hi(s->multipole_lmax);
+    double buf[13 + 128];
+    HYD_CHECK(hipMemcpyAsync(buf, d->mom, sizeof(double) * (13 + nhi), hipMemcpyDeviceToHost,
+                             d->stream));
+    HYD_CHECK(hipStreamSynchronize(d->stream));
+    memcpy(out13, buf, 13 * sizeof(double));
+    if (out_hi) memcpy(out_hi, buf + 13, nhi * sizeof(double));
+    return 0;
+}
+
+int gravity_boundary_potential(GravitySolver3D* s, const double* rho, double* phi_out) {
+    GravDevice* d = s->dev;
+    if (upload_rho(s, rho)) return -2;
+    if (s->bc_type == GRAV_BC_ISOLATED)
+        HYD_CHECK(hyd::launch_moments(d->g, d->rho_tmp, s->multipole_lmax, d->mom_scratch, d->mom,
+                                      d->stream));
+    if (apply_boundary(s)) return -2;
+    if (!d->nat_tmp && dev_alloc(&d->nat_tmp, d->n)) return -2;
+    HYD_CHECK(hyd::launch_split_to_natural(d->g, d->phi_r, d->phi_b, d->nat_tmp, d->stream));
+    HYD_CHECK(hipMemcpyAsync(phi_out, d->nat_tmp, sizeof(double) * d->n, hipMemcpyDeviceToHost,
+                             d->stream));
+    HYD_CHECK(hipStreamSynchronize(d->stream));
+    return 0;
+}
+
+int gravity_rel_residual(GravitySolver3D* s, const double* rho, double* rel) {
+    GravDevice* d = s->dev;
+    if (upload_rho(s, rho)) return -2;
+    HYD_CHECK(hyd::launch_natural_to_split(d->g, d->rho_tmp, 4.0 * kPi * s->G, d->rhs_r, d->rhs_b,
+                                           d->stream));
+    return rel_residual_dev(s, nullptr, nullptr, rel);
+}
+
+int gravity_solve_device(GravitySolver3D* s, const double* d_rho) {
+    return grav_internal_solve(s, d_rho, s->dev->stream);
+}
+
+}  // extern "C"
+
+// Hydro-side entry: potential and, when it converged, acceleration, all on
+// the device (solve_gravity_field, hydro_solver_3d.c:558-591; a solve that
+// did not converge leaves g as it was, gravity_solver_3d.c:907).
+int grav_internal_solve(GravitySolver3D* s, const double* d_rho, hipStream_t) {
+    int st = potential_device(s, d_rho);
+    if (st == 0 && accel_device(s)) return -2;
+    return st;
+}
+
+const double* grav_internal_g(GravitySolver3D* s, int comp) {
+    GravDevice* d = s->dev;
+    return comp == 0 ? d->gx : (comp == 1 ? d->gy : d->gz);
+}

@@ -1,0 +1,662 @@
+// gravity_kernels.hip -- red-black SOR, multipole boundary potential,
+// acceleration and the FP32 defect-correction pieces.
+//
+// The potential is stored red-black split (see gravity_kernels.hpp): a colour
+// sweep reads the other colour's array and writes whole lines of its own.
+// Red-black SOR is order independent within a colour, so each sweep matches
+// the reference's sequential sweep (gravity_solver_3d.c:642-708) bit for bit.
+#include "device_common.hpp"
+#include "gravity_kernels.hpp"
+
+namespace hyd {
+
+__device__ __forceinline__ long sidx(const GridDims& g, int i, int j, int k) {
+    return ((long)i * g.ny + j) * g.nzh + (k >> 1);
+}
+__device__ __forceinline__ double phi_at(const GridDims& g, const double* r, const double* b,
+                                         int i, int j, int k) {
+    return (((i + j + k) & 1) ? b : r)[sidx(g, i, j, k)];
+}
+
+// ------------------------------------------------------------- layouts --
+__global__ void natural_to_split_kernel(GridDims g, const double* __restrict__ nat, double scale,
+                                        double* __restrict__ red, double* __restrict__ black) {
+    const long n = (long)g.nx * g.ny * g.nz;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (long)gridDim.x * blockDim.x) {
+        const int k = (int)(t % g.nz);
+        const long ij = t / g.nz;
+        const int j = (int)(ij % g.ny), i = (int)(ij / g.ny);
+        double* dst = ((i + j + k) & 1) ? black : red;
+        dst[sidx(g, i, j, k)] = scale * nat[t];
+    }
+}
+
+__global__ void split_to_natural_kernel(GridDims g, const double* __restrict__ red,
+                                        const double* __restrict__ black,
+                                        double* __restrict__ nat) {
+    const long n = (long)g.nx * g.ny * g.nz;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (long)gridDim.x * blockDim.x) {
+        const int k = (int)(t % g.nz);
+        const long ij = t / g.nz;
+        const int j = (int)(ij % g.ny), i = (int)(ij / g.ny);
+        nat[t] = phi_at(g, red, black, i, j, k);
+    }
+}
+
+// ----------------------------------------------------------------- SOR --
+// gravity_solver_3d.c:642-708 for one colour.  Thread t covers split entry
+// m of interior row (i, j); its cell is k = 2m + ((i+j+colour)&1).
+__global__ void __launch_bounds__(256) sor_sweep_kernel(GridDims g, SorCoef c, int colour, int it,
+                                                        double tol, double* __restrict__ mine,
+                                                        const double* __restrict__ other,
+                                                        const double* __restrict__ rhs,
+                                                        SorCtl* ctl) {
+    // stop rule of gravity_solver_3d.c:715: iteration it runs only while the
+    // residual of iteration it-1 is still >= tol
+    if (ctl->converged) return;
+    if (it > 0) {
+        const double prev = bitsd(ctl->res[(it - 1) % 3]);
+        if (prev < tol) {
+            if (colour == 0 && blockIdx.x == 0 && threadIdx.x == 0) {
+                ctl->converged = 1;
+                ctl->iters = it;
+                ctl->last_res = prev;
+            }
+            return;
+        }
+    }
+    if (colour == 0 && blockIdx.x == 0 && threadIdx.x == 0) ctl->res[(it + 1) % 3] = 0ull;
+
+    const long nrow = (long)(g.nx - 2) * (g.ny - 2);
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    unsigned long long rb = 0ull;
+    if (t < nrow * g.nzh) {
+        const int m = (int)(t % g.nzh);
+        const long row = t / g.nzh;
+        const int j = 1 + (int)(row % (g.ny - 2));
+        const int i = 1 + (int)(row / (g.ny - 2));
+        const int k = 2 * m + ((i + j + colour) & 1);
+        if (k >= 1 && k <= g.nz - 2) {
+            const long base = ((long)i * g.ny + j) * g.nzh;
+            const long sxh = (long)g.ny * g.nzh;
+            const double o_im = other[base - sxh + m], o_ip = other[base + sxh + m];
+            const double o_jm = other[base - g.nzh + m], o_jp = other[base + g.nzh + m];
+            const double o_km = other[base + ((k - 1) >> 1)], o_kp = other[base + ((k + 1) >> 1)];
+            const double old = mine[base + m];
+            const double lap = (o_im + o_ip) * c.dx2i + (o_jm + o_jp) * c.dy2i + (o_km + o_kp) * c.dz2i;
+            const double gs = (lap - rhs[base + m]) / c.factor;
+            mine[base + m] = old + c.omega * (gs - old);
+            rb = dbits(fabs(gs - old));
+        }
+    }
+    rb = wave_max_u64(rb);
+    if ((threadIdx.x & 63) == 0 && rb != 0ull) atomicMax(&ctl->res[it % 3], rb);
+}
+
+__global__ void sor_reset_kernel(SorCtl* ctl) {
+    ctl->res[0] = 0ull;
+    ctl->res[1] = 0ull;
+    ctl->res[2] = 0ull;
+    ctl->converged = 0;
+    ctl->iters = 0;
+    ctl->last_res = 0.0;
+}
+
+// ---------------------------------------------------- multipole boundary --
+// Regular solid harmonics, the same recurrence as the moment pass:
+//   R_m^m = (2m-1)!! (x+iy)^m, R_{m+1}^m = (2m+1) z R_m^m,
+//   (l-m) R_l^m = (2l-1) z R_{l-1}^m - (l+m-1) r^2 R_{l-2}^m.
+__device__ void solid_harmonics(int lmax, double x, double y, double z,
+                                double C[MP_LMAX_MAX + 1][MP_LMAX_MAX + 1],
+                                double S[MP_LMAX_MAX + 1][MP_LMAX_MAX + 1]) {
+    const double r2 = x * x + y * y + z * z;
+    double cr = 1.0, ci = 0.0;
+    for (int m = 0; m <= lmax; ++m) {
+        if (m > 0) {
+            const double f = (double)(2 * m - 1);
+            const double nr = f * (x * cr - y * ci);
+            const double ni = f * (x * ci + y * cr);
+            cr = nr;
+            ci = ni;
+        }
+        C[m][m] = cr;
+        S[m][m] = ci;
+        if (m + 1 <= lmax) {
+            const double f = (double)(2 * m + 1);
+            C[m + 1][m] = f * z * cr;
+            S[m + 1][m] = f * z * ci;
+        }
+        for (int l = m + 2; l <= lmax; ++l) {
+            const double a = (double)(2 * l - 1), b = (double)(l + m - 1), d = (double)(l - m);
+            C[l][m] = (a * z * C[l - 1][m] - b * r2 * C[l - 2][m]) / d;
+            S[l][m] = (a * z * S[l - 1][m] - b * r2 * S[l - 2][m]) / d;
+        }
+    }
+}
+
+// Multipole potential at a boundary cell centre: monopole + Cartesian
+// quadrupole exactly as gravity_solver_3d.c:341-360, then the l >= 3 terms
+// of the addition theorem for 1/|r - r'| (no reference counterpart).
+__device__ double mp_potential(double G, int lmax, const double* mom, const double* wlm, double x,
+                               double y, double z) {
+    const double M = mom[0];
+    const double* Q = mom + 4;
+    const double rx = x - mom[1], ry = y - mom[2], rz = z - mom[3];
+    const double r = sqrt(rx * rx + ry * ry + rz * rz);
+    double ph = 0.0;
+    if (r > 0.0) {
+        ph = -G * M / r;
+        const double r5 = r * r * r * r * r;
+        const double Qr = Q[0] * rx * rx + Q[1] * rx * ry + Q[2] * rx * rz + Q[3] * ry * rx +
+                          Q[4] * ry * ry + Q[5] * ry * rz + Q[6] * rz * rx + Q[7] * rz * ry +
+                          Q[8] * rz * rz;
+        const double pq = -G * Qr / (2.0 * r5);
+        ph += pq;
+        if (lmax > 2) {
+            double C[MP_LMAX_MAX + 1][MP_LMAX_MAX + 1], S[MP_LMAX_MAX + 1][MP_LMAX_MAX + 1];
+            solid_harmonics(lmax, rx, ry, rz, C, S);
+            const double* hi = mom + 13;
+            double extra = 0.0, rp = r5;
+            int q = 0;
+            for (int l = 3; l <= lmax; ++l) {
+                rp = rp * r * r;
+                double sl = 0.0;
+                for (int m = 0; m <= l; ++m) {
+                    const double A = hi[q], B = hi[q + 1];
+                    q += 2;
+                    sl = sl + wlm[l * (MP_LMAX_MAX + 1) + m] * (A * C[l][m] + B * S[l][m]);
+                }
+                extra = extra + sl / rp;
+            }
+            ph = ph + (-G) * extra;
+        }
+    }
+    return ph;
+}
+
+// Boundary shell: x faces (all j,k), y faces (interior i), z faces (interior
+// i, j).  For an edge cell the reference's last writer is the z face, then
+// the y face, and its coordinate formula differs per face
+// (gravity_solver_3d.c:322-324 vs :399-401 vs :454-456).
+__device__ __forceinline__ void shell_cell(long t, int nx, int ny, int nz, int& i, int& j, int& k) {
+    const long nX = 2L * ny * nz, nY = 2L * (nx - 2) * nz;
+    if (t < nX) {
+        const long side = t / ((long)ny * nz), rem = t % ((long)ny * nz);
+        i = side ? nx - 1 : 0; j = (int)(rem / nz); k = (int)(rem % nz);
+    } else if (t < nX + nY) {
+        const long u = t - nX, per = (long)(nx - 2) * nz;
+        const long side = u / per, rem = u % per;
+        i = 1 + (int)(rem / nz); j = side ? ny - 1 : 0; k = (int)(rem % nz);
+    } else {
+        const long u = t - nX - nY, per = (long)(nx - 2) * (ny - 2);
+        const long side = u / per, rem = u % per;
+        i = 1 + (int)(rem / (ny - 2)); j = 1 + (int)(rem % (ny - 2)); k = side ? nz - 1 : 0;
+    }
+}
+
+__global__ void __launch_bounds__(256) sor_boundary_kernel(GridDims g, int bc, double G, int lmax,
+                                                           const double* __restrict__ mom,
+                                                           const double* __restrict__ wlm,
+                                                           double* __restrict__ phi_r,
+                                                           double* __restrict__ phi_b) {
+    const long tot = 2L * g.ny * g.nz + 2L * (g.nx - 2) * g.nz + 2L * (g.nx - 2) * (g.ny - 2);
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= tot) return;
+    int i, j, k;
+    shell_cell(t, g.nx, g.ny, g.nz, i, j, k);
+    double v;
+    if (bc == 1) {  // isolated: multipole expansion
+        double x, y, z;
+        if (k == 0 || k == g.nz - 1) {
+            x = g.xmin + (i + 0.5) * g.dx;
+            y = g.ymin + (j + 0.5) * g.dy;
+            z = (k == 0) ? g.zmin + 0.5 * g.dz : g.zmax - 0.5 * g.dz;
+        } else if (j == 0 || j == g.ny - 1) {
+            x = g.xmin + (i + 0.5) * g.dx;
+            y = (j == 0) ? g.ymin + 0.5 * g.dy : g.ymax - 0.5 * g.dy;
+            z = g.zmin + (k + 0.5) * g.dz;
+        } else {
+            x = (i == 0) ? g.xmin + 0.5 * g.dx : g.xmax - 0.5 * g.dx;
+            y = g.ymin + (j + 0.5) * g.dy;
+            z = g.zmin + (k + 0.5) * g.dz;
+        }
+        v = mp_potential(G, lmax, mom, wlm, x, y, z);
+    } else if (bc == 0) {  // periodic copy, gravity_solver_3d.c:273-299
+        const int si = (i == 0) ? g.nx - 1 : i;
+        const int sj = (j == 0) ? g.ny - 1 : j;
+        const int sk = (k == 0) ? g.nz - 1 : k;
+        v = phi_at(g, phi_r, phi_b, si, sj, sk);
+    } else {
+        return;  // Dirichlet: boundary left as it is
+    }
+    (((i + j + k) & 1) ? phi_b : phi_r)[sidx(g, i, j, k)] = v;
+}
+
+// --------------------------------------------------------- acceleration --
+// gravity_solver_3d.c:820-897.  The sequential x, y, z boundary copies leave
+// every boundary cell with the interior value at the clamped index.
+__global__ void acceleration_kernel(GridDims g, const double* __restrict__ pr,
+                                    const double* __restrict__ pb, double* __restrict__ gx,
+                                    double* __restrict__ gy, double* __restrict__ gz) {
+    const long n = (long)g.nx * g.ny * g.nz;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (long)gridDim.x * blockDim.x) {
+        const int k = (int)(t % g.nz);
+        const long ij = t / g.nz;
+        const int j = (int)(ij % g.ny), i = (int)(ij / g.ny);
+        const int a = min(max(i, 1), g.nx - 2);
+        const int b = min(max(j, 1), g.ny - 2);
+        const int c = min(max(k, 1), g.nz - 2);
+        gx[t] = -(phi_at(g, pr, pb, a + 1, b, c) - phi_at(g, pr, pb, a - 1, b, c)) / (2.0 * g.dx);
+        gy[t] = -(phi_at(g, pr, pb, a, b + 1, c) - phi_at(g, pr, pb, a, b - 1, c)) / (2.0 * g.dy);
+        gz[t] = -(phi_at(g, pr, pb, a, b, c + 1) - phi_at(g, pr, pb, a, b, c - 1)) / (2.0 * g.dz);
+    }
+}
+
+// -------------------------------------------------------------- moments --
+// Each quantity is summed as a pairwise double-double tree over the flat
+// cell index (zero padded to a power of two): a block reduces 256
+// consecutive cells (lane butterfly, then the four wavefronts pairwise), and
+// reduce passes fold 256 consecutive partials at a time.  The result does
+// not depend on the launch shape beyond that tree, and it is accurate to a
+// few units in the 106th bit before the final rounding.
+
+__device__ __forceinline__ void block_dd_store(DD v, int q, double* __restrict__ part, long nblk,
+                                               DD (*s)[4]) {
+    v = wave_dd_tree(v);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) s[q][w] = v;
+}
+
+__device__ __forceinline__ void cell_coord(const GridDims& g, long t, int& i, int& j, int& k) {
+    k = (int)(t % g.nz);
+    const long ij = t / g.nz;
+    j = (int)(ij % g.ny);
+    i = (int)(ij / g.ny);
+}
+
+// pass 1: M and the first moments (gravity_solver_3d.c:182-198)
+__global__ void __launch_bounds__(256) moments_pass1_kernel(GridDims g,
+                                                            const double* __restrict__ rho,
+                                                            double* __restrict__ part) {
+    __shared__ DD s[4][4];
+    const long n = (long)g.nx * g.ny * g.nz;
+    const long t = (long)blockIdx.x * 256 + threadIdx.x;
+    double term[4] = {0.0, 0.0, 0.0, 0.0};
+    if (t < n) {
+        const double r = rho[t];
+        if (r > 0.0) {
+            int i, j, k;
+            cell_coord(g, t, i, j, k);
+            const double dV = g.dx * g.dy * g.dz;
+            const double x = g.xmin + (i + 0.5) * g.dx;
+            const double y = g.ymin + (j + 0.5) * g.dy;
+            const double z = g.zmin + (k + 0.5) * g.dz;
+            const double dm = r * dV;
+            term[0] = dm; term[1] = dm * x; term[2] = dm * y; term[3] = dm * z;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) block_dd_store(DD{term[q], 0.0}, q, part, gridDim.x, s);
+    __syncthreads();
+    if (threadIdx.x < 4) {
+        const int q = threadIdx.x;
+        const DD v = dd_add(dd_add(s[q][0], s[q][1]), dd_add(s[q][2], s[q][3]));
+        part[2 * ((long)q * gridDim.x + blockIdx.x)] = v.hi;
+        part[2 * ((long)q * gridDim.x + blockIdx.x) + 1] = v.lo;
+    }
+}
+
+// pass 2: Q_ij (gravity_solver_3d.c:226-258) and A_lm, B_lm for l >= 3
+__global__ void __launch_bounds__(256) moments_pass2_kernel(GridDims g, int lmax,
+                                                            const double* __restrict__ rho,
+                                                            const double* __restrict__ mom,
+                                                            double* __restrict__ part) {
+    __shared__ DD s[9 + 2 * 42][4];
+    const long n = (long)g.nx * g.ny * g.nz;
+    const long t = (long)blockIdx.x * 256 + threadIdx.x;
+    const int nq = 9 + [&] { int c = 0; for (int l = 3; l <= lmax; ++l) c += 2 * (l + 1); return c; }();
+    const double r = (t < n) ? rho[t] : 0.0;
+    const bool in = (t < n) && (r > 0.0);
+    double x = 0.0, y = 0.0, z = 0.0, dm = 0.0, r2 = 0.0;
+    if (in) {
+        int i, j, k;
+        cell_coord(g, t, i, j, k);
+        const double dV = g.dx * g.dy * g.dz;
+        x = g.xmin + (i + 0.5) * g.dx - mom[1];
+        y = g.ymin + (j + 0.5) * g.dy - mom[2];
+        z = g.zmin + (k + 0.5) * g.dz - mom[3];
+        dm = r * dV;
+        r2 = x * x + y * y + z * z;
+    }
+    {
+        double term[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+        if (in) {
+            term[0] = dm * (3.0 * x * x - r2);
+            term[1] = dm * (3.0 * x * y);
+            term[2] = dm * (3.0 * x * z);
+            term[3] = dm * (3.0 * y * x);
+            term[4] = dm * (3.0 * y * y - r2);
+            term[5] = dm * (3.0 * y * z);
+            term[6] = dm * (3.0 * z * x);
+            term[7] = dm * (3.0 * z * y);
+            term[8] = dm * (3.0 * z * z - r2);
+        }
+#pragma unroll
+        for (int q = 0; q < 9; ++q) block_dd_store(DD{term[q], 0.0}, q, part, gridDim.x, s);
+    }
+    if (lmax >= 3) {
+        // walk the recurrence per m; emit each (l, m) with l >= 3 at its slot
+        double cr = 1.0, ci = 0.0;
+        for (int m = 0; m <= lmax; ++m) {
+            if (m > 0) {
+                const double f = (double)(2 * m - 1);
+                const double nr = f * (x * cr - y * ci);
+                const double ni = f * (x * ci + y * cr);
+                cr = nr;
+                ci = ni;
+            }
+            double c2 = cr, s2 = ci;  // R_{l-2}, R_{l-1} as the walk advances
+            double c1 = 0.0, s1 = 0.0;
+            for (int l = m; l <= lmax; ++l) {
+                double cl, sl;
+                if (l == m) {
+                    cl = cr; sl = ci;
+                } else if (l == m + 1) {
+                    const double f = (double)(2 * m + 1);
+                    cl = f * z * cr; sl = f * z * ci;
+                } else {
+                    const double a = (double)(2 * l - 1), b = (double)(l + m - 1), d = (double)(l - m);
+                    cl = (a * z * c1 - b * r2 * c2) / d;
+                    sl = (a * z * s1 - b * r2 * s2) / d;
+                }
+                if (l == m) {
+                    c1 = cl; s1 = sl;
+                } else {
+                    c2 = c1; s2 = s1; c1 = cl; s1 = sl;
+                }
+                if (l >= 3) {
+                    int q = 9;
+                    for (int ll = 3; ll < l; ++ll) q += 2 * (ll + 1);
+                    q += 2 * m;
+                    block_dd_store(DD{in ? dm * cl : 0.0, 0.0}, q, part, gridDim.x, s);
+                    block_dd_store(DD{in ? dm * sl : 0.0, 0.0}, q + 1, part, gridDim.x, s);
+                }
+            }
+        }
+    }
+    __syncthreads();
+    for (int q = threadIdx.x; q < nq; q += 256) {
+        const DD v = dd_add(dd_add(s[q][0], s[q][1]), dd_add(s[q][2], s[q][3]));
+        part[2 * ((long)q * gridDim.x + blockIdx.x)] = v.hi;
+        part[2 * ((long)q * gridDim.x + blockIdx.x) + 1] = v.lo;
+    }
+}
+
+// Fold 256 consecutive partials of quantity q (zero padded) pairwise.
+__global__ void __launch_bounds__(256) moments_reduce_kernel(const double* __restrict__ in,
+                                                             long nin, double* __restrict__ out,
+                                                             long nout) {
+    __shared__ DD s[256];
+    const int q = blockIdx.y;
+    const long idx = (long)blockIdx.x * 256 + threadIdx.x;
+    DD v{0.0, 0.0};
+    if (idx < nin) {
+        v.hi = in[2 * ((long)q * nin + idx)];
+        v.lo = in[2 * ((long)q * nin + idx) + 1];
+    }
+    s[threadIdx.x] = v;
+    __syncthreads();
+    for (int h = 128; h > 0; h >>= 1) {
+        DD u{0.0, 0.0};
+        if (threadIdx.x < h) u = dd_add(s[2 * threadIdx.x], s[2 * threadIdx.x + 1]);
+        __syncthreads();
+        if (threadIdx.x < h) s[threadIdx.x] = u;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[2 * ((long)q * nout + blockIdx.x)] = s[0].hi;
+        out[2 * ((long)q * nout + blockIdx.x) + 1] = s[0].lo;
+    }
+}
+
+// Round the sums to double; pass 1 also forms the centre of mass
+// (gravity_solver_3d.c:200-209).
+__global__ void moments_finish_kernel(const double* __restrict__ sums, int nq, int pass,
+                                      GridDims g, double* __restrict__ mom) {
+    if (threadIdx.x != 0) return;
+    if (pass == 1) {
+        const double M = sums[0];
+        mom[0] = M;
+        if (M > 0.0) {
+            mom[1] = sums[2] / M;
+            mom[2] = sums[4] / M;
+            mom[3] = sums[6] / M;
+        } else {
+            mom[1] = 0.5 * (g.xmin + g.xmax);
+            mom[2] = 0.5 * (g.ymin + g.ymax);
+            mom[3] = 0.5 * (g.zmin + g.zmax);
+        }
+    } else {
+        for (int q = 0; q < nq; ++q) mom[4 + q] = sums[2 * q];
+    }
+}
+
+// ---------------------------------------------------- defect correction --
+__global__ void __launch_bounds__(256) residual_kernel(GridDims g, SorCoef c,
+                                                       const double* __restrict__ pr,
+                                                       const double* __restrict__ pb,
+                                                       const double* __restrict__ fr,
+                                                       const double* __restrict__ fb,
+                                                       float* __restrict__ rr,
+                                                       float* __restrict__ rbk,
+                                                       double* __restrict__ acc) {
+    const long n = (long)g.nx * g.ny * g.nzh * 2;
+    double s_rr = 0.0, s_ff = 0.0;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (long)gridDim.x * blockDim.x) {
+        const int colour = (int)(t & 1);
+        const long e = t >> 1;
+        const int m = (int)(e % g.nzh);
+        const long row = e / g.nzh;
+        const int j = (int)(row % g.ny), i = (int)(row / g.ny);
+        const int k = 2 * m + ((i + j + colour) & 1);
+        if (k >= g.nz) continue;
+        const double* mine = colour ? pb : pr;
+        const double* other = colour ? pr : pb;
+        float* rout = colour ? rbk : rr;
+        const long base = ((long)i * g.ny + j) * g.nzh;
+        if (i < 1 || i > g.nx - 2 || j < 1 || j > g.ny - 2 || k < 1 || k > g.nz - 2) {
+            if (rout) rout[base + m] = 0.0f;
+            continue;
+        }
+        const long sxh = (long)g.ny * g.nzh;
+        const double lap = (other[base - sxh + m] + other[base + sxh + m]) * c.dx2i +
+                           (other[base - g.nzh + m] + other[base + g.nzh + m]) * c.dy2i +
+                           (other[base + ((k - 1) >> 1)] + other[base + ((k + 1) >> 1)]) * c.dz2i -
+                           c.factor * mine[base + m];
+        const double f = (colour ? fb : fr)[base + m];
+        const double r = f - lap;
+        if (rout) rout[base + m] = (float)r;
+        s_rr += r * r;
+        s_ff += f * f;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        s_rr += __shfl_xor(s_rr, o, 64);
+        s_ff += __shfl_xor(s_ff, o, 64);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&acc[0], s_rr);
+        atomicAdd(&acc[1], s_ff);
+    }
+}
+
+__global__ void __launch_bounds__(256) sor_sweep_f32_kernel(GridDims g, float dx2i, float dy2i,
+                                                            float dz2i, float factor, float omega,
+                                                            int colour, float* __restrict__ mine,
+                                                            const float* __restrict__ other,
+                                                            const float* __restrict__ rhs) {
+    const long nrow = (long)(g.nx - 2) * (g.ny - 2);
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nrow * g.nzh) return;
+    const int m = (int)(t % g.nzh);
+    const long row = t / g.nzh;
+    const int j = 1 + (int)(row % (g.ny - 2));
+    const int i = 1 + (int)(row / (g.ny - 2));
+    const int k = 2 * m + ((i + j + colour) & 1);
+    if (k < 1 || k > g.nz - 2) return;
+    const long base = ((long)i * g.ny + j) * g.nzh;
+    const long sxh = (long)g.ny * g.nzh;
+    const float lap = (other[base - sxh + m] + other[base + sxh + m]) * dx2i +
+                      (other[base - g.nzh + m] + other[base + g.nzh + m]) * dy2i +
+                      (other[base + ((k - 1) >> 1)] + other[base + ((k + 1) >> 1)]) * dz2i;
+    const float old = mine[base + m];
+    const float gs = (lap - rhs[base + m]) / factor;
+    mine[base + m] = old + omega * (gs - old);
+}
+
+__global__ void correct_kernel(GridDims g, double* __restrict__ pr, double* __restrict__ pb,
+                               float* __restrict__ er, float* __restrict__ eb) {
+    const long n = (long)g.nx * g.ny * g.nzh;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (long)gridDim.x * blockDim.x) {
+        pr[t] = pr[t] + (double)er[t];
+        pb[t] = pb[t] + (double)eb[t];
+        er[t] = 0.0f;
+        eb[t] = 0.0f;
+    }
+}
+
+// ----------------------------------------------------------- launchers --
+static unsigned grid_for(long n, long cap = 16384) {
+    long nb = (n + 255) / 256;
+    if (nb > cap) nb = cap;
+    if (nb < 1) nb = 1;
+    return (unsigned)nb;
+}
+
+long moments_scratch_len(long ncells, int lmax) {
+    const long nb = (ncells + 255) / 256;
+    const long nq = mp_n_pass2(lmax) > 4 ? mp_n_pass2(lmax) : 4;
+    const long nb2 = (nb + 255) / 256;
+    return 2 * nq * (nb + nb2 + 2) + 2 * nq;
+}
+
+// Reduce `nb` partial double-doubles per quantity down to one, in place of
+// the scratch area; returns the pointer holding nq final pairs.
+static double* reduce_partials(double* part, long nb, int nq, hipStream_t st) {
+    double* cur = part;
+    double* nxt = part + 2L * nq * nb;
+    long n = nb;
+    while (n > 1) {
+        const long nout = (n + 255) / 256;
+        hipLaunchKernelGGL(moments_reduce_kernel, dim3((unsigned)nout, nq), dim3(256), 0, st, cur,
+                           n, nxt, nout);
+        double* tmp = cur;
+        cur = nxt;
+        nxt = tmp;
+        n = nout;
+    }
+    return cur;
+}
+
+hipError_t launch_moments(const GridDims& g, const double* rho, int lmax, double* scratch,
+                          double* mom_out, hipStream_t st) {
+    const long n = (long)g.nx * g.ny * g.nz;
+    const long nb = (n + 255) / 256;
+    hipLaunchKernelGGL(moments_pass1_kernel, dim3((unsigned)nb), dim3(256), 0, st, g, rho, scratch);
+    double* s1 = reduce_partials(scratch, nb, 4, st);
+    hipLaunchKernelGGL(moments_finish_kernel, dim3(1), dim3(64), 0, st, s1, 4, 1, g, mom_out);
+    const int nq = mp_n_pass2(lmax);
+    hipLaunchKernelGGL(moments_pass2_kernel, dim3((unsigned)nb), dim3(256), 0, st, g, lmax, rho,
+                       mom_out, scratch);
+    double* s2 = reduce_partials(scratch, nb, nq, st);
+    hipLaunchKernelGGL(moments_finish_kernel, dim3(1), dim3(64), 0, st, s2, nq, 2, g, mom_out);
+    return hipGetLastError();
+}
+
+hipError_t launch_natural_to_split(const GridDims& g, const double* nat, double scale,
+                                   double* red, double* black, hipStream_t st) {
+    const long n = (long)g.nx * g.ny * g.nz;
+    hipLaunchKernelGGL(natural_to_split_kernel, dim3(grid_for(n)), dim3(256), 0, st, g, nat, scale,
+                       red, black);
+    return hipGetLastError();
+}
+
+hipError_t launch_split_to_natural(const GridDims& g, const double* red, const double* black,
+                                   double* nat, hipStream_t st) {
+    const long n = (long)g.nx * g.ny * g.nz;
+    hipLaunchKernelGGL(split_to_natural_kernel, dim3(grid_for(n)), dim3(256), 0, st, g, red, black,
+                       nat);
+    return hipGetLastError();
+}
+
+hipError_t launch_sor_sweep(const GridDims& g, const SorCoef& c, int colour, int it, double tol,
+                            double* phi_r, double* phi_b, const double* rhs_r,
+                            const double* rhs_b, SorCtl* ctl, hipStream_t st) {
+    const long n = (long)(g.nx - 2) * (g.ny - 2) * g.nzh;
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    double* mine = colour ? phi_b : phi_r;
+    const double* other = colour ? phi_r : phi_b;
+    const double* rhs = colour ? rhs_b : rhs_r;
+    hipLaunchKernelGGL(sor_sweep_kernel, dim3(nb), dim3(256), 0, st, g, c, colour, it, tol, mine,
+                       other, rhs, ctl);
+    return hipGetLastError();
+}
+
+hipError_t launch_sor_boundary(const GridDims& g, int bc, double G, int lmax, const double* mom,
+                               const double* wlm, double* phi_r, double* phi_b, hipStream_t st) {
+    if (bc == 2) return hipSuccess;
+    const long tot = 2L * g.ny * g.nz + 2L * (g.nx - 2) * g.nz + 2L * (g.nx - 2) * (g.ny - 2);
+    hipLaunchKernelGGL(sor_boundary_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
+                       g, bc, G, lmax, mom, wlm, phi_r, phi_b);
+    return hipGetLastError();
+}
+
+hipError_t launch_sor_reset(SorCtl* ctl, hipStream_t st) {
+    hipLaunchKernelGGL(sor_reset_kernel, dim3(1), dim3(1), 0, st, ctl);
+    return hipGetLastError();
+}
+
+hipError_t launch_acceleration(const GridDims& g, const double* phi_r, const double* phi_b,
+                               double* gx, double* gy, double* gz, hipStream_t st) {
+    const long n = (long)g.nx * g.ny * g.nz;
+    hipLaunchKernelGGL(acceleration_kernel, dim3(grid_for(n)), dim3(256), 0, st, g, phi_r, phi_b,
+                       gx, gy, gz);
+    return hipGetLastError();
+}
+
+hipError_t launch_residual(const GridDims& g, const SorCoef& c, const double* phi_r,
+                           const double* phi_b, const double* rhs_r, const double* rhs_b,
+                           float* r_r, float* r_b, double* acc, hipStream_t st) {
+    const long n = (long)g.nx * g.ny * g.nzh * 2;
+    hipLaunchKernelGGL(residual_kernel, dim3(grid_for(n, 8192)), dim3(256), 0, st, g, c, phi_r,
+                       phi_b, rhs_r, rhs_b, r_r, r_b, acc);
+    return hipGetLastError();
+}
+
+hipError_t launch_sor_sweep_f32(const GridDims& g, const SorCoef& c, int colour, float* e_r,
+                                float* e_b, const float* r_r, const float* r_b, hipStream_t st) {
+    const long n = (long)(g.nx - 2) * (g.ny - 2) * g.nzh;
+    const unsigned nb = (unsigned)((n + 255) / 256);
+    float* mine = colour ? e_b : e_r;
+    const float* other = colour ? e_r : e_b;
+    const float* rhs = colour ? r_b : r_r;
+    hipLaunchKernelGGL(sor_sweep_f32_kernel, dim3(nb), dim3(256), 0, st, g, (float)c.dx2i,
+                       (float)c.dy2i, (float)c.dz2i, (float)c.factor, (float)c.omega, colour, mine,
+                       other, rhs);
+    return hipGetLastError();
+}
+
+hipError_t launch_correct(const GridDims& g, double* phi_r, double* phi_b, float* e_r,
+                          float* e_b, hipStream_t st) {
+    const long n = (long)g.nx * g.ny * g.nzh;
+    hipLaunchKernelGGL(correct_kernel, dim3(grid_for(n)), dim3(256), 0, st, g, phi_r, phi_b, e_r,
+                       e_b);
+    return hipGetLastError();
+}
+
+}  // namespace hyd

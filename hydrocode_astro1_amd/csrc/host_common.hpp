@@ -1,0 +1,79 @@
+// host_common.hpp -- host-side helpers shared by the C-ABI implementation:
+// HIP error mapping (report on stderr, return an error code, never abort)
+// and the device contexts hung off the public structs.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdio.h>
+
+#include "gravity_kernels.hpp"
+#include "hydro_kernels.hpp"
+
+#define HYD_CHECK(expr)                                                                  \
+    do {                                                                                 \
+        hipError_t e_ = (expr);                                                          \
+        if (e_ != hipSuccess) {                                                          \
+            fprintf(stderr, "hydro_mi355x: %s failed at %s:%d: %s\n", #expr, __FILE__,   \
+                    __LINE__, hipGetErrorString(e_));                                    \
+            return -2;                                                                   \
+        }                                                                                \
+    } while (0)
+
+namespace hyd {
+
+struct Timer {
+    hipEvent_t a = nullptr, b = nullptr;
+};
+
+}  // namespace hyd
+
+// Device context of a GravitySolver3D.
+struct GravDevice {
+    hyd::GridDims g;
+    long n = 0;       // cells
+    long nsplit = 0;  // entries per colour array
+    double *phi_r = nullptr, *phi_b = nullptr;
+    double *rhs_r = nullptr, *rhs_b = nullptr;
+    double *gx = nullptr, *gy = nullptr, *gz = nullptr;
+    double *rho_tmp = nullptr;  // device copy of a host density
+    double *nat_tmp = nullptr;  // natural-layout staging
+    double *mom = nullptr;      // moment results
+    double *mom_scratch = nullptr;
+    long mom_scratch_len = 0;
+    double *wlm = nullptr;      // multipole weights
+    hyd::SorCtl *ctl = nullptr;
+    hyd::SorCtl *ctl_host = nullptr;  // pinned readback
+    double *acc = nullptr;            // residual sums (device)
+    double *acc_host = nullptr;       // pinned readback
+    float *e_r = nullptr, *e_b = nullptr, *r_r = nullptr, *r_b = nullptr;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    bool g_valid = false;  // device g has been computed at least once
+};
+
+// Device context of a GodunovHLLC3D.
+struct HydroDevice {
+    int nx = 0, ny = 0, nz = 0;
+    long n = 0;
+    double *U[2][5] = {{nullptr}};
+    int cur = 0;
+    double *face = nullptr;
+    unsigned long long *ms = nullptr;  // [3] signal-speed slots
+    double *dt_log = nullptr;          // [DT_LOG] dt per step (ring)
+    double *g_zero = nullptr;          // zero g planes when gravity is off
+    double *aos = nullptr;             // staging for AoS transfers
+    long step = 0;                     // steps since the last upload
+    int cx = 32;                       // x planes per block of the step kernel
+    bool uploaded = false;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    double last_hydro_ms = 0.0;
+    hipEvent_t record_ev[3] = {nullptr, nullptr, nullptr};  // benchmark brackets
+    bool skip_gravity = false;  // gravity already solved by the caller
+    static constexpr int DT_LOG = 1 << 16;
+};
+
+// Internal interface between the hydro driver and the gravity solver.
+struct GravitySolver3D;
+int grav_internal_solve(struct GravitySolver3D *s, const double *d_rho, hipStream_t st);
+const double *grav_internal_g(struct GravitySolver3D *s, int comp);

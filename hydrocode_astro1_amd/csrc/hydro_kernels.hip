@@ -107,14 +107,21 @@ __device__ __forceinline__ long face_index(int nx, int ny, int nz, int i, int j,
 }
 
 // LDS coordinates: cell (row, col) with row in -1..8, col in -1..64 lives at
-// [row+1][col+1].  s_fy[r+1] holds F_y(r, r+1) for r = -1..7.
+// [row+1][col+1].  s_fy[r+1] holds F_y(r, r+1) for r = -1..7.  s_fzl[p][r]
+// holds F_z(k0-1, k0) of row r for the plane of parity p; wave 0 forms it one
+// plane ahead, right after that plane lands in LDS.
+//
+// The three roles (cell rows, the low halo row, the high halo row) run
+// separate loops with the same barrier sequence (B1, B2, B3 per plane), so
+// each role's registers are allocated for that role alone.
 __global__ void __launch_bounds__(NT, 2) hydro_step_kernel(HydroStepArgs a) {
     __shared__ double s_cell[NF][LR][LC];
     __shared__ double s_fy[5][TJ + 1][TK];
-    __shared__ double s_fzl[5][TJ];  // F_z(k0-1, k0) for rows 0..7
+    __shared__ double s_fzl[2][5][TJ];
 
     const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const int wave = threadIdx.x >> 6;
+    // wavefront index made scalar so role branches are uniform jumps
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
 
     // Block -> tile.  Consecutive blocks land on different XCDs; regroup so
@@ -134,7 +141,6 @@ __global__ void __launch_bounds__(NT, 2) hydro_step_kernel(HydroStepArgs a) {
     const int j = j0 + row;
     const int k = k0 + lane;
     const bool valid = (j >= 0) && (j < ny) && (k < nz);
-    const bool compute = (wave >= 1) && (wave <= TJ);
 
     // z halo cells: wave 0 lanes 0..7 load (j0+lane, k0-1), wave NW-1 lanes
     // 0..7 load (j0+lane, k0+64).
@@ -153,63 +159,60 @@ __global__ void __launch_bounds__(NT, 2) hydro_step_kernel(HydroStepArgs a) {
         *a.dt_out = dt;
         *a.ms_clear = 0ull;
     }
-    const double ax = dt / a.dx, ay = dt / a.dy, az = dt / a.dz;
     const double gamma = a.gamma;
 
-    double fxl[5] = {0, 0, 0, 0, 0};
-    unsigned long long smax = 0ull;
-
-    {  // prologue: plane ib into LDS, F_x(ib-1) when plane ib is updated
+    // prologue: plane ib into LDS
+    {
         Raw r;
         load_raw(a, cidx, valid, r);
-        const Cell c = to_cell(r, gamma);
-        lds_put(s_cell, row + 1, lane + 1, c);
+        lds_put(s_cell, row + 1, lane + 1, to_cell(r, gamma));
         if (has_halo) {
             Raw h;
             load_raw(a, hidx, hvalid, h);
             lds_put(s_cell, lane + 1, hcol, to_cell(h, gamma));
         }
-        if (ib > 0 && compute) {
+    }
+    __syncthreads();
+
+    if (wave >= 1 && wave <= TJ) {
+        // ---------------------------------------------------- cell rows
+        const double ax = dt / a.dx, ay = dt / a.dy, az = dt / a.dz;
+        double fxl[5] = {0, 0, 0, 0, 0};
+        unsigned long long smax = 0ull;
+        if (ib > 0) {  // F_x(ib-1) for the update of plane ib
             Raw pr;
             load_raw(a, cidx - sx, valid, pr);
-            hllc<0>(to_cell(pr, gamma), c, gamma, fxl);
+            hllc<0>(to_cell(pr, gamma), lds_get(s_cell, row + 1, lane + 1), gamma, fxl);
         }
-    }
+        for (int i = ib; i < ie; ++i) {
+            const long off = (long)(i - ib) * sx;
+            const bool has_next = (i + 1 < nx);
+            __syncthreads();  // B1: plane i complete in LDS
 
-    for (int i = ib; i < ie; ++i) {
-        const long off = (long)(i - ib) * sx;
-        const bool has_next = (i + 1 < nx);
-        Raw rn, rh;
-        if (has_next) {  // prefetch plane i+1
-            load_raw(a, cidx + off + sx, valid, rn);
-            if (has_halo) load_raw(a, hidx + off + sx, hvalid, rh);
-        }
-        __syncthreads();  // B1: plane i complete in LDS
-
-        const Cell cur = lds_get(s_cell, row + 1, lane + 1);
-        if (wave <= TJ) {  // rows -1..7: F_y(row, row+1)
-            double fy[5];
-            hllc<1>(cur, lds_get(s_cell, row + 2, lane + 1), gamma, fy);
+            const Cell cur = lds_get(s_cell, row + 1, lane + 1);
+            {
+                double fy[5];
+                hllc<1>(cur, lds_get(s_cell, row + 2, lane + 1), gamma, fy);
 #pragma unroll
-            for (int f = 0; f < 5; ++f) s_fy[f][row + 1][lane] = fy[f];
-        }
-        double fz[5] = {0, 0, 0, 0, 0};
-        if (compute) hllc<2>(cur, lds_get(s_cell, row + 1, lane + 2), gamma, fz);
-        if (wave == 0 && lane < TJ) {  // F_z(k0-1, k0) of row `lane`
-            double fh[5];
-            hllc<2>(lds_get(s_cell, lane + 1, 0), lds_get(s_cell, lane + 1, 1), gamma, fh);
+                for (int f = 0; f < 5; ++f) s_fy[f][row + 1][lane] = fy[f];
+            }
+            double dz[5];
+            {
+                double fz[5];
+                hllc<2>(cur, lds_get(s_cell, row + 1, lane + 2), gamma, fz);
 #pragma unroll
-            for (int f = 0; f < 5; ++f) s_fzl[f][lane] = fh[f];
-        }
-        double fzl[5];
-#pragma unroll
-        for (int f = 0; f < 5; ++f) fzl[f] = __shfl_up(fz[f], 1, 64);
-
-        double tx[5] = {0, 0, 0, 0, 0};
-        Cell nxt;
-        if (has_next) {
-            nxt = to_cell(rn, gamma);
-            if (compute) {
+                for (int f = 0; f < 5; ++f) {
+                    double l = __shfl_up(fz[f], 1, 64);
+                    if (lane == 0) l = s_fzl[i & 1][f][row];
+                    dz[f] = fz[f] - l;
+                }
+            }
+            double tx[5] = {0, 0, 0, 0, 0};
+            Cell nxt;
+            if (has_next) {
+                Raw rn;
+                load_raw(a, cidx + off + sx, valid, rn);
+                nxt = to_cell(rn, gamma);
                 double fxr[5];
                 hllc<0>(cur, nxt, gamma, fxr);
 #pragma unroll
@@ -218,59 +221,108 @@ __global__ void __launch_bounds__(NT, 2) hydro_step_kernel(HydroStepArgs a) {
                     fxl[f] = fxr[f];
                 }
             }
-        }
-        __syncthreads();  // B2: all reads of plane i done
+            __syncthreads();  // B2: all reads of plane i done
+            if (has_next) lds_put(s_cell, row + 1, lane + 1, nxt);
+            __syncthreads();  // B3: F_y of plane i visible
 
-        if (has_next) {
-            lds_put(s_cell, row + 1, lane + 1, nxt);
-            if (has_halo) lds_put(s_cell, lane + 1, hcol, to_cell(rh, gamma));
-        }
-        __syncthreads();  // B3: fluxes visible
-
-        if (compute && valid) {
-            if (lane == 0) {
+            if (valid) {
+                double q[5] = {cur.d, cur.mx, cur.my, cur.mz, cur.e};
+                const bool interior = (i >= 1) && (i <= nx - 2) && (j >= 1) && (j <= ny - 2) &&
+                                      (k >= 1) && (k <= nz - 2);
+                if (interior) {
+                    // hydro_solver_3d.c:738-762: x, then y, then z, left to right
 #pragma unroll
-                for (int f = 0; f < 5; ++f) fzl[f] = s_fzl[f][row];
-            }
-            double q[5] = {cur.d, cur.mx, cur.my, cur.mz, cur.e};
-            const bool interior = (i >= 1) && (i <= nx - 2) && (j >= 1) && (j <= ny - 2) &&
-                                  (k >= 1) && (k <= nz - 2);
-            if (interior) {
-                // hydro_solver_3d.c:738-762: x, then y, then z, left to right
-#pragma unroll
-                for (int f = 0; f < 5; ++f) {
-                    double t = tx[f] - ay * (s_fy[f][row + 1][lane] - s_fy[f][row][lane]);
-                    t = t - az * (fz[f] - fzl[f]);
-                    q[f] = q[f] + t;
+                    for (int f = 0; f < 5; ++f) {
+                        double t = tx[f] - ay * (s_fy[f][row + 1][lane] - s_fy[f][row][lane]);
+                        t = t - az * dz[f];
+                        q[f] = q[f] + t;
+                    }
+                    floor_roundtrip(q[0], q[1], q[2], q[3], q[4], gamma);
                 }
-                floor_roundtrip(q[0], q[1], q[2], q[3], q[4], gamma);
-            }
-            const long idx = cidx + off;
-            if (a.gravity) {
-                // apply_gravity_source_terms, hydro_solver_3d.c:606-627
-                const double gx = a.g[0][idx], gy = a.g[1][idx], gz = a.g[2][idx];
-                const double rho = q[0];
-                const double u = q[1] / rho, v = q[2] / rho, w = q[3] / rho;
-                q[1] = q[1] + dt * rho * gx;
-                q[2] = q[2] + dt * rho * gy;
-                q[3] = q[3] + dt * rho * gz;
-                const double work = rho * (u * gx + v * gy + w * gz);
-                q[4] = q[4] + dt * work;
-            }
-            if (interior) {
+                const long idx = cidx + off;
+                if (a.gravity) {
+                    // apply_gravity_source_terms, hydro_solver_3d.c:606-627
+                    const double gx = a.g[0][idx], gy = a.g[1][idx], gz = a.g[2][idx];
+                    const double rho = q[0];
+                    const double u = q[1] / rho, v = q[2] / rho, w = q[3] / rho;
+                    q[1] = q[1] + dt * rho * gx;
+                    q[2] = q[2] + dt * rho * gy;
+                    q[3] = q[3] + dt * rho * gz;
+                    const double work = rho * (u * gx + v * gy + w * gz);
+                    q[4] = q[4] + dt * work;
+                }
+                if (interior) {
 #pragma unroll
-                for (int f = 0; f < 5; ++f) a.uout[f][idx] = q[f];
-                unsigned long long sb = dbits(signal_speed(q[0], q[1], q[2], q[3], q[4], gamma));
-                smax = sb > smax ? sb : smax;
-            } else {
+                    for (int f = 0; f < 5; ++f) a.uout[f][idx] = q[f];
+                    unsigned long long sb =
+                        dbits(signal_speed(q[0], q[1], q[2], q[3], q[4], gamma));
+                    smax = sb > smax ? sb : smax;
+                } else {
 #pragma unroll
-                for (int f = 0; f < 5; ++f) a.face[face_index(nx, ny, nz, i, j, k, f)] = q[f];
+                    for (int f = 0; f < 5; ++f) a.face[face_index(nx, ny, nz, i, j, k, f)] = q[f];
+                }
             }
+        }
+        smax = wave_max_u64(smax);
+        if (lane == 0 && smax != 0ull) atomicMax(a.ms_out, smax);
+    } else if (wave == 0) {
+        // ------------------------------------------- low halo row (j0-1)
+        // F_y(j0-1, j0) each plane; loads the left z halo and forms
+        // F_z(k0-1, k0) of every cell row one plane ahead.
+        if (lane < TJ) {
+            double fh[5];
+            hllc<2>(lds_get(s_cell, lane + 1, 0), lds_get(s_cell, lane + 1, 1), gamma, fh);
+#pragma unroll
+            for (int f = 0; f < 5; ++f) s_fzl[ib & 1][f][lane] = fh[f];
+        }
+        for (int i = ib; i < ie; ++i) {
+            const long off = (long)(i - ib) * sx;
+            const bool has_next = (i + 1 < nx);
+            __syncthreads();  // B1
+            {
+                double fy[5];
+                hllc<1>(lds_get(s_cell, 0, lane + 1), lds_get(s_cell, 1, lane + 1), gamma, fy);
+#pragma unroll
+                for (int f = 0; f < 5; ++f) s_fy[f][0][lane] = fy[f];
+            }
+            Raw rn, rh;
+            if (has_next) {
+                load_raw(a, cidx + off + sx, valid, rn);
+                if (has_halo) load_raw(a, hidx + off + sx, hvalid, rh);
+            }
+            __syncthreads();  // B2
+            if (has_next) {
+                lds_put(s_cell, 0, lane + 1, to_cell(rn, gamma));
+                if (has_halo) lds_put(s_cell, lane + 1, 0, to_cell(rh, gamma));
+            }
+            __syncthreads();  // B3
+            if (has_next && lane < TJ) {
+                double fh[5];
+                hllc<2>(lds_get(s_cell, lane + 1, 0), lds_get(s_cell, lane + 1, 1), gamma, fh);
+#pragma unroll
+                for (int f = 0; f < 5; ++f) s_fzl[(i + 1) & 1][f][lane] = fh[f];
+            }
+        }
+    } else {
+        // ------------------------------------------ high halo row (j0+8)
+        // loads row j0+8 and the right z halo of the next plane.
+        for (int i = ib; i < ie; ++i) {
+            const long off = (long)(i - ib) * sx;
+            const bool has_next = (i + 1 < nx);
+            __syncthreads();  // B1
+            Raw rn, rh;
+            if (has_next) {
+                load_raw(a, cidx + off + sx, valid, rn);
+                if (has_halo) load_raw(a, hidx + off + sx, hvalid, rh);
+            }
+            __syncthreads();  // B2
+            if (has_next) {
+                lds_put(s_cell, LR - 1, lane + 1, to_cell(rn, gamma));
+                if (has_halo) lds_put(s_cell, lane + 1, LC - 1, to_cell(rh, gamma));
+            }
+            __syncthreads();  // B3
         }
     }
-
-    smax = wave_max_u64(smax);
-    if (lane == 0 && smax != 0ull) atomicMax(a.ms_out, smax);
 }
 
 // ---------------------------------------------------------------------------

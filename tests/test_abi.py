@@ -1,0 +1,221 @@
+"""The C ABI of libhydro_mi355x.so, checked on the CPU (no device calls):
+every function the headers in include/ declare is exported, the public
+structs keep the reference's field offsets, and the input-deck parser reads
+a deck exactly as the reference parser does."""
+import ctypes as C
+import os
+import re
+import shutil
+import subprocess
+import tempfile
+
+import numpy as np
+import pytest
+
+import hydrocode_astro1_amd as H
+import oracle_lib as OL
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+INC = os.path.join(ROOT, "include")
+REF = "/root/reference"
+
+
+def declared_functions():
+    names = set()
+    for f in os.listdir(INC):
+        if not f.endswith(".h"):
+            continue
+        text = open(os.path.join(INC, f)).read()
+        text = re.sub(r"/\*.*?\*/", "", text, flags=re.S)
+        for m in re.finditer(r"^[A-Za-z_][\w\s\*]*?\b([a-z_][a-z0-9_]*)\s*\(", text, re.M):
+            if not text[m.start():m.end()].lstrip().startswith(("#", "typedef", "return")):
+                names.add(m.group(1))
+    return names
+
+
+def test_library_exports_every_declared_function():
+    lib = H.lib()
+    names = declared_functions()
+    assert len(names) >= 40
+    missing = [n for n in sorted(names) if not hasattr(lib, n)]
+    assert not missing, missing
+    assert {n for n, _, _ in H.EXPORTS} >= names - {"IDX3D", "IDX3D_GRAV"}
+
+
+def test_no_device_on_this_host_is_reported_not_faked():
+    assert H.device_count() >= 0  # returns 0 here; never raises
+
+
+OFFSET_FIELDS = {
+    "GodunovHLLC3D": ["nx", "ny", "nz", "xmin", "xmax", "ymin", "ymax", "zmin", "zmax", "dx",
+                      "dy", "dz", "dt", "cfl", "bc_xmin", "bc_xmax", "bc_ymin", "bc_ymax",
+                      "bc_zmin", "bc_zmax", "problem", "gamma", "U", "U_half", "F_x", "F_y",
+                      "F_z", "gx", "gy", "gz", "gravity_solver", "enable_gravity", "G_const"],
+    "GravitySolver3D": ["nx", "ny", "nz", "dx", "dy", "dz", "xmin", "xmax", "ymin", "ymax",
+                        "zmin", "zmax", "G", "phi", "rho", "gx", "gy", "gz", "omega",
+                        "max_iterations", "tolerance", "bc_type", "last_iterations",
+                        "last_residual"],
+    "InputDeck": ["nx", "xmin", "t_end", "output_dt", "cfl", "gamma", "G_const", "problem",
+                  "bc_xmin", "bc_zmax", "G_M_PNS", "L_heat", "output_dir", "restart",
+                  "restart_file", "restart_step", "restart_time", "star1_profile",
+                  "star2_profile", "star1_x", "star2_vz", "binary_separation",
+                  "binary_auto_orbit", "stellar_profile", "explosion_energy",
+                  "explosion_radius", "explosion_type", "enable_perturbations",
+                  "perturbation_seed", "perturb_velocity_amplitude", "perturb_velocity_modes",
+                  "perturb_density_amplitude", "perturb_density_modes",
+                  "enable_sph_harmonic_mode", "sph_harmonic_l", "sph_harmonic_amp_vel",
+                  "enable_torus", "torus_major_radius", "torus_rotation_velocity"],
+    "State3D": ["rho", "rhou", "rhov", "rhow", "E"],
+}
+
+
+def _offsets(incdir, tmp, tag):
+    src = os.path.join(tmp, f"off_{tag}.c")
+    with open(src, "w") as f:
+        f.write("#include <stdio.h>\n#include <stddef.h>\n#include \"hydro_solver_3d.h\"\n")
+        f.write("int main(void){\n")
+        for st, fields in OFFSET_FIELDS.items():
+            for fld in fields:
+                f.write(f'printf("{st}.{fld} %zu\\n", offsetof({st}, {fld}));\n')
+        f.write('printf("State3D.size %zu\\n", sizeof(State3D));\nreturn 0;}\n')
+    exe = os.path.join(tmp, f"off_{tag}")
+    subprocess.run(["gcc", "-std=c11", "-D_DEFAULT_SOURCE", "-I" + incdir, src, "-o", exe],
+                   check=True, capture_output=True)
+    out = subprocess.run([exe], check=True, capture_output=True, text=True).stdout
+    return dict(line.split() for line in out.splitlines())
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(REF, "hydro_solver_3d.h")) or
+                    shutil.which("gcc") is None, reason="reference headers not present")
+def test_struct_layout_matches_reference_headers():
+    with tempfile.TemporaryDirectory() as tmp:
+        ours = _offsets(INC, tmp, "ours")
+        ref = _offsets(REF, tmp, "ref")
+    assert ours == ref
+
+
+def test_ctypes_mirror_matches_header_offsets():
+    with tempfile.TemporaryDirectory() as tmp:
+        ours = _offsets(INC, tmp, "ours")
+    for st, cls in (("GodunovHLLC3D", H.GodunovHLLC3D), ("GravitySolver3D", H.GravitySolver3D)):
+        for fld in OFFSET_FIELDS[st]:
+            assert getattr(cls, fld).offset == int(ours[f"{st}.{fld}"]), (st, fld)
+
+
+DECK = """# synthetic deck in the reference's key = value format
+nx = 48
+ny = 40   # trailing comment
+nz = 36
+xmin = -0.5
+xmax = 0.5
+t_end = 0.25
+output_dt = 0.05
+cfl = 0.35
+gamma = 1.4
+G_const = 0.0
+problem = STELLAR_OCTANT_EXPLOSION_3D
+bc_xmin = OUTFLOW
+bc_xmax = REFLECTIVE
+bc_ymin = PERIODIC
+bc_ymax = PERIODIC
+bc_zmin = BOGUS
+restart = true
+restart_file = out/restart_000012.csv
+restart_step = 12
+restart_time = 0.125
+explosion_energy = 2.5e3
+explosion_type = kinetic
+enable_torus = 1
+torus_major_radius = 0.8
+perturbation_seed = 7
+unknown_key = 5
+"""
+
+_PREFIX_FIELDS = [
+    ("nx", C.c_int), ("ny", C.c_int), ("nz", C.c_int),
+    ("xmin", C.c_double), ("xmax", C.c_double), ("ymin", C.c_double), ("ymax", C.c_double),
+    ("zmin", C.c_double), ("zmax", C.c_double), ("t_end", C.c_double),
+    ("output_dt", C.c_double), ("cfl", C.c_double), ("gamma", C.c_double),
+    ("G_const", C.c_double), ("problem", C.c_int),
+    ("bc_xmin", C.c_int), ("bc_xmax", C.c_int), ("bc_ymin", C.c_int), ("bc_ymax", C.c_int),
+    ("bc_zmin", C.c_int), ("bc_zmax", C.c_int),
+    ("G_M_PNS", C.c_double), ("L_nu", C.c_double), ("g_strength", C.c_double),
+    ("L_heat", C.c_double), ("output_dir", C.c_char * 256), ("restart", C.c_bool),
+    ("restart_file", C.c_char * 256), ("restart_step", C.c_int), ("restart_time", C.c_double),
+    ("star1_profile", C.c_char * 256), ("star2_profile", C.c_char * 256),
+] + [(f"s{i}", C.c_double) for i in range(14)] + [
+    ("binary_auto_orbit", C.c_bool), ("stellar_profile", C.c_char * 256),
+    ("explosion_energy", C.c_double), ("explosion_radius", C.c_double),
+    ("explosion_type", C.c_char * 32), ("enable_perturbations", C.c_bool),
+    ("perturbation_seed", C.c_int), ("pva", C.c_double), ("pvkmin", C.c_double),
+    ("pvkmax", C.c_double), ("pvm", C.c_int), ("pda", C.c_double), ("pdkmin", C.c_double),
+    ("pdkmax", C.c_double), ("pdm", C.c_int), ("enable_sph_harmonic_mode", C.c_bool),
+    ("shl", C.c_int), ("shm", C.c_int), ("sha_rho", C.c_double), ("sha_vel", C.c_double),
+    ("enable_torus", C.c_bool), ("torus_major_radius", C.c_double),
+    ("torus_minor_radius", C.c_double), ("torus_density", C.c_double),
+    ("torus_pressure", C.c_double), ("torus_rotation_velocity", C.c_double),
+]
+
+
+class DeckPrefix(C.Structure):
+    _fields_ = _PREFIX_FIELDS
+
+
+def _read(lib, path):
+    lib.input_deck_create.restype = C.c_void_p
+    lib.input_deck_create.argtypes = [C.c_char_p]
+    lib.input_deck_destroy.argtypes = [C.c_void_p]
+    p = lib.input_deck_create(path.encode())
+    d = DeckPrefix.from_address(p)
+    vals = {name: getattr(d, name) for name, _ in _PREFIX_FIELDS}
+    vals = {k: (bytes(v) if isinstance(v, bytes) else v) for k, v in vals.items()}
+    lib.input_deck_destroy(p)
+    return vals
+
+
+def test_deck_parser_reads_like_reference(tmp_path):
+    path = str(tmp_path / "deck.txt")
+    with open(path, "w") as f:
+        f.write(DECK)
+    ours = _read(H.lib(), path)
+    assert ours["nx"] == 48 and ours["ny"] == 40 and ours["cfl"] == 0.35
+    assert ours["bc_xmax"] == H.BC_REFLECTIVE and ours["bc_zmin"] == H.BC_OUTFLOW
+    assert ours["bc_zmax"] == H.BC_PERIODIC  # default kept
+    assert ours["restart"] and ours["restart_step"] == 12
+    assert ours["output_dir"] == b"output"
+    ref = OL.ref_lib()
+    if ref is None:
+        pytest.skip("reference not built here")
+    theirs = _read(ref, path)
+    assert ours == theirs
+
+
+def test_deck_added_keys(tmp_path):
+    path = str(tmp_path / "deck2.txt")
+    with open(path, "w") as f:
+        f.write("nx = 32\nmax_steps = 200\nsor_tol = 1e-8\nmultipole_lmax = 6\n"
+                "poisson_mode = fp32dc\nsor_criterion = rel_resid\noutput_enable = false\n"
+                "ic_file = state.bin\n")
+    class Deck(C.Structure):
+        _fields_ = _PREFIX_FIELDS + [
+            ("max_steps", C.c_int), ("sor_omega", C.c_double), ("sor_max_iter", C.c_int),
+            ("sor_tol", C.c_double), ("sor_criterion", C.c_int), ("multipole_lmax", C.c_int),
+            ("poisson_mode", C.c_int), ("nn_guess", C.c_char * 256), ("output_enable", C.c_bool),
+            ("ic_file", C.c_char * 256)]
+
+    lib = H.lib()
+    p = lib.input_deck_create(path.encode())
+    d = Deck.from_address(p)
+    got = (d.nx, d.max_steps, d.sor_omega, d.sor_max_iter, d.sor_tol, d.sor_criterion,
+           d.multipole_lmax, d.poisson_mode, d.output_enable, d.ic_file)
+    lib.input_deck_destroy(p)
+    assert got == (32, 200, 1.5, 10000, 1e-8, H.SOR_CRIT_REL_RESID, 6, H.POISSON_FP32_DC, False,
+                   b"state.bin")
+
+
+def test_initial_state_file_roundtrip(tmp_path):
+    from hydrocode_astro1_amd import ics
+    U = ics.sod(8, 4, 4)
+    path = str(tmp_path / "s.bin")
+    ics.write_ic_file(path, U)
+    assert np.array_equal(ics.read_ic_file(path), U)
