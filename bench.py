@@ -126,6 +126,9 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-poisson", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--calibrate", action="store_true",
+                    help="only run the device layout-conversion copy (40 B read + 40 B written "
+                         "per cell) used to scale HBM byte counters, then exit")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the slab "
@@ -140,6 +143,15 @@ def main():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
     n = args.n
+    if args.calibrate:
+        import numpy as np
+        s = H.Solver(n, n, n, G=0.0)
+        s.set_state(np.ones((n, n, n, 5)))
+        for _ in range(3):
+            s.state()  # soa_to_aos_kernel: reads and writes 40 B per cell
+        print(json.dumps({"calibration": "soa_to_aos_kernel", "cells": n ** 3,
+                          "bytes_read": 40 * n ** 3, "bytes_written": 40 * n ** 3}))
+        return
     t0 = time.perf_counter()
     U = ics.explosion_n3(n)
     print(f"[bench] initial state {n}^3 built in {time.perf_counter() - t0:.1f}s", file=sys.stderr,

@@ -110,6 +110,7 @@ EXPORTS = [
     ("solver3d_run", C.c_int, [_SP, C.c_double, C.c_int, _dp, _dp, _ip, C.c_int]),
     ("solver3d_last_timing", C.c_int, [_SP, _dp, _dp]),
     ("solver3d_benchmark_steps", C.c_int, [_SP, C.c_int, _dp, _dp, _dp]),
+    ("hydro_check_division", C.c_long, [_dp, _dp, C.c_long]),
     ("gravity_set_multipole_lmax", C.c_int, [_GP, C.c_int]),
     ("gravity_set_poisson_mode", C.c_int, [_GP, C.c_int]),
     ("gravity_set_criterion", C.c_int, [_GP, C.c_int]),

@@ -19,44 +19,78 @@ namespace hyd {
 __device__ __forceinline__ double tmin(double a, double b) { return (a < b) ? a : b; }
 __device__ __forceinline__ double tmax(double a, double b) { return (a > b) ? a : b; }
 
-// Conserved state of one cell plus its primitives (P.rho is recomputed from
-// d by the floor when needed, so it is not stored).
+// a / b given yb = RN(1/b) (an IEEE division 1.0 / b), correctly rounded by
+// one Markstein correction: q0 = RN(a yb) is within one ulp of a/b, the
+// remainder a - b q0 is exact in a fused multiply-add, and RN(q0 + r yb) is
+// RN(a/b) (Markstein 1990; Muller et al., Handbook of Floating-Point
+// Arithmetic, the quotient theorem).  Zero, tiny and huge quotients, where the
+// remainder could underflow or the sign of zero would differ, take the IEEE
+// division instead, so the result always equals a / b bit for bit.
+__device__ __forceinline__ double div_rcp(double a, double b, double yb) {
+    const double q0 = a * yb;
+    const double r = fma(-q0, b, a);
+    double q = fma(r, yb, q0);
+    const double m = fabs(q0);
+    if (!(m >= 0x1p-900 && m <= 0x1p+900)) q = a / b;
+    return q;
+}
+
+// Conserved state of one cell plus its primitives and the reciprocal of
+// the floored density (P.rho itself is recomputed from d by the floor).
 struct Cell {
     double d, mx, my, mz, e;  // rho, rho u, rho v, rho w, E
     double u, v, w, p, c;     // velocities, pressure (floored), sound speed
+    double yr;                // RN(1 / max(rho, floor))
 };
 
 __device__ __forceinline__ double rho_floor(double d) { return tmax(d, HYD_RHO_FLOOR); }
 
 // cons_to_prim_3d, hydro_solver_3d.c:162-175; sound_speed_3d :191-193.
 __device__ __forceinline__ void prims(Cell& q, double gamma) {
-    double r = rho_floor(q.d);
-    q.u = q.mx / r;
-    q.v = q.my / r;
-    q.w = q.mz / r;
+    const double r = rho_floor(q.d);
+    const double yr = 1.0 / r;
+    q.yr = yr;
+    q.u = div_rcp(q.mx, r, yr);
+    q.v = div_rcp(q.my, r, yr);
+    q.w = div_rcp(q.mz, r, yr);
     double ke = 0.5 * r * (q.u * q.u + q.v * q.v + q.w * q.w);
     double p = (gamma - 1.0) * (q.e - ke);
     q.p = tmax(p, HYD_P_FLOOR);
-    q.c = sqrt(gamma * q.p / r);
+    q.c = sqrt(div_rcp(gamma * q.p, r, yr));
+}
+
+// Constants of one step shared by every cell.
+struct EosConst {
+    double gamma, gm1, ygm1;  // gamma, gamma - 1, RN(1 / (gamma - 1))
+    double yfloor;            // RN(1 / RHO_FLOOR)
+};
+
+__device__ __forceinline__ EosConst eos_const(double gamma) {
+    EosConst k;
+    k.gamma = gamma;
+    k.gm1 = gamma - 1.0;
+    k.ygm1 = 1.0 / k.gm1;
+    k.yfloor = 1.0 / HYD_RHO_FLOOR;
+    return k;
 }
 
 // Floors and the cons->prim->cons round trip of the update,
-// hydro_solver_3d.c:765-768 with prim_to_cons_3d :177-189.
-__device__ __forceinline__ void floor_roundtrip(double& d, double& mx, double& my, double& mz,
-                                                double& e, double gamma) {
-    d = tmax(d, HYD_RHO_FLOOR);
-    double r = tmax(d, HYD_RHO_FLOOR);
-    double u = mx / r, v = my / r, w = mz / r;
-    double ke = 0.5 * r * (u * u + v * v + w * w);
-    double p = (gamma - 1.0) * (e - ke);
-    p = tmax(p, HYD_P_FLOOR);
+// hydro_solver_3d.c:765-768 with prim_to_cons_3d :177-189.  Returns
+// RN(1 / rho) of the result.
+__device__ __forceinline__ double floor_roundtrip(double& d, double& mx, double& my, double& mz,
+                                                  double& e, const EosConst& k) {
+    const double r = tmax(d, HYD_RHO_FLOOR);
+    const double yr = 1.0 / r;
+    const double u = div_rcp(mx, r, yr), v = div_rcp(my, r, yr), w = div_rcp(mz, r, yr);
+    const double ke = 0.5 * r * (u * u + v * v + w * w);
+    double p = k.gm1 * (e - ke);
     p = tmax(p, HYD_P_FLOOR);
     d = r;
     mx = r * u;
     my = r * v;
     mz = r * w;
-    double ke2 = 0.5 * r * (u * u + v * v + w * w);
-    e = p / (gamma - 1.0) + ke2;
+    e = div_rcp(p, k.gm1, k.ygm1) + ke;
+    return yr;
 }
 
 // Reflect the velocity component normal to a face through cons->prim->cons,
@@ -92,6 +126,18 @@ __device__ __forceinline__ double signal_speed(double d, double mx, double my, d
     double c = sqrt(gamma * p / r);
     double a = fabs(u), b = fabs(v), z = fabs(w);
     double m = tmax(tmax(a, b), z);
+    return m + c;
+}
+
+// The same with the reciprocal of the floored density supplied.
+__device__ __forceinline__ double signal_speed_rcp(double d, double mx, double my, double mz,
+                                                   double e, double yr, const EosConst& k) {
+    const double r = rho_floor(d);
+    const double u = div_rcp(mx, r, yr), v = div_rcp(my, r, yr), w = div_rcp(mz, r, yr);
+    const double ke = 0.5 * r * (u * u + v * v + w * w);
+    const double p = tmax(k.gm1 * (e - ke), HYD_P_FLOOR);
+    const double c = sqrt(div_rcp(k.gamma * p, r, yr));
+    const double m = tmax(tmax(fabs(u), fabs(v)), fabs(w));
     return m + c;
 }
 
@@ -147,7 +193,8 @@ __device__ __forceinline__ void hllc(const Cell& L, const Cell& R, double gamma,
     const double smx = sd * (DIR == 0 ? Ss : ku);
     const double smy = sd * (DIR == 1 ? Ss : kv);
     const double smz = sd * (DIR == 2 ? Ss : kw);
-    const double se = sd * (ke / kr + (Ss - kn) * (Ss + kp / (kr * (S - kn))));
+    const double kyr = left ? L.yr : R.yr;
+    const double se = sd * (div_rcp(ke, kr, kyr) + (Ss - kn) * (Ss + kp / (kr * (S - kn))));
 
     const bool sup = sup_l || sup_r;
     F[0] = sup ? f0 : f0 + S * (sd - kd);
@@ -211,6 +258,21 @@ __device__ __forceinline__ unsigned long long wave_max_u64(unsigned long long v)
     for (int o = 32; o > 0; o >>= 1) {
         unsigned long long p = __shfl_xor(v, o, 64);
         v = p > v ? p : v;
+    }
+    return v;
+}
+
+// Max over a block (blockDim.x a multiple of 64, at most 1024 threads); the
+// result is valid in thread 0.  Every thread of the block must call it.
+__device__ __forceinline__ unsigned long long block_max_u64(unsigned long long v) {
+    __shared__ unsigned long long s_part[16];
+    v = wave_max_u64(v);
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) s_part[w] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const int nw = blockDim.x >> 6;
+        for (int q = 1; q < nw; ++q) v = s_part[q] > v ? s_part[q] : v;
     }
     return v;
 }

@@ -91,8 +91,8 @@ __global__ void __launch_bounds__(256) sor_sweep_kernel(GridDims g, SorCoef c, i
             rb = dbits(fabs(gs - old));
         }
     }
-    rb = wave_max_u64(rb);
-    if ((threadIdx.x & 63) == 0 && rb != 0ull) atomicMax(&ctl->res[it % 3], rb);
+    rb = block_max_u64(rb);
+    if (threadIdx.x == 0 && rb != 0ull) atomicMax(&ctl->res[it % 3], rb);
 }
 
 __global__ void sor_reset_kernel(SorCtl* ctl) {

@@ -605,6 +605,29 @@ int solver3d_benchmark_steps(GodunovHLLC3D* s, int nsteps, double* wall_ms, doub
     return rc;
 }
 
+long hydro_check_division(const double* a, const double* b, long n) {
+    hipStream_t st = hyd_stream();
+    if (!st || n <= 0) return -2;
+    double *da = nullptr, *db = nullptr;
+    unsigned long long* dm = nullptr;
+    unsigned long long hm = 0;
+    long rc = -2;
+    if (hipMalloc((void**)&da, sizeof(double) * n) == hipSuccess &&
+        hipMalloc((void**)&db, sizeof(double) * n) == hipSuccess &&
+        hipMalloc((void**)&dm, sizeof(unsigned long long)) == hipSuccess &&
+        hipMemcpyAsync(da, a, sizeof(double) * n, hipMemcpyHostToDevice, st) == hipSuccess &&
+        hipMemcpyAsync(db, b, sizeof(double) * n, hipMemcpyHostToDevice, st) == hipSuccess &&
+        hipMemsetAsync(dm, 0, sizeof(unsigned long long), st) == hipSuccess &&
+        hyd::launch_check_division(da, db, n, dm, st) == hipSuccess &&
+        hipMemcpyAsync(&hm, dm, sizeof hm, hipMemcpyDeviceToHost, st) == hipSuccess &&
+        hipStreamSynchronize(st) == hipSuccess)
+        rc = (long)hm;
+    if (da) (void)hipFree(da);
+    if (db) (void)hipFree(db);
+    if (dm) (void)hipFree(dm);
+    return rc;
+}
+
 int solver3d_last_timing(const GodunovHLLC3D* s, double* hydro_ms, double* gravity_ms) {
     HydroDevice* d = s->dev;
     float ms = 0.0f;

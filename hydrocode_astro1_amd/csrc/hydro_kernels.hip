@@ -40,7 +40,7 @@ constexpr int NW = TJ + 2;        // wavefronts per block
 constexpr int NT = NW * 64;       // threads per block
 constexpr int LR = TJ + 2;        // LDS rows: j0-1 .. j0+8
 constexpr int LC = TK + 2;        // LDS cols: k0-1 .. k0+64
-constexpr int NF = 10;            // LDS fields per cell: d mx my mz e u v w p c
+constexpr int NF = 11;            // LDS fields per cell: d mx my mz e u v w p c yr
 }  // namespace
 
 __device__ __forceinline__ double dt_from_speed(unsigned long long bits, double dx, double dy,
@@ -79,13 +79,13 @@ typedef double LdsCells[NF][LR][LC];
 __device__ __forceinline__ void lds_put(LdsCells& s, int r, int c, const Cell& q) {
     s[0][r][c] = q.d; s[1][r][c] = q.mx; s[2][r][c] = q.my; s[3][r][c] = q.mz;
     s[4][r][c] = q.e; s[5][r][c] = q.u; s[6][r][c] = q.v; s[7][r][c] = q.w;
-    s[8][r][c] = q.p; s[9][r][c] = q.c;
+    s[8][r][c] = q.p; s[9][r][c] = q.c; s[10][r][c] = q.yr;
 }
 __device__ __forceinline__ Cell lds_get(const LdsCells& s, int r, int c) {
     Cell q;
     q.d = s[0][r][c]; q.mx = s[1][r][c]; q.my = s[2][r][c]; q.mz = s[3][r][c];
     q.e = s[4][r][c]; q.u = s[5][r][c]; q.v = s[6][r][c]; q.w = s[7][r][c];
-    q.p = s[8][r][c]; q.c = s[9][r][c];
+    q.p = s[8][r][c]; q.c = s[9][r][c]; q.yr = s[10][r][c];
     return q;
 }
 
@@ -118,6 +118,7 @@ __global__ void __launch_bounds__(NT, 2) hydro_step_kernel(HydroStepArgs a) {
     __shared__ double s_cell[NF][LR][LC];
     __shared__ double s_fy[5][TJ + 1][TK];
     __shared__ double s_fzl[2][5][TJ];
+    __shared__ unsigned long long s_smax[NW];
 
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     // wavefront index made scalar so role branches are uniform jumps
@@ -160,6 +161,7 @@ __global__ void __launch_bounds__(NT, 2) hydro_step_kernel(HydroStepArgs a) {
         *a.ms_clear = 0ull;
     }
     const double gamma = a.gamma;
+    const EosConst kc = eos_const(gamma);
 
     // prologue: plane ib into LDS
     {
@@ -229,6 +231,7 @@ __global__ void __launch_bounds__(NT, 2) hydro_step_kernel(HydroStepArgs a) {
                 double q[5] = {cur.d, cur.mx, cur.my, cur.mz, cur.e};
                 const bool interior = (i >= 1) && (i <= nx - 2) && (j >= 1) && (j <= ny - 2) &&
                                       (k >= 1) && (k <= nz - 2);
+                double yq;
                 if (interior) {
                     // hydro_solver_3d.c:738-762: x, then y, then z, left to right
 #pragma unroll
@@ -237,14 +240,17 @@ __global__ void __launch_bounds__(NT, 2) hydro_step_kernel(HydroStepArgs a) {
                         t = t - az * dz[f];
                         q[f] = q[f] + t;
                     }
-                    floor_roundtrip(q[0], q[1], q[2], q[3], q[4], gamma);
+                    yq = floor_roundtrip(q[0], q[1], q[2], q[3], q[4], kc);
+                } else {
+                    yq = 1.0 / q[0];  // boundary cells keep their raw density
                 }
                 const long idx = cidx + off;
                 if (a.gravity) {
                     // apply_gravity_source_terms, hydro_solver_3d.c:606-627
                     const double gx = a.g[0][idx], gy = a.g[1][idx], gz = a.g[2][idx];
                     const double rho = q[0];
-                    const double u = q[1] / rho, v = q[2] / rho, w = q[3] / rho;
+                    const double u = div_rcp(q[1], rho, yq), v = div_rcp(q[2], rho, yq),
+                                 w = div_rcp(q[3], rho, yq);
                     q[1] = q[1] + dt * rho * gx;
                     q[2] = q[2] + dt * rho * gy;
                     q[3] = q[3] + dt * rho * gz;
@@ -254,8 +260,9 @@ __global__ void __launch_bounds__(NT, 2) hydro_step_kernel(HydroStepArgs a) {
                 if (interior) {
 #pragma unroll
                     for (int f = 0; f < 5; ++f) a.uout[f][idx] = q[f];
+                    // interior density is already floored: yq is its reciprocal
                     unsigned long long sb =
-                        dbits(signal_speed(q[0], q[1], q[2], q[3], q[4], gamma));
+                        dbits(signal_speed_rcp(q[0], q[1], q[2], q[3], q[4], yq, kc));
                     smax = sb > smax ? sb : smax;
                 } else {
 #pragma unroll
@@ -263,8 +270,7 @@ __global__ void __launch_bounds__(NT, 2) hydro_step_kernel(HydroStepArgs a) {
                 }
             }
         }
-        smax = wave_max_u64(smax);
-        if (lane == 0 && smax != 0ull) atomicMax(a.ms_out, smax);
+        s_smax[wave] = wave_max_u64(smax);
     } else if (wave == 0) {
         // ------------------------------------------- low halo row (j0-1)
         // F_y(j0-1, j0) each plane; loads the left z halo and forms
@@ -323,6 +329,13 @@ __global__ void __launch_bounds__(NT, 2) hydro_step_kernel(HydroStepArgs a) {
             __syncthreads();  // B3
         }
     }
+    // one atomic per block for the signal speed of the updated interior
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        unsigned long long m = 0ull;
+        for (int w = 1; w <= TJ; ++w) m = s_smax[w] > m ? s_smax[w] : m;
+        if (m != 0ull) atomicMax(a.ms_out, m);
+    }
 }
 
 // ---------------------------------------------------------------------------
@@ -343,9 +356,9 @@ __device__ __forceinline__ void maybe_reflect(bool on, double q[5], double gamma
 __global__ void __launch_bounds__(256) hydro_bc_kernel(HydroBCArgs a) {
     const int nx = a.nx, ny = a.ny, nz = a.nz;
     const long nX = 2L * ny * nz, nY = 2L * (nx - 2) * nz, nZ = 2L * (nx - 2) * (ny - 2);
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
     unsigned long long smax = 0ull;
-    if (t < nX + nY + nZ) {
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < nX + nY + nZ;
+         t += (long)gridDim.x * blockDim.x) {
         int i, j, k;
         if (t < nX) {
             long side = t / ((long)ny * nz), rem = t % ((long)ny * nz);
@@ -397,10 +410,11 @@ __global__ void __launch_bounds__(256) hydro_bc_kernel(HydroBCArgs a) {
         const long idx = ((long)i * ny + j) * nz + k;
 #pragma unroll
         for (int f = 0; f < 5; ++f) a.u[f][idx] = q[f];
-        smax = dbits(signal_speed(q[0], q[1], q[2], q[3], q[4], a.gamma));
+        const unsigned long long sb = dbits(signal_speed(q[0], q[1], q[2], q[3], q[4], a.gamma));
+        smax = sb > smax ? sb : smax;
     }
-    smax = wave_max_u64(smax);
-    if ((threadIdx.x & 63) == 0 && smax != 0ull) atomicMax(a.ms_out, smax);
+    smax = block_max_u64(smax);
+    if (threadIdx.x == 0 && smax != 0ull) atomicMax(a.ms_out, smax);
 }
 
 // Signal speed of a whole state (after an upload).
@@ -417,8 +431,8 @@ __global__ void __launch_bounds__(256) hydro_maxspeed_kernel(const double* __res
         unsigned long long sb = dbits(signal_speed(d[c], mx[c], my[c], mz[c], e[c], gamma));
         smax = sb > smax ? sb : smax;
     }
-    smax = wave_max_u64(smax);
-    if ((threadIdx.x & 63) == 0 && smax != 0ull) atomicMax(out, smax);
+    smax = block_max_u64(smax);
+    if (threadIdx.x == 0 && smax != 0ull) atomicMax(out, smax);
 }
 
 // AoS (host layout, 5 doubles per cell) <-> SoA planes.
@@ -439,6 +453,29 @@ __global__ void soa_to_aos_kernel(double* __restrict__ aos, const double* d, con
     }
 }
 
+// Diagnostic: count quotients where the reciprocal-corrected division of the
+// step kernels differs in any bit from the IEEE division.
+__global__ void check_division_kernel(const double* __restrict__ a, const double* __restrict__ b,
+                                      long n, unsigned long long* mismatches) {
+    unsigned long long bad = 0;
+    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
+         t += (long)gridDim.x * blockDim.x) {
+        const double x = a[t], y = b[t];
+        const double q1 = x / y;
+        const double q2 = div_rcp(x, y, 1.0 / y);
+        bad += (dbits(q1) != dbits(q2)) ? 1ull : 0ull;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) bad += __shfl_xor(bad, o, 64);
+    if ((threadIdx.x & 63) == 0 && bad) atomicAdd(mismatches, bad);
+}
+
+hipError_t launch_check_division(const double* a, const double* b, long n,
+                                 unsigned long long* mismatches, hipStream_t st) {
+    hipLaunchKernelGGL(check_division_kernel, dim3(1024), dim3(256), 0, st, a, b, n, mismatches);
+    return hipGetLastError();
+}
+
 // ---------------------------------------------------------------- launchers
 hipError_t launch_hydro_step(const HydroStepArgs& a, hipStream_t st) {
     const int ntj = (a.ny + TJ - 1) / TJ, ntk = (a.nz + TK - 1) / TK;
@@ -452,15 +489,16 @@ hipError_t launch_hydro_step(const HydroStepArgs& a, hipStream_t st) {
 
 hipError_t launch_hydro_bc(const HydroBCArgs& a, hipStream_t st) {
     const long tot = 2L * a.ny * a.nz + 2L * (a.nx - 2) * a.nz + 2L * (a.nx - 2) * (a.ny - 2);
-    const unsigned nb = (unsigned)((tot + 255) / 256);
-    hipLaunchKernelGGL(hydro_bc_kernel, dim3(nb), dim3(256), 0, st, a);
+    long nb = (tot + 255) / 256;
+    if (nb > 512) nb = 512;  // grid stride: few blocks, one atomic each
+    hipLaunchKernelGGL(hydro_bc_kernel, dim3((unsigned)nb), dim3(256), 0, st, a);
     return hipGetLastError();
 }
 
 hipError_t launch_hydro_maxspeed(double* const u[5], long n, double gamma, unsigned long long* out,
                                  hipStream_t st) {
     long nb = (n + 255) / 256;
-    if (nb > 8192) nb = 8192;
+    if (nb > 1024) nb = 1024;
     hipLaunchKernelGGL(hydro_maxspeed_kernel, dim3((unsigned)nb), dim3(256), 0, st, u[0], u[1],
                        u[2], u[3], u[4], n, gamma, out);
     return hipGetLastError();

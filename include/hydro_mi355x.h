@@ -60,6 +60,11 @@ int solver3d_benchmark_steps(GodunovHLLC3D *solver, int nsteps, double *wall_ms,
  * the last gravity solve. */
 int solver3d_last_timing(const GodunovHLLC3D *solver, double *hydro_ms, double *gravity_ms);
 
+/* Diagnostic: number of the n quotients a[i] / b[i] (host arrays) for which
+ * the reciprocal-corrected division used by the step kernels differs in any
+ * bit from the IEEE division on the device (0 expected), < 0 on error. */
+long hydro_check_division(const double *a, const double *b, long n);
+
 /* ---- gravity settings (no reference counterpart) ----------------------- */
 
 /* Multipole order of the isolated boundary: 2 = monopole + quadrupole as
