@@ -21,7 +21,7 @@ import numpy as np
 from . import ics  # noqa: F401  (initial conditions for the benchmark configs)
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(_HERE, "libhydro_mi355x.so")
+LIB_PATH = os.environ.get("HYDRO_MI355X_LIB") or os.path.join(_HERE, "libhydro_mi355x.so")
 
 BC_PERIODIC, BC_REFLECTIVE, BC_OUTFLOW = 0, 1, 2
 GRAV_BC_PERIODIC, GRAV_BC_ISOLATED, GRAV_BC_DIRICHLET = 0, 1, 2

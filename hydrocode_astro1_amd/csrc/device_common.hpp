@@ -19,28 +19,48 @@ namespace hyd {
 __device__ __forceinline__ double tmin(double a, double b) { return (a < b) ? a : b; }
 __device__ __forceinline__ double tmax(double a, double b) { return (a > b) ? a : b; }
 
-// a / b given yb = RN(1/b) (an IEEE division 1.0 / b), correctly rounded by
-// one Markstein correction: q0 = RN(a yb) is within one ulp of a/b, the
-// remainder a - b q0 is exact in a fused multiply-add, and RN(q0 + r yb) is
-// RN(a/b) (Markstein 1990; Muller et al., Handbook of Floating-Point
-// Arithmetic, the quotient theorem).  Zero, tiny and huge quotients, where the
-// remainder could underflow or the sign of zero would differ, take the IEEE
-// division instead, so the result always equals a / b bit for bit.
-__device__ __forceinline__ double div_rcp(double a, double b, double yb) {
+// Quotients a / b that share a positive denominator b, with yb = RN(1/b)
+// (an IEEE division 1.0 / b), by one Markstein correction each: q0 = RN(a yb)
+// is within one ulp of a/b, the remainder a - b q0 is exact in a fused
+// multiply-add, and RN(q0 + r yb) is RN(a/b) (Markstein 1990; Muller et al.,
+// Handbook of Floating-Point Arithmetic, the quotient theorem).  The theorem
+// needs the remainder clear of underflow and q0 in range: `ok` turns false
+// for a tiny nonzero dividend or an extreme quotient, and the caller then
+// recomputes the whole group with IEEE divisions, so every result equals
+// a / b bit for bit.  A zero dividend returns itself (b > 0 keeps its sign).
+__device__ __forceinline__ double div_rcp_q(double a, double b, double yb, bool& ok) {
     const double q0 = a * yb;
     const double r = fma(-q0, b, a);
-    double q = fma(r, yb, q0);
-    const double m = fabs(q0);
-    if (!(m >= 0x1p-900 && m <= 0x1p+900)) q = a / b;
+    const double q = fma(r, yb, q0);
+    const double m = fabs(q0), ma = fabs(a);
+    ok = ok && ((ma == 0.0) || (ma >= 0x1p-900 && m >= 0x1p-900 && m <= 0x1p+900));
+    return (ma == 0.0) ? a : q;
+}
+
+// Single quotient with its own IEEE fallback (used by the self-check).
+__device__ __forceinline__ double div_rcp(double a, double b, double yb) {
+    bool ok = true;
+    double q = div_rcp_q(a, b, yb, ok);
+    if (!ok) q = a / b;
     return q;
 }
 
-// Conserved state of one cell plus its primitives and the reciprocal of
-// the floored density (P.rho itself is recomputed from d by the floor).
+// Off by default: on gfx950 the grouped Markstein form plus its range
+// checks issues about as many VALU operations as the IEEE divisions it
+// replaces, and measured slower in the step kernel.
+#ifndef HYD_FASTDIV
+#define HYD_FASTDIV 0
+#endif
+// Keeps the rare IEEE recomputation a real branch: the compiler may not
+// speculate across a volatile asm statement, so the common path never
+// executes the full divisions.
+#define HYD_COLD_PATH() asm volatile("")
+
+// Conserved state of one cell plus its primitives (P.rho itself is
+// recomputed from d by the floor).
 struct Cell {
     double d, mx, my, mz, e;  // rho, rho u, rho v, rho w, E
     double u, v, w, p, c;     // velocities, pressure (floored), sound speed
-    double yr;                // RN(1 / max(rho, floor))
 };
 
 __device__ __forceinline__ double rho_floor(double d) { return tmax(d, HYD_RHO_FLOOR); }
@@ -48,15 +68,34 @@ __device__ __forceinline__ double rho_floor(double d) { return tmax(d, HYD_RHO_F
 // cons_to_prim_3d, hydro_solver_3d.c:162-175; sound_speed_3d :191-193.
 __device__ __forceinline__ void prims(Cell& q, double gamma) {
     const double r = rho_floor(q.d);
+#if HYD_FASTDIV
     const double yr = 1.0 / r;
-    q.yr = yr;
-    q.u = div_rcp(q.mx, r, yr);
-    q.v = div_rcp(q.my, r, yr);
-    q.w = div_rcp(q.mz, r, yr);
+    bool ok = true;
+    q.u = div_rcp_q(q.mx, r, yr, ok);
+    q.v = div_rcp_q(q.my, r, yr, ok);
+    q.w = div_rcp_q(q.mz, r, yr, ok);
+    double ke = 0.5 * r * (q.u * q.u + q.v * q.v + q.w * q.w);
+    q.p = tmax((gamma - 1.0) * (q.e - ke), HYD_P_FLOOR);
+    double c2 = div_rcp_q(gamma * q.p, r, yr, ok);
+    if (!ok) {
+        HYD_COLD_PATH();
+        q.u = q.mx / r;
+        q.v = q.my / r;
+        q.w = q.mz / r;
+        ke = 0.5 * r * (q.u * q.u + q.v * q.v + q.w * q.w);
+        q.p = tmax((gamma - 1.0) * (q.e - ke), HYD_P_FLOOR);
+        c2 = gamma * q.p / r;
+    }
+    q.c = sqrt(c2);
+#else
+    q.u = q.mx / r;
+    q.v = q.my / r;
+    q.w = q.mz / r;
     double ke = 0.5 * r * (q.u * q.u + q.v * q.v + q.w * q.w);
     double p = (gamma - 1.0) * (q.e - ke);
     q.p = tmax(p, HYD_P_FLOOR);
-    q.c = sqrt(div_rcp(gamma * q.p, r, yr));
+    q.c = sqrt(gamma * q.p / r);
+#endif
 }
 
 // Constants of one step shared by every cell.
@@ -75,22 +114,38 @@ __device__ __forceinline__ EosConst eos_const(double gamma) {
 }
 
 // Floors and the cons->prim->cons round trip of the update,
-// hydro_solver_3d.c:765-768 with prim_to_cons_3d :177-189.  Returns
-// RN(1 / rho) of the result.
-__device__ __forceinline__ double floor_roundtrip(double& d, double& mx, double& my, double& mz,
-                                                  double& e, const EosConst& k) {
+// hydro_solver_3d.c:765-768 with prim_to_cons_3d :177-189.
+__device__ __forceinline__ void floor_roundtrip(double& d, double& mx, double& my, double& mz,
+                                                double& e, const EosConst& k) {
     const double r = tmax(d, HYD_RHO_FLOOR);
+#if HYD_FASTDIV
     const double yr = 1.0 / r;
-    const double u = div_rcp(mx, r, yr), v = div_rcp(my, r, yr), w = div_rcp(mz, r, yr);
+    bool ok = true;
+    double u = div_rcp_q(mx, r, yr, ok), v = div_rcp_q(my, r, yr, ok),
+           w = div_rcp_q(mz, r, yr, ok);
+    double ke = 0.5 * r * (u * u + v * v + w * w);
+    double p = tmax(k.gm1 * (e - ke), HYD_P_FLOOR);
+    double ep = div_rcp_q(p, k.gm1, k.ygm1, ok);
+    if (!ok) {
+        HYD_COLD_PATH();
+        u = mx / r;
+        v = my / r;
+        w = mz / r;
+        ke = 0.5 * r * (u * u + v * v + w * w);
+        p = tmax(k.gm1 * (e - ke), HYD_P_FLOOR);
+        ep = p / k.gm1;
+    }
+#else
+    const double u = mx / r, v = my / r, w = mz / r;
     const double ke = 0.5 * r * (u * u + v * v + w * w);
-    double p = k.gm1 * (e - ke);
-    p = tmax(p, HYD_P_FLOOR);
+    const double p = tmax(k.gm1 * (e - ke), HYD_P_FLOOR);
+    const double ep = p / k.gm1;
+#endif
     d = r;
     mx = r * u;
     my = r * v;
     mz = r * w;
-    e = div_rcp(p, k.gm1, k.ygm1) + ke;
-    return yr;
+    e = ep + ke;
 }
 
 // Reflect the velocity component normal to a face through cons->prim->cons,
@@ -129,16 +184,53 @@ __device__ __forceinline__ double signal_speed(double d, double mx, double my, d
     return m + c;
 }
 
-// The same with the reciprocal of the floored density supplied.
-__device__ __forceinline__ double signal_speed_rcp(double d, double mx, double my, double mz,
-                                                   double e, double yr, const EosConst& k) {
-    const double r = rho_floor(d);
-    const double u = div_rcp(mx, r, yr), v = div_rcp(my, r, yr), w = div_rcp(mz, r, yr);
+// The same for a state whose density is already at or above the floor.
+__device__ __forceinline__ double signal_speed_floored(double r, double mx, double my, double mz,
+                                                       double e, const EosConst& k) {
+#if HYD_FASTDIV
+    const double yr = 1.0 / r;
+    bool ok = true;
+    double u = div_rcp_q(mx, r, yr, ok), v = div_rcp_q(my, r, yr, ok),
+           w = div_rcp_q(mz, r, yr, ok);
+    double ke = 0.5 * r * (u * u + v * v + w * w);
+    double p = tmax(k.gm1 * (e - ke), HYD_P_FLOOR);
+    double c2 = div_rcp_q(k.gamma * p, r, yr, ok);
+    if (!ok) {
+        HYD_COLD_PATH();
+        u = mx / r;
+        v = my / r;
+        w = mz / r;
+        ke = 0.5 * r * (u * u + v * v + w * w);
+        p = tmax(k.gm1 * (e - ke), HYD_P_FLOOR);
+        c2 = k.gamma * p / r;
+    }
+#else
+    const double u = mx / r, v = my / r, w = mz / r;
     const double ke = 0.5 * r * (u * u + v * v + w * w);
     const double p = tmax(k.gm1 * (e - ke), HYD_P_FLOOR);
-    const double c = sqrt(div_rcp(k.gamma * p, r, yr));
+    const double c2 = k.gamma * p / r;
+#endif
     const double m = tmax(tmax(fabs(u), fabs(v)), fabs(w));
-    return m + c;
+    return m + sqrt(c2);
+}
+
+// Velocities of the gravity source (hydro_solver_3d.c:612-614): no floor.
+__device__ __forceinline__ void velocities(double rho, double mx, double my, double mz, double& u,
+                                           double& v, double& w) {
+#if HYD_FASTDIV
+    if (rho > 0.0) {
+        const double y = 1.0 / rho;
+        bool ok = true;
+        u = div_rcp_q(mx, rho, y, ok);
+        v = div_rcp_q(my, rho, y, ok);
+        w = div_rcp_q(mz, rho, y, ok);
+        if (ok) return;
+        HYD_COLD_PATH();
+    }
+#endif
+    u = mx / rho;
+    v = my / rho;
+    w = mz / rho;
 }
 
 // HLLC flux through the face between L and R along DIR (0 x, 1 y, 2 z),
@@ -193,8 +285,7 @@ __device__ __forceinline__ void hllc(const Cell& L, const Cell& R, double gamma,
     const double smx = sd * (DIR == 0 ? Ss : ku);
     const double smy = sd * (DIR == 1 ? Ss : kv);
     const double smz = sd * (DIR == 2 ? Ss : kw);
-    const double kyr = left ? L.yr : R.yr;
-    const double se = sd * (div_rcp(ke, kr, kyr) + (Ss - kn) * (Ss + kp / (kr * (S - kn))));
+    const double se = sd * (ke / kr + (Ss - kn) * (Ss + kp / (kr * (S - kn))));
 
     const bool sup = sup_l || sup_r;
     F[0] = sup ? f0 : f0 + S * (sd - kd);

@@ -91,8 +91,13 @@ __global__ void __launch_bounds__(256) sor_sweep_kernel(GridDims g, SorCoef c, i
             rb = dbits(fabs(gs - old));
         }
     }
+    if (tol < 0.0) return;  // relative-residual mode: the max update is not tracked
     rb = block_max_u64(rb);
-    if (threadIdx.x == 0 && rb != 0ull) atomicMax(&ctl->res[it % 3], rb);
+    // most blocks see a running max already above theirs: skip their atomic
+    if (threadIdx.x == 0 && rb != 0ull) {
+        const unsigned long long cur = __atomic_load_n(&ctl->res[it % 3], __ATOMIC_RELAXED);
+        if (rb > cur) atomicMax(&ctl->res[it % 3], rb);
+    }
 }
 
 __global__ void sor_reset_kernel(SorCtl* ctl) {

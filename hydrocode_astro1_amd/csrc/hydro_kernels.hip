@@ -34,13 +34,16 @@
 namespace hyd {
 
 namespace {
+#ifndef HYD_TJ
+#define HYD_TJ 8
+#endif
 constexpr int TK = 64;            // lanes (k) per row
-constexpr int TJ = 8;             // cell rows (j) per block
+constexpr int TJ = HYD_TJ;        // cell rows (j) per block
 constexpr int NW = TJ + 2;        // wavefronts per block
 constexpr int NT = NW * 64;       // threads per block
-constexpr int LR = TJ + 2;        // LDS rows: j0-1 .. j0+8
+constexpr int LR = TJ + 2;        // LDS rows: j0-1 .. j0+TJ
 constexpr int LC = TK + 2;        // LDS cols: k0-1 .. k0+64
-constexpr int NF = 11;            // LDS fields per cell: d mx my mz e u v w p c yr
+constexpr int NF = 10;            // LDS fields per cell: d mx my mz e u v w p c
 }  // namespace
 
 __device__ __forceinline__ double dt_from_speed(unsigned long long bits, double dx, double dy,
@@ -79,13 +82,13 @@ typedef double LdsCells[NF][LR][LC];
 __device__ __forceinline__ void lds_put(LdsCells& s, int r, int c, const Cell& q) {
     s[0][r][c] = q.d; s[1][r][c] = q.mx; s[2][r][c] = q.my; s[3][r][c] = q.mz;
     s[4][r][c] = q.e; s[5][r][c] = q.u; s[6][r][c] = q.v; s[7][r][c] = q.w;
-    s[8][r][c] = q.p; s[9][r][c] = q.c; s[10][r][c] = q.yr;
+    s[8][r][c] = q.p; s[9][r][c] = q.c;
 }
 __device__ __forceinline__ Cell lds_get(const LdsCells& s, int r, int c) {
     Cell q;
     q.d = s[0][r][c]; q.mx = s[1][r][c]; q.my = s[2][r][c]; q.mz = s[3][r][c];
     q.e = s[4][r][c]; q.u = s[5][r][c]; q.v = s[6][r][c]; q.w = s[7][r][c];
-    q.p = s[8][r][c]; q.c = s[9][r][c]; q.yr = s[10][r][c];
+    q.p = s[8][r][c]; q.c = s[9][r][c];
     return q;
 }
 
@@ -106,16 +109,18 @@ __device__ __forceinline__ long face_index(int nx, int ny, int nz, int i, int j,
     return off + ((long)(side * 5 + f) * nx + i) * ny + j;
 }
 
-// LDS coordinates: cell (row, col) with row in -1..8, col in -1..64 lives at
-// [row+1][col+1].  s_fy[r+1] holds F_y(r, r+1) for r = -1..7.  s_fzl[p][r]
-// holds F_z(k0-1, k0) of row r for the plane of parity p; wave 0 forms it one
-// plane ahead, right after that plane lands in LDS.
+// LDS coordinates: cell (row, col) with row in -1..TJ, col in -1..64 lives
+// at [row+1][col+1] of the plane buffer; planes alternate between two
+// buffers, so each plane needs two barriers: B1 (the plane is complete) and
+// B2 (its F_y and the next plane are complete).  s_fy[r+1] holds F_y(r, r+1)
+// for r = -1..TJ-1.  s_fzl[p][r] holds F_z(k0-1, k0) of row r for the plane
+// of parity p; wave 0 forms it one plane ahead.
 //
 // The three roles (cell rows, the low halo row, the high halo row) run
-// separate loops with the same barrier sequence (B1, B2, B3 per plane), so
-// each role's registers are allocated for that role alone.
-__global__ void __launch_bounds__(NT, 2) hydro_step_kernel(HydroStepArgs a) {
-    __shared__ double s_cell[NF][LR][LC];
+// separate loops with the same barrier sequence, so each role's registers
+// are allocated for that role alone.
+__global__ void __launch_bounds__(NT) hydro_step_kernel(HydroStepArgs a) {
+    __shared__ double s_cell[2][NF][LR][LC];
     __shared__ double s_fy[5][TJ + 1][TK];
     __shared__ double s_fzl[2][5][TJ];
     __shared__ unsigned long long s_smax[NW];
@@ -138,13 +143,13 @@ __global__ void __launch_bounds__(NT, 2) hydro_step_kernel(HydroStepArgs a) {
     const int ib = xc * a.cx;
     const int ie = min(nx, ib + a.cx);
 
-    const int row = wave - 1;  // -1 .. 8
+    const int row = wave - 1;  // -1 .. TJ
     const int j = j0 + row;
     const int k = k0 + lane;
     const bool valid = (j >= 0) && (j < ny) && (k < nz);
 
-    // z halo cells: wave 0 lanes 0..7 load (j0+lane, k0-1), wave NW-1 lanes
-    // 0..7 load (j0+lane, k0+64).
+    // z halo cells: wave 0 lanes 0..TJ-1 load (j0+lane, k0-1), wave NW-1
+    // lanes 0..TJ-1 load (j0+lane, k0+64).
     const bool has_halo = (wave == 0 || wave == NW - 1) && (lane < TJ);
     const int hj = j0 + lane;
     const int hk = (wave == 0) ? k0 - 1 : k0 + TK;
@@ -163,15 +168,15 @@ __global__ void __launch_bounds__(NT, 2) hydro_step_kernel(HydroStepArgs a) {
     const double gamma = a.gamma;
     const EosConst kc = eos_const(gamma);
 
-    // prologue: plane ib into LDS
+    // prologue: plane ib into buffer ib & 1
     {
         Raw r;
         load_raw(a, cidx, valid, r);
-        lds_put(s_cell, row + 1, lane + 1, to_cell(r, gamma));
+        lds_put(s_cell[ib & 1], row + 1, lane + 1, to_cell(r, gamma));
         if (has_halo) {
             Raw h;
             load_raw(a, hidx, hvalid, h);
-            lds_put(s_cell, lane + 1, hcol, to_cell(h, gamma));
+            lds_put(s_cell[ib & 1], lane + 1, hcol, to_cell(h, gamma));
         }
     }
     __syncthreads();
@@ -184,37 +189,38 @@ __global__ void __launch_bounds__(NT, 2) hydro_step_kernel(HydroStepArgs a) {
         if (ib > 0) {  // F_x(ib-1) for the update of plane ib
             Raw pr;
             load_raw(a, cidx - sx, valid, pr);
-            hllc<0>(to_cell(pr, gamma), lds_get(s_cell, row + 1, lane + 1), gamma, fxl);
+            hllc<0>(to_cell(pr, gamma), lds_get(s_cell[ib & 1], row + 1, lane + 1), gamma, fxl);
         }
         for (int i = ib; i < ie; ++i) {
             const long off = (long)(i - ib) * sx;
             const bool has_next = (i + 1 < nx);
+            const int pb = i & 1;
             __syncthreads();  // B1: plane i complete in LDS
+            Raw rn;
+            if (has_next) load_raw(a, cidx + off + sx, valid, rn);  // prefetch plane i+1
 
-            const Cell cur = lds_get(s_cell, row + 1, lane + 1);
+            const Cell cur = lds_get(s_cell[pb], row + 1, lane + 1);
             {
                 double fy[5];
-                hllc<1>(cur, lds_get(s_cell, row + 2, lane + 1), gamma, fy);
+                hllc<1>(cur, lds_get(s_cell[pb], row + 2, lane + 1), gamma, fy);
 #pragma unroll
                 for (int f = 0; f < 5; ++f) s_fy[f][row + 1][lane] = fy[f];
             }
             double dz[5];
             {
                 double fz[5];
-                hllc<2>(cur, lds_get(s_cell, row + 1, lane + 2), gamma, fz);
+                hllc<2>(cur, lds_get(s_cell[pb], row + 1, lane + 2), gamma, fz);
 #pragma unroll
                 for (int f = 0; f < 5; ++f) {
                     double l = __shfl_up(fz[f], 1, 64);
-                    if (lane == 0) l = s_fzl[i & 1][f][row];
+                    if (lane == 0) l = s_fzl[pb][f][row];
                     dz[f] = fz[f] - l;
                 }
             }
             double tx[5] = {0, 0, 0, 0, 0};
-            Cell nxt;
             if (has_next) {
-                Raw rn;
-                load_raw(a, cidx + off + sx, valid, rn);
-                nxt = to_cell(rn, gamma);
+                const Cell nxt = to_cell(rn, gamma);
+                lds_put(s_cell[pb ^ 1], row + 1, lane + 1, nxt);
                 double fxr[5];
                 hllc<0>(cur, nxt, gamma, fxr);
 #pragma unroll
@@ -223,15 +229,12 @@ __global__ void __launch_bounds__(NT, 2) hydro_step_kernel(HydroStepArgs a) {
                     fxl[f] = fxr[f];
                 }
             }
-            __syncthreads();  // B2: all reads of plane i done
-            if (has_next) lds_put(s_cell, row + 1, lane + 1, nxt);
-            __syncthreads();  // B3: F_y of plane i visible
+            __syncthreads();  // B2: F_y of plane i and plane i+1 complete
 
             if (valid) {
                 double q[5] = {cur.d, cur.mx, cur.my, cur.mz, cur.e};
                 const bool interior = (i >= 1) && (i <= nx - 2) && (j >= 1) && (j <= ny - 2) &&
                                       (k >= 1) && (k <= nz - 2);
-                double yq;
                 if (interior) {
                     // hydro_solver_3d.c:738-762: x, then y, then z, left to right
 #pragma unroll
@@ -240,17 +243,15 @@ __global__ void __launch_bounds__(NT, 2) hydro_step_kernel(HydroStepArgs a) {
                         t = t - az * dz[f];
                         q[f] = q[f] + t;
                     }
-                    yq = floor_roundtrip(q[0], q[1], q[2], q[3], q[4], kc);
-                } else {
-                    yq = 1.0 / q[0];  // boundary cells keep their raw density
+                    floor_roundtrip(q[0], q[1], q[2], q[3], q[4], kc);
                 }
                 const long idx = cidx + off;
                 if (a.gravity) {
                     // apply_gravity_source_terms, hydro_solver_3d.c:606-627
                     const double gx = a.g[0][idx], gy = a.g[1][idx], gz = a.g[2][idx];
                     const double rho = q[0];
-                    const double u = div_rcp(q[1], rho, yq), v = div_rcp(q[2], rho, yq),
-                                 w = div_rcp(q[3], rho, yq);
+                    double u, v, w;
+                    velocities(rho, q[1], q[2], q[3], u, v, w);
                     q[1] = q[1] + dt * rho * gx;
                     q[2] = q[2] + dt * rho * gy;
                     q[3] = q[3] + dt * rho * gz;
@@ -260,9 +261,9 @@ __global__ void __launch_bounds__(NT, 2) hydro_step_kernel(HydroStepArgs a) {
                 if (interior) {
 #pragma unroll
                     for (int f = 0; f < 5; ++f) a.uout[f][idx] = q[f];
-                    // interior density is already floored: yq is its reciprocal
+                    // the interior density is at or above the floor here
                     unsigned long long sb =
-                        dbits(signal_speed_rcp(q[0], q[1], q[2], q[3], q[4], yq, kc));
+                        dbits(signal_speed_floored(q[0], q[1], q[2], q[3], q[4], kc));
                     smax = sb > smax ? sb : smax;
                 } else {
 #pragma unroll
@@ -277,56 +278,57 @@ __global__ void __launch_bounds__(NT, 2) hydro_step_kernel(HydroStepArgs a) {
         // F_z(k0-1, k0) of every cell row one plane ahead.
         if (lane < TJ) {
             double fh[5];
-            hllc<2>(lds_get(s_cell, lane + 1, 0), lds_get(s_cell, lane + 1, 1), gamma, fh);
+            hllc<2>(lds_get(s_cell[ib & 1], lane + 1, 0), lds_get(s_cell[ib & 1], lane + 1, 1),
+                    gamma, fh);
 #pragma unroll
             for (int f = 0; f < 5; ++f) s_fzl[ib & 1][f][lane] = fh[f];
         }
         for (int i = ib; i < ie; ++i) {
             const long off = (long)(i - ib) * sx;
             const bool has_next = (i + 1 < nx);
+            const int pb = i & 1;
             __syncthreads();  // B1
+            Raw rn, rh;
+            if (has_next) {
+                load_raw(a, cidx + off + sx, valid, rn);
+                if (has_halo) load_raw(a, hidx + off + sx, hvalid, rh);
+            }
             {
                 double fy[5];
-                hllc<1>(lds_get(s_cell, 0, lane + 1), lds_get(s_cell, 1, lane + 1), gamma, fy);
+                hllc<1>(lds_get(s_cell[pb], 0, lane + 1), lds_get(s_cell[pb], 1, lane + 1), gamma,
+                        fy);
 #pragma unroll
                 for (int f = 0; f < 5; ++f) s_fy[f][0][lane] = fy[f];
             }
-            Raw rn, rh;
             if (has_next) {
-                load_raw(a, cidx + off + sx, valid, rn);
-                if (has_halo) load_raw(a, hidx + off + sx, hvalid, rh);
+                lds_put(s_cell[pb ^ 1], 0, lane + 1, to_cell(rn, gamma));
+                if (has_halo) lds_put(s_cell[pb ^ 1], lane + 1, 0, to_cell(rh, gamma));
             }
-            __syncthreads();  // B2
-            if (has_next) {
-                lds_put(s_cell, 0, lane + 1, to_cell(rn, gamma));
-                if (has_halo) lds_put(s_cell, lane + 1, 0, to_cell(rh, gamma));
-            }
-            __syncthreads();  // B3
+            __syncthreads();  // B2: plane i+1 complete
             if (has_next && lane < TJ) {
                 double fh[5];
-                hllc<2>(lds_get(s_cell, lane + 1, 0), lds_get(s_cell, lane + 1, 1), gamma, fh);
+                hllc<2>(lds_get(s_cell[pb ^ 1], lane + 1, 0), lds_get(s_cell[pb ^ 1], lane + 1, 1),
+                        gamma, fh);
 #pragma unroll
-                for (int f = 0; f < 5; ++f) s_fzl[(i + 1) & 1][f][lane] = fh[f];
+                for (int f = 0; f < 5; ++f) s_fzl[pb ^ 1][f][lane] = fh[f];
             }
         }
     } else {
-        // ------------------------------------------ high halo row (j0+8)
-        // loads row j0+8 and the right z halo of the next plane.
+        // ------------------------------------------ high halo row (j0+TJ)
+        // loads row j0+TJ and the right z halo of the next plane.
         for (int i = ib; i < ie; ++i) {
             const long off = (long)(i - ib) * sx;
             const bool has_next = (i + 1 < nx);
+            const int pb = i & 1;
             __syncthreads();  // B1
-            Raw rn, rh;
             if (has_next) {
+                Raw rn, rh;
                 load_raw(a, cidx + off + sx, valid, rn);
                 if (has_halo) load_raw(a, hidx + off + sx, hvalid, rh);
+                lds_put(s_cell[pb ^ 1], LR - 1, lane + 1, to_cell(rn, gamma));
+                if (has_halo) lds_put(s_cell[pb ^ 1], lane + 1, LC - 1, to_cell(rh, gamma));
             }
             __syncthreads();  // B2
-            if (has_next) {
-                lds_put(s_cell, LR - 1, lane + 1, to_cell(rn, gamma));
-                if (has_halo) lds_put(s_cell, lane + 1, LC - 1, to_cell(rh, gamma));
-            }
-            __syncthreads();  // B3
         }
     }
     // one atomic per block for the signal speed of the updated interior
@@ -334,7 +336,7 @@ __global__ void __launch_bounds__(NT, 2) hydro_step_kernel(HydroStepArgs a) {
     if (threadIdx.x == 0) {
         unsigned long long m = 0ull;
         for (int w = 1; w <= TJ; ++w) m = s_smax[w] > m ? s_smax[w] : m;
-        if (m != 0ull) atomicMax(a.ms_out, m);
+        if (m != 0ull && m > __atomic_load_n(a.ms_out, __ATOMIC_RELAXED)) atomicMax(a.ms_out, m);
     }
 }
 
@@ -414,7 +416,8 @@ __global__ void __launch_bounds__(256) hydro_bc_kernel(HydroBCArgs a) {
         smax = sb > smax ? sb : smax;
     }
     smax = block_max_u64(smax);
-    if (threadIdx.x == 0 && smax != 0ull) atomicMax(a.ms_out, smax);
+    if (threadIdx.x == 0 && smax != 0ull && smax > __atomic_load_n(a.ms_out, __ATOMIC_RELAXED))
+        atomicMax(a.ms_out, smax);
 }
 
 // Signal speed of a whole state (after an upload).
@@ -432,7 +435,8 @@ __global__ void __launch_bounds__(256) hydro_maxspeed_kernel(const double* __res
         smax = sb > smax ? sb : smax;
     }
     smax = block_max_u64(smax);
-    if (threadIdx.x == 0 && smax != 0ull) atomicMax(out, smax);
+    if (threadIdx.x == 0 && smax != 0ull && smax > __atomic_load_n(out, __ATOMIC_RELAXED))
+        atomicMax(out, smax);
 }
 
 // AoS (host layout, 5 doubles per cell) <-> SoA planes.
@@ -490,7 +494,7 @@ hipError_t launch_hydro_step(const HydroStepArgs& a, hipStream_t st) {
 hipError_t launch_hydro_bc(const HydroBCArgs& a, hipStream_t st) {
     const long tot = 2L * a.ny * a.nz + 2L * (a.nx - 2) * a.nz + 2L * (a.nx - 2) * (a.ny - 2);
     long nb = (tot + 255) / 256;
-    if (nb > 512) nb = 512;  // grid stride: few blocks, one atomic each
+    if (nb > 256) nb = 256;  // grid stride: few blocks, one atomic each
     hipLaunchKernelGGL(hydro_bc_kernel, dim3((unsigned)nb), dim3(256), 0, st, a);
     return hipGetLastError();
 }
