@@ -26,6 +26,22 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "zone-cycles/s @256^3; Poisson ms to 1e-8 resid: plain SOR vs NN-guess+FP32"
+# HBM bytes per launch of the step kernel from rocprofv3 FETCH_SIZE / WRITE_SIZE
+# runs scaled by a copy-kernel calibration (written next to the summary)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles", "hydro_step_256_traffic.json")
+
+
+def measured_traffic(n):
+    try:
+        with open(TRAFFIC_FILE) as f:
+            t = json.load(f)
+    except (OSError, ValueError):
+        return None
+    if t.get("cells") != n ** 3:
+        return None
+    return t
+
+
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_ZONE = 80.0
 
@@ -131,9 +147,9 @@ def main():
                          "per cell) used to scale HBM byte counters, then exit")
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the slab "
-              f"decomposition across GPUs is described in DESIGN.md, not built)",
-              file=sys.stderr)
+        print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path "
+              f"would shard as x slabs with halo planes and a dt all-reduce; that is not "
+              f"built)", file=sys.stderr)
         sys.exit(2)
 
     import hydrocode_astro1_amd as H
@@ -181,6 +197,16 @@ def main():
                      "kernel": "hydro_step_kernel", "kernel_ms": step_ms,
                      "boundary_kernel_ms": bc_ms, "bytes_per_zone": BYTES_PER_ZONE},
     }
+    t = measured_traffic(n)
+    if t is not None:
+        rf = res["roofline"]
+        rf["traffic"] = t["traffic_bytes"]
+        rf["traffic_source"] = "profiles/hydro_step_256.md (rocprofv3 FETCH_SIZE + WRITE_SIZE)"
+        if "valu_busy" in t:
+            # the step kernel issues FP64 VALU work at this fraction of the
+            # wave64 issue rate (one instruction per 4 cycles per SIMD)
+            rf["valu_per_zone"] = t["valu_per_cell"]
+            rf["valu_busy"] = t["valu_busy"]
     if not args.no_poisson:
         try:
             res["poisson"] = poisson_block(H, ics, n)

@@ -25,11 +25,12 @@ LIB_PATH = os.environ.get("HYDRO_MI355X_LIB") or os.path.join(_HERE, "libhydro_m
 
 BC_PERIODIC, BC_REFLECTIVE, BC_OUTFLOW = 0, 1, 2
 GRAV_BC_PERIODIC, GRAV_BC_ISOLATED, GRAV_BC_DIRICHLET = 0, 1, 2
-POISSON_FP64_SOR, POISSON_FP32_DC = 0, 1
+POISSON_FP64_SOR, POISSON_FP32_DC, POISSON_NN_FP32_DC = 0, 1, 2
 SOR_CRIT_MAX_UPDATE, SOR_CRIT_REL_RESID = 0, 1
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
+_fp = C.POINTER(C.c_float)
 
 
 class GravitySolver3D(C.Structure):
@@ -49,6 +50,8 @@ class GravitySolver3D(C.Structure):
         ("multipole_lmax", C.c_int), ("poisson_mode", C.c_int), ("criterion", C.c_int),
         ("last_status", C.c_int),
         ("last_rel_residual", C.c_double), ("last_solve_ms", C.c_double),
+        ("nn_model", C.c_void_p), ("last_guess", C.c_int),
+        ("last_guess_rel_residual", C.c_double), ("last_warm_rel_residual", C.c_double),
     ]
 
 
@@ -120,6 +123,26 @@ EXPORTS = [
     ("gravity_boundary_potential", C.c_int, [_GP, _dp, _dp]),
     ("gravity_rel_residual", C.c_int, [_GP, _dp, _dp]),
     ("gravity_solve_device", C.c_int, [_GP, C.c_void_p]),
+    ("gravity_set_nn_model", C.c_int, [_GP, C.c_void_p]),
+    ("nn_model_create", C.c_void_p, [C.c_ulonglong]),
+    ("nn_model_destroy", None, [C.c_void_p]),
+    ("nn_model_num_params", C.c_long, [C.c_void_p]),
+    ("nn_model_param_len", C.c_long, [C.c_void_p]),
+    ("nn_model_param_offset", C.c_long, [C.c_void_p, C.c_int, C.c_int]),
+    ("nn_model_save", C.c_int, [C.c_void_p, C.c_char_p]),
+    ("nn_model_load", C.c_int, [C.c_void_p, C.c_char_p]),
+    ("nn_model_get_params", C.c_int, [C.c_void_p, _fp]),
+    ("nn_model_set_params", C.c_int, [C.c_void_p, _fp]),
+    ("nn_model_forward_backward", C.c_int,
+     [C.c_void_p, _fp, _fp, C.c_int, _dp, C.c_double, _fp, _fp, _fp, _fp]),
+    ("nn_dataset_create", C.c_void_p, [C.c_int, C.c_int]),
+    ("nn_dataset_destroy", None, [C.c_void_p]),
+    ("nn_dataset_fill", C.c_int,
+     [C.c_void_p, C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_double, C.c_double, _GP]),
+    ("nn_dataset_get", C.c_int, [C.c_void_p, C.c_int, _fp, _fp, _dp]),
+    ("nn_train", C.c_int, [C.c_void_p, C.c_void_p, _ip, C.c_int, C.c_int, C.c_float, _fp]),
+    ("nn_evaluate", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, _fp]),
+    ("nn_probe_mfma", C.c_int, [_fp]),
 ]
 
 _lib = None
@@ -194,6 +217,12 @@ class Gravity:
     def set_mode(self, mode):
         if self.L.gravity_set_poisson_mode(self.h, mode) != 0:
             raise ValueError(f"bad mode {mode}")
+
+    def set_nn_model(self, model):
+        """Attach an nn.Model for POISSON_NN_FP32_DC (None detaches)."""
+        self._nn = model
+        _check(self.L.gravity_set_nn_model(self.h, model.h if model is not None else None),
+               "gravity_set_nn_model")
 
     def set_criterion(self, crit):
         if self.L.gravity_set_criterion(self.h, crit) != 0:

@@ -44,7 +44,7 @@ void dev_free(GravDevice* d) {
     if (!d) return;
     void* ptrs[] = {d->phi_r, d->phi_b, d->rhs_r, d->rhs_b, d->gx,  d->gy, d->gz,
                     d->rho_tmp, d->nat_tmp, d->mom, d->mom_scratch, d->wlm, d->ctl, d->acc,
-                    d->e_r, d->e_b, d->r_r, d->r_b};
+                    d->e_r, d->e_b, d->r_r, d->r_b, d->guess_r, d->guess_b};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (d->ctl_host) (void)hipHostFree(d->ctl_host);
@@ -273,6 +273,42 @@ int sor_fp32_dc(GravitySolver3D* s) {
     return rel <= tol ? 0 : -1;
 }
 
+// Starting potential of POISSON_NN_FP32_DC: the network's guess replaces
+// the warm start (the potential left by the previous solve) only when its
+// relative residual, with this density's boundary values, is smaller.
+int choose_guess(GravitySolver3D* s, const double* d_rho) {
+    GravDevice* d = s->dev;
+    s->last_guess = -1;
+    s->last_guess_rel_residual = -1.0;
+    s->last_warm_rel_residual = -1.0;
+    const bool cubic = s->nx == s->ny && s->ny == s->nz && s->dx == s->dy && s->dy == s->dz;
+    if (!s->nn_model || !cubic || s->bc_type != GRAV_BC_ISOLATED) return 0;
+    if (!d->guess_r && (dev_alloc(&d->guess_r, d->nsplit) || dev_alloc(&d->guess_b, d->nsplit)))
+        return -2;
+    if (!d->nat_tmp && dev_alloc(&d->nat_tmp, d->n)) return -2;
+    const double box[4] = {s->xmin, s->ymin, s->zmin, s->xmax - s->xmin};
+    const int rc = nn_guess_device(s->nn_model, d_rho, s->nx, d->mom, s->G, box, d->nat_tmp);
+    if (rc == -1) return 0;
+    if (rc) return -2;
+    double rw, rn;
+    if (apply_boundary(s) || rel_residual_dev(s, nullptr, nullptr, &rw)) return -2;
+    HYD_CHECK(hyd::launch_natural_to_split(d->g, d->nat_tmp, 1.0, d->guess_r, d->guess_b,
+                                           d->stream));
+    std::swap(d->phi_r, d->guess_r);
+    std::swap(d->phi_b, d->guess_b);
+    if (apply_boundary(s) || rel_residual_dev(s, nullptr, nullptr, &rn)) return -2;
+    s->last_warm_rel_residual = rw;
+    s->last_guess_rel_residual = rn;
+    if (rn < rw) {
+        s->last_guess = 1;
+    } else {
+        std::swap(d->phi_r, d->guess_r);
+        std::swap(d->phi_b, d->guess_b);
+        s->last_guess = 0;
+    }
+    return 0;
+}
+
 int potential_device(GravitySolver3D* s, const double* d_rho) {
     GravDevice* d = s->dev;
     HYD_CHECK(hipEventRecord(d->ev0, d->stream));
@@ -281,7 +317,13 @@ int potential_device(GravitySolver3D* s, const double* d_rho) {
     if (s->bc_type == GRAV_BC_ISOLATED)
         HYD_CHECK(hyd::launch_moments(d->g, d_rho, s->multipole_lmax, d->mom_scratch, d->mom,
                                       d->stream));
-    int st = (s->poisson_mode == POISSON_FP32_DC) ? sor_fp32_dc(s) : sor_fp64(s);
+    int st;
+    if (s->poisson_mode == POISSON_FP64_SOR) {
+        st = sor_fp64(s);
+    } else {
+        if (s->poisson_mode == POISSON_NN_FP32_DC && choose_guess(s, d_rho)) return -2;
+        st = sor_fp32_dc(s);
+    }
     HYD_CHECK(hipEventRecord(d->ev1, d->stream));
     HYD_CHECK(hipEventSynchronize(d->ev1));
     float ms = 0.0f;
@@ -378,6 +420,7 @@ GravitySolver3D* gravity_solver_create(int nx, int ny, int nz, double xmin, doub
 
 void gravity_solver_destroy(GravitySolver3D* s) {
     if (!s) return;
+    if (s->dev && s->dev->owns_nn) nn_model_destroy(s->nn_model);
     free(s->phi);
     free(s->gx);
     free(s->gy);
@@ -456,8 +499,18 @@ int gravity_set_multipole_lmax(GravitySolver3D* s, int lmax) {
 }
 
 int gravity_set_poisson_mode(GravitySolver3D* s, int mode) {
-    if (mode != POISSON_FP64_SOR && mode != POISSON_FP32_DC) return -1;
+    if (mode != POISSON_FP64_SOR && mode != POISSON_FP32_DC && mode != POISSON_NN_FP32_DC)
+        return -1;
     s->poisson_mode = mode;
+    return 0;
+}
+
+int gravity_set_nn_model(GravitySolver3D* s, void* model) {
+    if (s->dev->owns_nn && s->nn_model != model) {
+        nn_model_destroy(s->nn_model);
+        s->dev->owns_nn = false;
+    }
+    s->nn_model = model;
     return 0;
 }
 

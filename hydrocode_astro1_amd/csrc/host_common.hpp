@@ -46,6 +46,8 @@ struct GravDevice {
     double *acc = nullptr;            // residual sums (device)
     double *acc_host = nullptr;       // pinned readback
     float *e_r = nullptr, *e_b = nullptr, *r_r = nullptr, *r_b = nullptr;
+    double *guess_r = nullptr, *guess_b = nullptr;  // learned guess (split layout)
+    bool owns_nn = false;  // the network was created from an input deck
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool g_valid = false;  // device g has been computed at least once
@@ -72,6 +74,11 @@ struct HydroDevice {
     bool skip_gravity = false;  // gravity already solved by the caller
     static constexpr int DT_LOG = 1 << 16;
 };
+
+// Learned potential guess of a cubic device density (nn_api.hip); -1 when
+// the grid does not fit the network.  box = x0, y0, z0, L.
+int nn_guess_device(void *model, const double *d_rho, int n, const double *d_mom, double G,
+                    const double box[4], double *d_phi_nat);
 
 // Internal interface between the hydro driver and the gravity solver.
 struct GravitySolver3D;

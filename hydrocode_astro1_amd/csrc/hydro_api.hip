@@ -342,6 +342,19 @@ GodunovHLLC3D* solver3d_create_from_deck(const InputDeck* deck) {
         gravity_set_multipole_lmax(s->gravity_solver, deck->multipole_lmax);
         gravity_set_poisson_mode(s->gravity_solver, deck->poisson_mode);
         gravity_set_criterion(s->gravity_solver, deck->sor_criterion);
+        if (deck->nn_guess[0]) {
+            void* m = nn_model_create(0);
+            if (m && nn_model_load(m, deck->nn_guess) == 0) {
+                gravity_set_nn_model(s->gravity_solver, m);
+                s->gravity_solver->dev->owns_nn = true;
+            } else {
+                fprintf(stderr,
+                        "Warning: cannot load the guess network %s; the Poisson solve starts "
+                        "from the previous potential.\n",
+                        deck->nn_guess);
+                nn_model_destroy(m);
+            }
+        }
     }
     return s;
 }

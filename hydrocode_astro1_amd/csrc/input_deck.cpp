@@ -180,8 +180,12 @@ InputDeck* input_deck_create(const char* filename) {
         if (strcmp(k, "bc_zmin") == 0) { d->bc_zmin = parse_bc_type(v); continue; }
         if (strcmp(k, "bc_zmax") == 0) { d->bc_zmax = parse_bc_type(v); continue; }
         if (strcmp(k, "poisson_mode") == 0) {
-            d->poisson_mode = (strcmp(v, "fp32dc") == 0 || strcmp(v, "1") == 0) ? POISSON_FP32_DC
-                                                                                : POISSON_FP64_SOR;
+            if (strcmp(v, "fp32dc") == 0 || strcmp(v, "1") == 0)
+                d->poisson_mode = POISSON_FP32_DC;
+            else if (strcmp(v, "nn_fp32dc") == 0 || strcmp(v, "2") == 0)
+                d->poisson_mode = POISSON_NN_FP32_DC;
+            else
+                d->poisson_mode = POISSON_FP64_SOR;
             continue;
         }
         if (strcmp(k, "sor_criterion") == 0) {
@@ -228,7 +232,9 @@ void input_deck_print(const InputDeck* d) {
            bc_name(d->bc_zmin), bc_name(d->bc_zmax));
     printf("\nPoisson:\n  omega = %.3f, max_iter = %d, tol = %.3e, lmax = %d, mode = %s\n",
            d->sor_omega, d->sor_max_iter, d->sor_tol, d->multipole_lmax,
-           d->poisson_mode == POISSON_FP32_DC ? "fp32dc" : "fp64");
+           d->poisson_mode == POISSON_FP32_DC      ? "fp32dc"
+           : d->poisson_mode == POISSON_NN_FP32_DC ? "nn_fp32dc"
+                                                   : "fp64");
     printf("\nOutput:\n  Directory: %s\n", d->output_dir);
     printf("==============================================\n\n");
 }

@@ -1,0 +1,659 @@
+// nn_kernels.hip -- kernels of the learned initial guess for the Poisson
+// solve: BF16 3x3x3 convolutions on MFMA (forward, input gradient through
+// the same kernel with the adjoint weights, weight gradient), pooling, the
+// loss and Adam.  No reference counterpart: the reference starts SOR from the
+// previous potential or from zero (gravity_solver_3d.c:87, :566).
+//
+// MFMA operand layout of v_mfma_f32_16x16x16bf16_1k (wave64): lane l holds
+// A[l % 16][4 (l / 16) + i], B[4 (l / 16) + i][l % 16] for i = 0..3 and
+// receives D[4 (l / 16) + i][l % 16]; probe_mfma checks this on the device.
+#include "nn_kernels.hpp"
+
+namespace hyd {
+namespace nn {
+
+typedef short v4s __attribute__((ext_vector_type(4)));
+typedef float v4f __attribute__((ext_vector_type(4)));
+
+__device__ __forceinline__ uint16_t f2bf(float f) {
+    uint32_t u = __float_as_uint(f);
+    if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;  // NaN
+    u += 0x7fffu + ((u >> 16) & 1u);                       // round to nearest even
+    return (uint16_t)(u >> 16);
+}
+__device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+__device__ __forceinline__ void tap_offset(int t, int& dx, int& dy, int& dz) {
+    dx = t / 9 - 1;
+    dy = (t / 3) % 3 - 1;
+    dz = t % 3 - 1;
+}
+
+// ------------------------------------------------------------- forward --
+// One wavefront computes 64 consecutive voxels (four 16-voxel M tiles) for
+// all output channels (one or two 16-channel N tiles).
+template <int CINP>
+__global__ void __launch_bounds__(256) conv_fwd_kernel(ConvArgs a) {
+    constexpr int KSTEPS = (27 * CINP + 15) / 16;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int n = a.n;
+    const long nvox = (long)n * n * n;
+    const long vbase = ((long)blockIdx.x * 4 + wave) * 64;
+    if (vbase >= nvox) return;
+    const int ntiles = (a.cout + 15) / 16;
+    const int nin = a.up ? n / 2 : n;
+    const int g = lane >> 4, m = lane & 15;
+
+    int vi[4], vj[4], vk[4];
+    bool vin[4];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt) {
+        const long v = vbase + mt * 16 + m;
+        vin[mt] = v < nvox;
+        vk[mt] = (int)(v % n);
+        vj[mt] = (int)((v / n) % n);
+        vi[mt] = (int)(v / ((long)n * n));
+    }
+    v4f acc[4][2];
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+        for (int nt = 0; nt < 2; ++nt) acc[mt][nt] = v4f{0.f, 0.f, 0.f, 0.f};
+
+    const v4s* wp = reinterpret_cast<const v4s*>(a.wp);
+    for (int ks = 0; ks < KSTEPS; ++ks) {
+        const int kk = (ks * 4 + g) * 4;
+        const int t = kk / CINP, ci = kk % CINP;
+        int dx = 0, dy = 0, dz = 0;
+        if (t < 27) tap_offset(t, dx, dy, dz);
+        v4s b0 = wp[(0 * KSTEPS + ks) * 64 + lane];
+        v4s b1 = v4s{0, 0, 0, 0};
+        if (ntiles > 1) b1 = wp[(1 * KSTEPS + ks) * 64 + lane];
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt) {
+            v4s av = v4s{0, 0, 0, 0};
+            const int ii = vi[mt] + dx, jj = vj[mt] + dy, kz = vk[mt] + dz;
+            if (t < 27 && vin[mt] && ii >= 0 && ii < n && jj >= 0 && jj < n && kz >= 0 && kz < n) {
+                const int si = a.up ? ii >> 1 : ii, sj = a.up ? jj >> 1 : jj,
+                          sk = a.up ? kz >> 1 : kz;
+                const long idx = (((long)si * nin + sj) * nin + sk) * CINP + ci;
+                av = *reinterpret_cast<const v4s*>(a.x + idx);
+            }
+            acc[mt][0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, b0, acc[mt][0], 0, 0, 0);
+            if (ntiles > 1)
+                acc[mt][1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, b1, acc[mt][1], 0, 0, 0);
+        }
+    }
+    // epilogue: bias, ReLU, optional skip add, stores
+#pragma unroll
+    for (int mt = 0; mt < 4; ++mt)
+        for (int nt = 0; nt < 2; ++nt) {
+            if (nt >= ntiles) break;
+            const int c = nt * 16 + m;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const long v = vbase + mt * 16 + g * 4 + i;
+                if (v >= nvox || c >= a.coutp) continue;
+                float val = acc[mt][nt][i];
+                if (c < a.cout) {
+                    if (a.bias) val += a.bias[c];
+                    if (a.relu) val = val > 0.f ? val : 0.f;
+                } else {
+                    val = 0.f;
+                }
+                const uint16_t rb = f2bf(val);
+                if (a.yr) a.yr[v * a.coutp + c] = rb;
+                if (a.yd) {
+                    const float s = a.skip ? bf2f(a.skip[v * a.coutp + c]) : 0.f;
+                    a.yd[v * a.coutp + c] = f2bf(bf2f(rb) + s);
+                }
+                if (a.yf && c == 0) a.yf[v] = a.add ? val + a.add[v] : val;
+            }
+        }
+}
+
+hipError_t conv_forward(const ConvArgs& a, hipStream_t st) {
+    if (a.cout < 1 || a.cout > 32 || a.coutp < a.cout || a.n < 1 || (a.up && a.n % 2))
+        return hipErrorInvalidValue;  // one wavefront covers at most two 16-channel tiles
+    const long nvox = (long)a.n * a.n * a.n;
+    const unsigned nb = (unsigned)((nvox + 255) / 256);
+    switch (a.cinp) {
+        case 4: hipLaunchKernelGGL(conv_fwd_kernel<4>, dim3(nb), dim3(256), 0, st, a); break;
+        case 8: hipLaunchKernelGGL(conv_fwd_kernel<8>, dim3(nb), dim3(256), 0, st, a); break;
+        case 12: hipLaunchKernelGGL(conv_fwd_kernel<12>, dim3(nb), dim3(256), 0, st, a); break;
+        case 16: hipLaunchKernelGGL(conv_fwd_kernel<16>, dim3(nb), dim3(256), 0, st, a); break;
+        case 24: hipLaunchKernelGGL(conv_fwd_kernel<24>, dim3(nb), dim3(256), 0, st, a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// -------------------------------------------------------- weight gradient --
+// dW[t][ci][co] = sum_v x[v + off_t][ci] g[v][co]: rows (t, ci) x columns co,
+// reduced over voxels 16 at a time.  Block = one wavefront = one 16 x 16 tile
+// over one chunk of voxels; partial tiles are added atomically.
+template <int CINP>
+__global__ void __launch_bounds__(64) conv_wgrad_kernel(WgradArgs a) {
+    const int lane = threadIdx.x & 63;
+    const int g4 = lane >> 4, m = lane & 15;
+    const int n = a.n;
+    const long nvox = (long)n * n * n;
+    const int nin = a.up ? n / 2 : n;
+    const int rt = blockIdx.y, ct = blockIdx.z;
+    const int row = rt * 16 + m;  // A row of this lane
+    const int t = row / CINP, ci = row % CINP;
+    int dx = 0, dy = 0, dz = 0;
+    if (t < 27) tap_offset(t, dx, dy, dz);
+    const int col = ct * 16 + m;  // B column of this lane
+    const long v0 = (long)blockIdx.x * a.vchunk;
+    const long v1 = min(nvox, v0 + a.vchunk);
+    v4f acc = v4f{0.f, 0.f, 0.f, 0.f};
+    for (long vb = v0; vb < v1; vb += 16) {
+        v4s av, bv;
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const long v = vb + g4 * 4 + i;
+            short aval = 0, bval = 0;
+            if (v < v1) {
+                if (t < 27) {
+                    const int vk = (int)(v % n), vj = (int)((v / n) % n),
+                              vi = (int)(v / ((long)n * n));
+                    const int ii = vi + dx, jj = vj + dy, kz = vk + dz;
+                    if (ii >= 0 && ii < n && jj >= 0 && jj < n && kz >= 0 && kz < n) {
+                        const int si = a.up ? ii >> 1 : ii, sj = a.up ? jj >> 1 : jj,
+                                  sk = a.up ? kz >> 1 : kz;
+                        aval = (short)a.x[(((long)si * nin + sj) * nin + sk) * CINP + ci];
+                    }
+                }
+                if (col < a.cout) bval = (short)a.g[v * a.coutp + col];
+            }
+            av[i] = aval;
+            bv[i] = bval;
+        }
+        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int r = rt * 16 + g4 * 4 + i;  // D row held by this lane
+        if (r < 27 * CINP && col < a.cout && acc[i] != 0.f)
+            atomicAdd(&a.dw[(long)r * a.cout + col], acc[i]);
+    }
+    // bias gradient from the first row tile
+    if (a.db && rt == 0) {
+        float s = 0.f;
+        if (col < a.cout)
+            for (long v = v0 + g4; v < v1; v += 4) s += bf2f(a.g[v * a.coutp + col]);
+        s += __shfl_xor(s, 16, 64);
+        s += __shfl_xor(s, 32, 64);
+        if (g4 == 0 && col < a.cout && s != 0.f) atomicAdd(&a.db[col], s);
+    }
+}
+
+hipError_t conv_wgrad(const WgradArgs& a, hipStream_t st) {
+    const long nvox = (long)a.n * a.n * a.n;
+    const unsigned nchunk = (unsigned)((nvox + a.vchunk - 1) / a.vchunk);
+    const unsigned rtiles = (unsigned)((27 * a.cinp + 15) / 16);
+    const unsigned ctiles = (unsigned)((a.cout + 15) / 16);
+    dim3 grid(nchunk, rtiles, ctiles);
+    switch (a.cinp) {
+        case 4: hipLaunchKernelGGL(conv_wgrad_kernel<4>, grid, dim3(64), 0, st, a); break;
+        case 8: hipLaunchKernelGGL(conv_wgrad_kernel<8>, grid, dim3(64), 0, st, a); break;
+        case 12: hipLaunchKernelGGL(conv_wgrad_kernel<12>, grid, dim3(64), 0, st, a); break;
+        case 16: hipLaunchKernelGGL(conv_wgrad_kernel<16>, grid, dim3(64), 0, st, a); break;
+        case 24: hipLaunchKernelGGL(conv_wgrad_kernel<24>, grid, dim3(64), 0, st, a); break;
+        default: return hipErrorInvalidValue;
+    }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------- packing --
+// w: FP32 [27][cinp][cout] (padded input channels stay zero).  Fragment
+// (ntile, kstep, lane, i) holds B[k][col] with k = 16 kstep + 4 (lane/16) + i,
+// col = 16 ntile + lane % 16.  The adjoint (transposed = 1) is the
+// convolution mapping output-channel gradients to input-channel gradients:
+// taps mirrored (t -> 26 - t), input and output channels swapped.
+__global__ void pack_kernel(const float* __restrict__ w, int cin, int cout, int transposed,
+                            uint16_t* __restrict__ wp) {
+    const int cinp = (cin + 3) & ~3;
+    const int kin = transposed ? cout : cin;            // channels read by the conv
+    const int kinp = (kin + 3) & ~3;
+    const int kout = transposed ? cin : cout;           // channels produced
+    const int ksteps = (27 * kinp + 15) / 16;
+    const int ntiles = (kout + 15) / 16;
+    const long total = (long)ntiles * ksteps * 64 * 4;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (long)gridDim.x * blockDim.x) {
+        const int i = (int)(e & 3);
+        const int lane = (int)((e >> 2) & 63);
+        const long r = e >> 8;
+        const int ks = (int)(r % ksteps), nt = (int)(r / ksteps);
+        const int k = ks * 16 + 4 * (lane >> 4) + i;
+        const int col = nt * 16 + (lane & 15);
+        const int t = k / kinp, c = k % kinp;
+        float val = 0.f;
+        if (t < 27 && c < kin && col < kout) {
+            if (!transposed)
+                val = w[((long)t * cinp + c) * cout + col];
+            else
+                val = w[((long)(26 - t) * cinp + col) * cout + c];
+        }
+        wp[e] = f2bf(val);
+    }
+}
+
+hipError_t pack_weights(const float* w, int cin, int cout, int transposed, uint16_t* wp,
+                        hipStream_t st) {
+    hipLaunchKernelGGL(pack_kernel, dim3(64), dim3(256), 0, st, w, cin, cout, transposed, wp);
+    return hipGetLastError();
+}
+
+// ------------------------------------------------------------ elementwise --
+__global__ void avgpool2_kernel(const uint16_t* __restrict__ x, int n, int cp,
+                                uint16_t* __restrict__ y) {
+    const int nc = n / 2;
+    const long total = (long)nc * nc * nc * cp;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (long)gridDim.x * blockDim.x) {
+        const int c = (int)(e % cp);
+        const long v = e / cp;
+        const int k = (int)(v % nc), j = (int)((v / nc) % nc), i = (int)(v / ((long)nc * nc));
+        float s = 0.f;
+        for (int d = 0; d < 8; ++d) {
+            const long fi = 2 * i + (d >> 2), fj = 2 * j + ((d >> 1) & 1), fk = 2 * k + (d & 1);
+            s += bf2f(x[((fi * n + fj) * n + fk) * cp + c]);
+        }
+        y[e] = f2bf(0.125f * s);
+    }
+}
+
+__global__ void sumpool2_kernel(const uint16_t* __restrict__ g, int n, int cp,
+                                uint16_t* __restrict__ out, int accumulate) {
+    const int nc = n / 2;
+    const long total = (long)nc * nc * nc * cp;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (long)gridDim.x * blockDim.x) {
+        const int c = (int)(e % cp);
+        const long v = e / cp;
+        const int k = (int)(v % nc), j = (int)((v / nc) % nc), i = (int)(v / ((long)nc * nc));
+        float s = accumulate ? bf2f(out[e]) : 0.f;
+        for (int d = 0; d < 8; ++d) {
+            const long fi = 2 * i + (d >> 2), fj = 2 * j + ((d >> 1) & 1), fk = 2 * k + (d & 1);
+            s += bf2f(g[((fi * n + fj) * n + fk) * cp + c]);
+        }
+        out[e] = f2bf(s);
+    }
+}
+
+__global__ void unpool_add_kernel(const uint16_t* __restrict__ g, int n, int cp,
+                                  uint16_t* __restrict__ out) {
+    const int nc = n / 2;
+    const long total = (long)n * n * n * cp;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (long)gridDim.x * blockDim.x) {
+        const int c = (int)(e % cp);
+        const long v = e / cp;
+        const int k = (int)(v % n), j = (int)((v / n) % n), i = (int)(v / ((long)n * n));
+        const long cv = (((long)(i >> 1) * nc + (j >> 1)) * nc + (k >> 1)) * cp + c;
+        out[e] = f2bf(bf2f(out[e]) + 0.125f * bf2f(g[cv]));
+    }
+}
+
+__global__ void relu_mask_kernel(const uint16_t* __restrict__ g, const uint16_t* __restrict__ r,
+                                 long count, uint16_t* __restrict__ out) {
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < count;
+         e += (long)gridDim.x * blockDim.x)
+        out[e] = (bf2f(r[e]) > 0.f) ? g[e] : (uint16_t)0;
+}
+
+__global__ void add_kernel(uint16_t* __restrict__ a, const uint16_t* __restrict__ b, long count) {
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < count;
+         e += (long)gridDim.x * blockDim.x)
+        a[e] = f2bf(bf2f(a[e]) + bf2f(b[e]));
+}
+
+__global__ void copy_kernel(const uint16_t* __restrict__ a, long count, uint16_t* __restrict__ o) {
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < count;
+         e += (long)gridDim.x * blockDim.x)
+        o[e] = a[e];
+}
+
+// sums over a block: wave shuffles, then one atomic per block
+__device__ __forceinline__ void block_add2(float s, float s2, float* acc) {
+    __shared__ float part[2][4];
+    for (int o = 32; o > 0; o >>= 1) {
+        s += __shfl_xor(s, o, 64);
+        s2 += __shfl_xor(s2, o, 64);
+    }
+    const int w = threadIdx.x >> 6;
+    if ((threadIdx.x & 63) == 0) {
+        part[0][w] = s;
+        part[1][w] = s2;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        float a = 0.f, b = 0.f;
+        for (int i = 0; i < (int)(blockDim.x >> 6); ++i) {
+            a += part[0][i];
+            b += part[1][i];
+        }
+        atomicAdd(&acc[0], a);
+        atomicAdd(&acc[1], b);
+    }
+}
+
+template <class T>
+__global__ void __launch_bounds__(256) log_stats_kernel(const T* __restrict__ rho, long nvox,
+                                                        float* acc) {
+    float s = 0.f, s2 = 0.f;
+    for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < nvox;
+         v += (long)gridDim.x * blockDim.x) {
+        const float l = log10f(fmaxf((float)rho[v], 1e-12f));
+        s += l;
+        s2 += l * l;
+    }
+    block_add2(s, s2, acc);
+}
+
+template <class T>
+__global__ void make_features_kernel(const T* __restrict__ rho, int n,
+                                     const float* __restrict__ acc,
+                                     const double* __restrict__ mom, double x0, double y0,
+                                     double z0, double L, uint16_t* __restrict__ x,
+                                     float* __restrict__ mono) {
+    const long nvox = (long)n * n * n;
+    const float mu = acc[0] / (float)nvox;
+    const float var = fmaxf(acc[1] / (float)nvox - mu * mu, 1e-12f);
+    const float inv = 1.0f / sqrtf(var);
+    // centre of mass in box units
+    const float cx = (float)((mom[1] - x0) / L), cy = (float)((mom[2] - y0) / L),
+                cz = (float)((mom[3] - z0) / L);
+    const float h = 1.0f / (float)n, eps2 = (1.5f * h) * (1.5f * h);
+    for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < nvox;
+         v += (long)gridDim.x * blockDim.x) {
+        const int k = (int)(v % n), j = (int)((v / n) % n), i = (int)(v / ((long)n * n));
+        const float l = log10f(fmaxf((float)rho[v], 1e-12f));
+        const float dx = (i + 0.5f) * h - cx, dy = (j + 0.5f) * h - cy, dz = (k + 0.5f) * h - cz;
+        const float m = -1.0f / sqrtf(dx * dx + dy * dy + dz * dz + eps2);
+        mono[v] = m;
+        x[4 * v] = f2bf((l - mu) * inv);
+        x[4 * v + 1] = f2bf(0.1f * m);
+        x[4 * v + 2] = 0;
+        x[4 * v + 3] = 0;
+    }
+}
+
+__global__ void __launch_bounds__(256) mse_kernel(const float* __restrict__ p,
+                                                  const float* __restrict__ t, long nvox,
+                                                  float scale, uint16_t* __restrict__ g,
+                                                  float* loss) {
+    float s = 0.f;
+    for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < nvox;
+         v += (long)gridDim.x * blockDim.x) {
+        const float d = p[v] - t[v];
+        s += d * d;
+        g[4 * v] = f2bf(2.0f * d * scale);
+        g[4 * v + 1] = 0;
+        g[4 * v + 2] = 0;
+        g[4 * v + 3] = 0;
+    }
+    __shared__ float part[4];
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) part[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) atomicAdd(loss, (part[0] + part[1] + part[2] + part[3]) * scale);
+}
+
+__global__ void adam_kernel(float* __restrict__ w, const float* __restrict__ g,
+                            float* __restrict__ m, float* __restrict__ v, long count, float lr,
+                            float b1, float b2, float eps, float c1, float c2) {
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < count;
+         e += (long)gridDim.x * blockDim.x) {
+        const float gr = g[e];
+        const float mm = b1 * m[e] + (1.f - b1) * gr;
+        const float vv = b2 * v[e] + (1.f - b2) * gr * gr;
+        m[e] = mm;
+        v[e] = vv;
+        w[e] -= lr * (mm / c1) / (sqrtf(vv / c2) + eps);
+    }
+}
+
+__global__ void probe_mfma_kernel(float* d) {
+    const int lane = threadIdx.x & 63;
+    const int g = lane >> 4, m = lane & 15;
+    v4s a, b;
+    for (int i = 0; i < 4; ++i) {
+        a[i] = (short)f2bf((float)(16 * m + 4 * g + i));  // A[m][4g + i]
+        b[i] = (short)f2bf((4 * g + i) == m ? 1.f : 0.f);  // B[4g + i][m] (column m)
+    }
+    v4f acc = v4f{0.f, 0.f, 0.f, 0.f};
+    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
+    for (int i = 0; i < 4; ++i) d[(4 * g + i) * 16 + m] = acc[i];
+}
+
+__global__ void restrict2_kernel(const double* __restrict__ f, int n, double* __restrict__ c) {
+    const int nc = n / 2;
+    const long total = (long)nc * nc * nc;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (long)gridDim.x * blockDim.x) {
+        const int k = (int)(e % nc), j = (int)((e / nc) % nc), i = (int)(e / ((long)nc * nc));
+        double s = 0.0;
+        for (int d = 0; d < 8; ++d) {
+            const long fi = 2 * i + (d >> 2), fj = 2 * j + ((d >> 1) & 1), fk = 2 * k + (d & 1);
+            s += f[(fi * n + fj) * n + fk];
+        }
+        c[e] = 0.125 * s;
+    }
+}
+
+// coarse index pair and weight of fine cell i (fine grid n = f nc): the
+// fine centre sits at coarse coordinate (i + 1/2) / f - 1/2
+__device__ __forceinline__ void lin_weights(int i, int f, int nc, int& i0, int& i1, float& w1) {
+    const float xc = (i + 0.5f) / (float)f - 0.5f;
+    int a = (int)floorf(xc);
+    float w = xc - (float)a;
+    if (a < 0) { a = 0; w = 0.0f; }
+    if (a >= nc - 1) { a = nc - 1; w = 0.0f; }
+    i0 = a;
+    i1 = min(a + 1, nc - 1);
+    w1 = w;
+}
+
+__global__ void prolong_kernel(const float* __restrict__ c, int nc, int n,
+                               const double* __restrict__ mom, double G, double L,
+                               double* __restrict__ fine) {
+    const long total = (long)n * n * n;
+    const int f = n / nc;
+    const double scale = G * mom[0] / L;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (long)gridDim.x * blockDim.x) {
+        const int k = (int)(e % n), j = (int)((e / n) % n), i = (int)(e / ((long)n * n));
+        int i0, i1, j0, j1, k0, k1;
+        float wi, wj, wk;
+        lin_weights(i, f, nc, i0, i1, wi);
+        lin_weights(j, f, nc, j0, j1, wj);
+        lin_weights(k, f, nc, k0, k1, wk);
+        auto at = [&](int a, int b, int d) { return c[((long)a * nc + b) * nc + d]; };
+        const float c00 = at(i0, j0, k0) * (1 - wk) + at(i0, j0, k1) * wk;
+        const float c01 = at(i0, j1, k0) * (1 - wk) + at(i0, j1, k1) * wk;
+        const float c10 = at(i1, j0, k0) * (1 - wk) + at(i1, j0, k1) * wk;
+        const float c11 = at(i1, j1, k0) * (1 - wk) + at(i1, j1, k1) * wk;
+        const float c0 = c00 * (1 - wj) + c01 * wj, c1 = c10 * (1 - wj) + c11 * wj;
+        fine[e] = scale * (double)(c0 * (1 - wi) + c1 * wi);
+    }
+}
+
+// ------------------------------------------------------------ launchers --
+static unsigned grid_n(long n) {
+    long b = (n + 255) / 256;
+    return (unsigned)(b > 8192 ? 8192 : (b < 1 ? 1 : b));
+}
+
+hipError_t avgpool2(const uint16_t* x, int n, int cp, uint16_t* y, hipStream_t st) {
+    const long tot = (long)(n / 2) * (n / 2) * (n / 2) * cp;
+    hipLaunchKernelGGL(avgpool2_kernel, dim3(grid_n(tot)), dim3(256), 0, st, x, n, cp, y);
+    return hipGetLastError();
+}
+hipError_t sumpool2_add(const uint16_t* g, int n, int cp, uint16_t* out, int accumulate,
+                        hipStream_t st) {
+    const long tot = (long)(n / 2) * (n / 2) * (n / 2) * cp;
+    hipLaunchKernelGGL(sumpool2_kernel, dim3(grid_n(tot)), dim3(256), 0, st, g, n, cp, out,
+                       accumulate);
+    return hipGetLastError();
+}
+hipError_t unpool_add(const uint16_t* g, int n, int cp, uint16_t* out, hipStream_t st) {
+    const long tot = (long)n * n * n * cp;
+    hipLaunchKernelGGL(unpool_add_kernel, dim3(grid_n(tot)), dim3(256), 0, st, g, n, cp, out);
+    return hipGetLastError();
+}
+hipError_t relu_mask(const uint16_t* g, const uint16_t* r, long count, uint16_t* out,
+                     hipStream_t st) {
+    hipLaunchKernelGGL(relu_mask_kernel, dim3(grid_n(count)), dim3(256), 0, st, g, r, count, out);
+    return hipGetLastError();
+}
+hipError_t add_inplace(uint16_t* a, const uint16_t* b, long count, hipStream_t st) {
+    hipLaunchKernelGGL(add_kernel, dim3(grid_n(count)), dim3(256), 0, st, a, b, count);
+    return hipGetLastError();
+}
+hipError_t copy_bf16(const uint16_t* a, long count, uint16_t* out, hipStream_t st) {
+    hipLaunchKernelGGL(copy_kernel, dim3(grid_n(count)), dim3(256), 0, st, a, count, out);
+    return hipGetLastError();
+}
+static unsigned grid_red(long n) {
+    const unsigned g = grid_n(n);
+    return g > 512 ? 512 : g;
+}
+
+hipError_t log_stats(const float* rho, long nvox, float* acc, hipStream_t st) {
+    hipLaunchKernelGGL(log_stats_kernel<float>, dim3(grid_red(nvox)), dim3(256), 0, st, rho, nvox,
+                       acc);
+    return hipGetLastError();
+}
+hipError_t log_stats_f64(const double* rho, long nvox, float* acc, hipStream_t st) {
+    hipLaunchKernelGGL(log_stats_kernel<double>, dim3(grid_red(nvox)), dim3(256), 0, st, rho,
+                       nvox, acc);
+    return hipGetLastError();
+}
+hipError_t make_features(const float* rho, int n, const float* acc, const double* mom,
+                         double x0, double y0, double z0, double L, uint16_t* x, float* mono,
+                         hipStream_t st) {
+    const long nv = (long)n * n * n;
+    hipLaunchKernelGGL(make_features_kernel<float>, dim3(grid_n(nv)), dim3(256), 0, st, rho, n,
+                       acc, mom, x0, y0, z0, L, x, mono);
+    return hipGetLastError();
+}
+hipError_t make_features_f64(const double* rho, int n, const float* acc, const double* mom,
+                             double x0, double y0, double z0, double L, uint16_t* x, float* mono,
+                             hipStream_t st) {
+    const long nv = (long)n * n * n;
+    hipLaunchKernelGGL(make_features_kernel<double>, dim3(grid_n(nv)), dim3(256), 0, st, rho, n,
+                       acc, mom, x0, y0, z0, L, x, mono);
+    return hipGetLastError();
+}
+hipError_t mse_loss(const float* pred, const float* target, long nvox, float scale, uint16_t* g,
+                    float* loss, hipStream_t st) {
+    hipLaunchKernelGGL(mse_kernel, dim3(grid_red(nvox)), dim3(256), 0, st, pred, target, nvox,
+                       scale, g, loss);
+    return hipGetLastError();
+}
+hipError_t adam_step(float* w, const float* g, float* m, float* v, long count, float lr, float b1,
+                     float b2, float eps, float c1, float c2, hipStream_t st) {
+    hipLaunchKernelGGL(adam_kernel, dim3(grid_n(count)), dim3(256), 0, st, w, g, m, v, count, lr,
+                       b1, b2, eps, c1, c2);
+    return hipGetLastError();
+}
+hipError_t probe_mfma(float* d, hipStream_t st) {
+    hipLaunchKernelGGL(probe_mfma_kernel, dim3(1), dim3(64), 0, st, d);
+    return hipGetLastError();
+}
+hipError_t restrict2_f64(const double* fine, int n, double* coarse, hipStream_t st) {
+    const long tot = (long)(n / 2) * (n / 2) * (n / 2);
+    hipLaunchKernelGGL(restrict2_kernel, dim3(grid_n(tot)), dim3(256), 0, st, fine, n, coarse);
+    return hipGetLastError();
+}
+hipError_t prolong_to_f64(const float* coarse, int nc, int n, const double* mom, double G,
+                          double L, double* fine, hipStream_t st) {
+    if (nc <= 0 || n % nc) return hipErrorInvalidValue;
+    const long tot = (long)n * n * n;
+    hipLaunchKernelGGL(prolong_kernel, dim3(grid_n(tot)), dim3(256), 0, st, coarse, nc, n, mom, G,
+                       L, fine);
+    return hipGetLastError();
+}
+
+}  // namespace nn
+}  // namespace hyd
+
+// ---------------------------------------------------------- training data --
+namespace hyd {
+namespace nn {
+
+// One density field: blobs (Gaussian or n=1 polytrope profiles) on a floor,
+// times exp(delta - sigma^2/2) with delta a sum of Fourier modes (amplitude
+// ~ 1/k, i.e. power ~ k^-2).  blob = [cx cy cz mass width type] x nblob,
+// mode = [kx ky kz phase amp] x nmode (integer k: periodic on the box).
+__global__ void blob_field_kernel(int n, const double* __restrict__ blob, int nblob,
+                                  const double* __restrict__ mode, int nmode, double floor_rho,
+                                  double sigma, double* __restrict__ rho64,
+                                  float* __restrict__ rho32) {
+    const long total = (long)n * n * n;
+    const double h = 1.0 / n;
+    const double twopi = 6.283185307179586;
+    for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < total;
+         v += (long)gridDim.x * blockDim.x) {
+        const int k = (int)(v % n), j = (int)((v / n) % n), i = (int)(v / ((long)n * n));
+        const double x = (i + 0.5) * h, y = (j + 0.5) * h, z = (k + 0.5) * h;
+        double r = floor_rho;
+        for (int b = 0; b < nblob; ++b) {
+            const double* p = blob + 6 * b;
+            const double dx = x - p[0], dy = y - p[1], dz = z - p[2];
+            const double rr = sqrt(dx * dx + dy * dy + dz * dz);
+            const double w = p[4];
+            if (p[5] < 0.5) {
+                const double norm = 15.749609945653303 * w * w * w;  // (2 pi)^1.5 w^3
+                r += p[3] * exp(-0.5 * (rr / w) * (rr / w)) / norm;
+            } else {
+                const double R = 2.0 * w;
+                if (rr < R) {
+                    const double q = 3.141592653589793 * rr / R;
+                    const double prof = q > 0.0 ? sin(q) / q : 1.0;
+                    r += p[3] * prof / (4.0 * R * R * R / 3.141592653589793);
+                }
+            }
+        }
+        double d = 0.0;
+        for (int m = 0; m < nmode; ++m) {
+            const double* p = mode + 5 * m;
+            d += p[4] * cos(twopi * (p[0] * x + p[1] * y + p[2] * z) + p[3]);
+        }
+        r *= exp(d - 0.5 * sigma * sigma);
+        rho64[v] = r;
+        if (rho32) rho32[v] = (float)r;
+    }
+}
+
+hipError_t blob_field(int n, const double* blob, int nblob, const double* mode, int nmode,
+                      double floor_rho, double sigma, double* rho64, float* rho32,
+                      hipStream_t st) {
+    const long tot = (long)n * n * n;
+    hipLaunchKernelGGL(blob_field_kernel, dim3(grid_n(tot)), dim3(256), 0, st, n, blob, nblob,
+                       mode, nmode, floor_rho, sigma, rho64, rho32);
+    return hipGetLastError();
+}
+
+// target potential in units of G M / L, natural layout, FP32
+__global__ void scale_f64_to_f32_kernel(const double* __restrict__ a, long count,
+                                        const double* __restrict__ mom, double G, double L,
+                                        float* __restrict__ out) {
+    const double s = (mom[0] > 0.0) ? L / (G * mom[0]) : 1.0;
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < count;
+         e += (long)gridDim.x * blockDim.x)
+        out[e] = (float)(a[e] * s);
+}
+
+hipError_t scale_potential(const double* phi, long count, const double* mom, double G, double L,
+                           float* out, hipStream_t st) {
+    hipLaunchKernelGGL(scale_f64_to_f32_kernel, dim3(grid_n(count)), dim3(256), 0, st, phi, count,
+                       mom, G, L, out);
+    return hipGetLastError();
+}
+
+}  // namespace nn
+}  // namespace hyd

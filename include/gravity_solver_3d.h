@@ -62,6 +62,11 @@ typedef struct GravitySolver3D {
     int last_status;     /* 0 converged, -1 not converged, < -1 device error */
     double last_rel_residual; /* ||f - L phi|| / ||f|| after the last solve  */
     double last_solve_ms;     /* device time of the last potential solve     */
+    void *nn_model;      /* learned-guess network (nn_model_create), or NULL */
+    int last_guess;      /* POISSON_NN_FP32_DC: 1 network guess used, 0 warm
+                            start kept, -1 no guess possible              */
+    double last_guess_rel_residual; /* residual of the network guess         */
+    double last_warm_rel_residual;  /* residual of the warm start            */
 } GravitySolver3D;
 
 /* gravity_solver_3d.h:64-68 */

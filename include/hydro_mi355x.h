@@ -70,7 +70,7 @@ long hydro_check_division(const double *a, const double *b, long n);
 /* Multipole order of the isolated boundary: 2 = monopole + quadrupole as
  * the reference (gravity_solver_3d.c:302-497); up to 8. */
 int gravity_set_multipole_lmax(GravitySolver3D *solver, int lmax);
-/* POISSON_FP64_SOR or POISSON_FP32_DC (input_deck.h). */
+/* POISSON_FP64_SOR, POISSON_FP32_DC or POISSON_NN_FP32_DC (input_deck.h). */
 int gravity_set_poisson_mode(GravitySolver3D *solver, int mode);
 /* SOR_CRIT_MAX_UPDATE (the reference) or SOR_CRIT_REL_RESID. */
 int gravity_set_criterion(GravitySolver3D *solver, int criterion);
@@ -91,6 +91,49 @@ int gravity_rel_residual(GravitySolver3D *solver, const double *rho, double *rel
 /* Poisson solve on a device density (nx*ny*nz doubles in device memory);
  * the potential and (when converged) g stay on the device. */
 int gravity_solve_device(GravitySolver3D *solver, const double *d_rho);
+
+/* ---- learned initial guess of the Poisson solve (no reference counterpart;
+ * the reference starts SOR from the previous potential or from zero,
+ * gravity_solver_3d.c:87 and :566) ---------------------------------------- */
+
+#define NN_MAX_BLOBS 64
+#define NN_MAX_MODES 256
+
+/* Network with He-initialised weights from `seed`; NULL on failure. */
+void *nn_model_create(unsigned long long seed);
+void nn_model_destroy(void *model);
+/* Number of trainable parameters, and the length of the packed parameter
+ * vector (weights [27][cin padded to 4][cout] then bias, per layer). */
+long nn_model_num_params(void *model);
+long nn_model_param_len(void *model);
+long nn_model_param_offset(void *model, int layer, int bias);
+int nn_model_save(void *model, const char *path);
+int nn_model_load(void *model, const char *path);
+int nn_model_get_params(void *model, float *out);
+int nn_model_set_params(void *model, const float *in);
+/* One forward + backward pass on host data (kernel self-check). */
+int nn_model_forward_backward(void *model, const float *rho, const float *target, int n,
+                              const double *mom4, double L, float *loss, float *out,
+                              float *feat, float *grads);
+/* Device-resident training set of `count` n^3 fields. */
+void *nn_dataset_create(int n, int count);
+void nn_dataset_destroy(void *dataset);
+/* Field `index`: blobs (cx cy cz mass width type, type 0 Gaussian, 1 n=1
+ * polytrope of radius 2 width) on floor_rho, times exp(delta - sigma^2/2)
+ * with delta = sum of amp cos(2 pi k.x + phase) over modes (kx ky kz phase
+ * amp); solved cold by `grav` (an isolated n^3 solver).  Returns its status. */
+int nn_dataset_fill(void *dataset, int index, const double *blob, int nblob, const double *mode,
+                    int nmode, double floor_rho, double sigma, GravitySolver3D *grav);
+int nn_dataset_get(void *dataset, int index, float *rho, float *phi, double *mom4);
+/* Adam over mini-batches of dataset fields listed in order[steps * batch]. */
+int nn_train(void *model, void *dataset, const int *order, int steps, int batch, float lr,
+             float *loss_log);
+int nn_evaluate(void *model, void *dataset, int first, int count, float *mse);
+/* MFMA operand-layout probe: D = A I with A[m][k] = 16 m + k (256 floats). */
+int nn_probe_mfma(float *out256);
+/* Attach a network to a solver for POISSON_NN_FP32_DC (NULL detaches; the
+ * caller keeps ownership). */
+int gravity_set_nn_model(GravitySolver3D *solver, void *model);
 
 #ifdef __cplusplus
 }

@@ -46,7 +46,9 @@ typedef enum { BC_PERIODIC, BC_REFLECTIVE, BC_OUTFLOW } BCType;
 /* Poisson solve modes added by this build. */
 typedef enum {
     POISSON_FP64_SOR = 0,   /* plain red-black SOR in FP64 (the reference) */
-    POISSON_FP32_DC = 1     /* FP64 defect correction, FP32 red-black sweeps */
+    POISSON_FP32_DC = 1,    /* FP64 defect correction, FP32 red-black sweeps */
+    POISSON_NN_FP32_DC = 2  /* learned initial guess (kept only when its
+                               residual beats the warm start), then FP32_DC */
 } PoissonMode;
 
 /* SOR stopping rules added by this build. */

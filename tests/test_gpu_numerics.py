@@ -11,9 +11,10 @@ pytestmark = pytest.mark.gpu
 
 
 def test_reciprocal_division_is_ieee_division():
-    """The step kernels divide by a shared density through RN(1/b) and one
-    Markstein correction; it must equal a / b in every bit, including the
-    zero, tiny, huge and all-ones-significand cases that take the fallback."""
+    """The reciprocal-corrected division (RN(1/b) and one Markstein
+    correction, with an IEEE fallback; used by the step kernel when built
+    with HYD_FASTDIV=1) must equal a / b in every bit, including the zero,
+    tiny, huge and all-ones-significand cases that take the fallback."""
     rng = np.random.default_rng(99)
     n = 4_000_000
     a = rng.standard_normal(n) * 10.0 ** rng.uniform(-300, 300, n)
