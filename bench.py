@@ -12,9 +12,10 @@ Prints one JSON line.  The roofline block prices the fused step kernel at
 80 algorithmic bytes per zone (read U 5x8 B, write U 5x8 B) against 8 TB/s;
 its duration is the HIP-event average of that kernel on the library's stream.
 The cpu_baseline block times the reference C solver (built from its own
-sources, one core) on a bounded sample of the same workload.  With --poisson
-(default) a "poisson" block reports the 256^3 Poisson solve to a 1e-8
-relative residual: plain FP64 warm-started SOR vs FP32 defect correction.
+sources, one core) on a bounded sample of the same workload.  Unless
+--no-poisson, a "poisson" block reports 256^3 Poisson solves to a 1e-8
+relative residual: plain FP64 SOR vs FP32 defect correction vs the learned
+guess + FP32 defect correction, from a warm and from a cold start.
 """
 import argparse
 import json
@@ -86,13 +87,20 @@ def cpu_baseline(U, n, budget_s=20.0):
             "ms_per_step": 1e3 * el / steps}
 
 
-def poisson_block(H, ics, n, budget_steps=1):
-    """Warm-started 256^3 Poisson solve to ||f - L phi|| / ||f|| <= 1e-8:
-    potential phi_0 of the initial state (converged cold), one hydro step
-    with gravity, then both solvers start from phi_0 on the new density."""
+def poisson_block(H, ics, n):
+    """256^3 Poisson solves to ||f - L phi|| / ||f|| <= 1e-8 (interior).
+
+    Warm start (the case of every step after the first): phi_0 = the
+    converged potential of the initial state, one hydro step with gravity,
+    then each arm starts from phi_0 on the new density: plain FP64 SOR, FP32
+    defect correction, and the network guess (kept only when its residual
+    beats phi_0) + FP32 defect correction.  Cold start (the first step):
+    plain FP64 SOR from zero vs network guess + FP32 defect correction."""
     import numpy as np
+    from hydrocode_astro1_amd import nn
 
     U = ics.explosion_n3(n)
+    rho0 = np.ascontiguousarray(U[..., 0])
     s = H.Solver(n, n, n, G=1.0, gamma=5.0 / 3.0, cfl=0.4, bc=(2,) * 6)
     gr = s.gravity
     gr.set_lmax(6)
@@ -107,29 +115,65 @@ def poisson_block(H, ics, n, budget_steps=1):
     phi0 = gr.phi()
     rho1 = s.state()[..., 0].copy()
     out = {"grid": n, "tolerance": 1e-8, "criterion": "relative residual, interior",
-           "omega": 1.5, "multipole_lmax": 6, "cold_fp64": cold}
-    arms = {}
-    phis = {}
-    for name, mode in (("plain_fp64_sor", H.POISSON_FP64_SOR), ("fp32_dc", H.POISSON_FP32_DC)):
+           "omega": 1.5, "multipole_lmax": 6}
+    model = None
+    if os.path.exists(nn.DEFAULT_WEIGHTS):
+        model = nn.Model(path=nn.DEFAULT_WEIGHTS)
+        gr.set_nn_model(model)
+        out["network"] = {"weights": os.path.relpath(nn.DEFAULT_WEIGHTS, ROOT),
+                          "parameters": model.num_params()}
+    else:
+        out["network"] = {"weights": None, "note": "no trained weights; network arms skipped"}
+
+    def arm(mode, phi_start, rho):
         gr.set_mode(mode)
-        gr.set_phi(phi0)
-        st = gr.solve_potential(rho1)
-        it, res = gr.diagnostics()
-        arms[name] = {"status": st, "iterations": it, "ms": gr.s.last_solve_ms,
-                      "rel_residual": gr.s.last_rel_residual}
-        phis[name] = gr.phi()
-    # tight FP64 solution as the accuracy yardstick
+        gr.set_phi(phi_start)
+        st = gr.solve_potential(rho)
+        it, _ = gr.diagnostics()
+        r = {"status": st, "iterations": it, "ms": gr.s.last_solve_ms,
+             "rel_residual": gr.s.last_rel_residual}
+        if mode == H.POISSON_NN_FP32_DC:
+            r["guess_used"] = bool(gr.s.last_guess == 1)
+            r["guess_rel_residual"] = gr.s.last_guess_rel_residual
+            r["start_rel_residual"] = gr.s.last_warm_rel_residual
+        return r, gr.phi()
+
+    warm, phis = {}, {}
+    arms = [("plain_fp64_sor", H.POISSON_FP64_SOR), ("fp32_dc", H.POISSON_FP32_DC)]
+    if model is not None:
+        arms.append(("nn_fp32_dc", H.POISSON_NN_FP32_DC))
+    for name, mode in arms:
+        warm[name], phis[name] = arm(mode, phi0, rho1)
+    coldres, cphis = {"plain_fp64_sor": cold}, {}
+    if model is not None:
+        coldres["nn_fp32_dc"], cphis["nn_fp32_dc"] = arm(H.POISSON_NN_FP32_DC,
+                                                         np.zeros_like(phi0), rho0)
+    # tight FP64 solutions as the accuracy yardstick
     gr.set_mode(H.POISSON_FP64_SOR)
     gr.set_params(1.5, 400000, 1e-11)
     gr.set_phi(phis["plain_fp64_sor"])
     gr.solve_potential(rho1)
-    ref = gr.phi()
-    for name in arms:
-        arms[name]["rel_l2_vs_tight_fp64"] = float(
-            np.linalg.norm(phis[name] - ref) / np.linalg.norm(ref))
-    out.update(arms)
-    if arms["fp32_dc"]["ms"] > 0:
-        out["speedup_fp32dc_vs_plain"] = arms["plain_fp64_sor"]["ms"] / arms["fp32_dc"]["ms"]
+    ref1 = gr.phi()
+    for name in warm:
+        warm[name]["rel_l2_vs_tight_fp64"] = float(
+            np.linalg.norm(phis[name] - ref1) / np.linalg.norm(ref1))
+    if cphis:
+        gr.set_phi(phi0)
+        gr.solve_potential(rho0)
+        ref0 = gr.phi()
+        for name in cphis:
+            coldres[name]["rel_l2_vs_tight_fp64"] = float(
+                np.linalg.norm(cphis[name] - ref0) / np.linalg.norm(ref0))
+    out["warm_start"] = warm
+    out["cold_start"] = coldres
+    plain = warm["plain_fp64_sor"]["ms"]
+    if warm["fp32_dc"]["ms"] > 0:
+        out["speedup_fp32dc_vs_plain"] = plain / warm["fp32_dc"]["ms"]
+    if "nn_fp32_dc" in warm and warm["nn_fp32_dc"]["ms"] > 0:
+        out["speedup_nn_fp32dc_vs_plain"] = plain / warm["nn_fp32_dc"]["ms"]
+    if "nn_fp32_dc" in coldres and coldres["nn_fp32_dc"]["ms"] > 0:
+        out["speedup_cold_nn_fp32dc_vs_plain"] = cold["ms"] / coldres["nn_fp32_dc"]["ms"]
+    gr.set_nn_model(None)
     s.close()
     return out
 
