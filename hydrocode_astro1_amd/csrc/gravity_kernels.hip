@@ -46,8 +46,11 @@ __global__ void split_to_natural_kernel(GridDims g, const double* __restrict__ r
 }
 
 // ----------------------------------------------------------------- SOR --
-// gravity_solver_3d.c:642-708 for one colour.  Thread t covers split entry
-// m of interior row (i, j); its cell is k = 2m + ((i+j+colour)&1).
+// gravity_solver_3d.c:642-708 for one colour.  A block is 64 split entries
+// (m) of 4 interior rows j of one plane i (blockIdx.z): lane m's cell is
+// k = 2m + ((i+j+colour)&1).  No integer division in the index math.
+constexpr int SOR_ROWS = 4;
+
 __global__ void __launch_bounds__(256) sor_sweep_kernel(GridDims g, SorCoef c, int colour, int it,
                                                         double tol, double* __restrict__ mine,
                                                         const double* __restrict__ other,
@@ -59,7 +62,8 @@ __global__ void __launch_bounds__(256) sor_sweep_kernel(GridDims g, SorCoef c, i
     if (it > 0) {
         const double prev = bitsd(ctl->res[(it - 1) % 3]);
         if (prev < tol) {
-            if (colour == 0 && blockIdx.x == 0 && threadIdx.x == 0) {
+            if (colour == 0 && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 &&
+                threadIdx.x == 0) {
                 ctl->converged = 1;
                 ctl->iters = it;
                 ctl->last_res = prev;
@@ -67,29 +71,25 @@ __global__ void __launch_bounds__(256) sor_sweep_kernel(GridDims g, SorCoef c, i
             return;
         }
     }
-    if (colour == 0 && blockIdx.x == 0 && threadIdx.x == 0) ctl->res[(it + 1) % 3] = 0ull;
+    if (colour == 0 && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0)
+        ctl->res[(it + 1) % 3] = 0ull;
 
-    const long nrow = (long)(g.nx - 2) * (g.ny - 2);
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    const int m = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = 1 + blockIdx.y * SOR_ROWS + (threadIdx.x >> 6);
+    const int i = 1 + blockIdx.z;
+    const int k = 2 * m + ((i + j + colour) & 1);
     unsigned long long rb = 0ull;
-    if (t < nrow * g.nzh) {
-        const int m = (int)(t % g.nzh);
-        const long row = t / g.nzh;
-        const int j = 1 + (int)(row % (g.ny - 2));
-        const int i = 1 + (int)(row / (g.ny - 2));
-        const int k = 2 * m + ((i + j + colour) & 1);
-        if (k >= 1 && k <= g.nz - 2) {
-            const long base = ((long)i * g.ny + j) * g.nzh;
-            const long sxh = (long)g.ny * g.nzh;
-            const double o_im = other[base - sxh + m], o_ip = other[base + sxh + m];
-            const double o_jm = other[base - g.nzh + m], o_jp = other[base + g.nzh + m];
-            const double o_km = other[base + ((k - 1) >> 1)], o_kp = other[base + ((k + 1) >> 1)];
-            const double old = mine[base + m];
-            const double lap = (o_im + o_ip) * c.dx2i + (o_jm + o_jp) * c.dy2i + (o_km + o_kp) * c.dz2i;
-            const double gs = (lap - rhs[base + m]) / c.factor;
-            mine[base + m] = old + c.omega * (gs - old);
-            rb = dbits(fabs(gs - old));
-        }
+    if (j <= g.ny - 2 && k >= 1 && k <= g.nz - 2) {
+        const long base = ((long)i * g.ny + j) * g.nzh;
+        const long sxh = (long)g.ny * g.nzh;
+        const double o_im = other[base - sxh + m], o_ip = other[base + sxh + m];
+        const double o_jm = other[base - g.nzh + m], o_jp = other[base + g.nzh + m];
+        const double o_km = other[base + ((k - 1) >> 1)], o_kp = other[base + ((k + 1) >> 1)];
+        const double old = mine[base + m];
+        const double lap = (o_im + o_ip) * c.dx2i + (o_jm + o_jp) * c.dy2i + (o_km + o_kp) * c.dz2i;
+        const double gs = (lap - rhs[base + m]) / c.factor;
+        mine[base + m] = old + c.omega * (gs - old);
+        rb = dbits(fabs(gs - old));
     }
     if (tol < 0.0) return;  // relative-residual mode: the max update is not tracked
     rb = block_max_u64(rb);
@@ -499,27 +499,26 @@ __global__ void __launch_bounds__(256) residual_kernel(GridDims g, SorCoef c,
     }
 }
 
+// One colour of red-black SOR on L e = r in FP32 (same indexing as
+// sor_sweep_kernel); gs = (lap - r) * (1 / factor).
 __global__ void __launch_bounds__(256) sor_sweep_f32_kernel(GridDims g, float dx2i, float dy2i,
-                                                            float dz2i, float factor, float omega,
-                                                            int colour, float* __restrict__ mine,
+                                                            float dz2i, float inv_factor,
+                                                            float omega, int colour,
+                                                            float* __restrict__ mine,
                                                             const float* __restrict__ other,
                                                             const float* __restrict__ rhs) {
-    const long nrow = (long)(g.nx - 2) * (g.ny - 2);
-    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= nrow * g.nzh) return;
-    const int m = (int)(t % g.nzh);
-    const long row = t / g.nzh;
-    const int j = 1 + (int)(row % (g.ny - 2));
-    const int i = 1 + (int)(row / (g.ny - 2));
+    const int m = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = 1 + blockIdx.y * SOR_ROWS + (threadIdx.x >> 6);
+    const int i = 1 + blockIdx.z;
     const int k = 2 * m + ((i + j + colour) & 1);
-    if (k < 1 || k > g.nz - 2) return;
+    if (j > g.ny - 2 || k < 1 || k > g.nz - 2) return;
     const long base = ((long)i * g.ny + j) * g.nzh;
     const long sxh = (long)g.ny * g.nzh;
     const float lap = (other[base - sxh + m] + other[base + sxh + m]) * dx2i +
                       (other[base - g.nzh + m] + other[base + g.nzh + m]) * dy2i +
                       (other[base + ((k - 1) >> 1)] + other[base + ((k + 1) >> 1)]) * dz2i;
     const float old = mine[base + m];
-    const float gs = (lap - rhs[base + m]) / factor;
+    const float gs = (lap - rhs[base + m]) * inv_factor;
     mine[base + m] = old + omega * (gs - old);
 }
 
@@ -541,6 +540,11 @@ static unsigned grid_for(long n, long cap = 16384) {
     if (nb > cap) nb = cap;
     if (nb < 1) nb = 1;
     return (unsigned)nb;
+}
+
+static dim3 sor_grid(const GridDims& g) {
+    return dim3((unsigned)((g.nzh + 63) / 64), (unsigned)((g.ny - 2 + SOR_ROWS - 1) / SOR_ROWS),
+                (unsigned)(g.nx - 2));
 }
 
 long moments_scratch_len(long ncells, int lmax) {
@@ -602,13 +606,11 @@ hipError_t launch_split_to_natural(const GridDims& g, const double* red, const d
 hipError_t launch_sor_sweep(const GridDims& g, const SorCoef& c, int colour, int it, double tol,
                             double* phi_r, double* phi_b, const double* rhs_r,
                             const double* rhs_b, SorCtl* ctl, hipStream_t st) {
-    const long n = (long)(g.nx - 2) * (g.ny - 2) * g.nzh;
-    const unsigned nb = (unsigned)((n + 255) / 256);
     double* mine = colour ? phi_b : phi_r;
     const double* other = colour ? phi_r : phi_b;
     const double* rhs = colour ? rhs_b : rhs_r;
-    hipLaunchKernelGGL(sor_sweep_kernel, dim3(nb), dim3(256), 0, st, g, c, colour, it, tol, mine,
-                       other, rhs, ctl);
+    hipLaunchKernelGGL(sor_sweep_kernel, sor_grid(g), dim3(256), 0, st, g, c, colour, it, tol,
+                       mine, other, rhs, ctl);
     return hipGetLastError();
 }
 
@@ -645,14 +647,12 @@ hipError_t launch_residual(const GridDims& g, const SorCoef& c, const double* ph
 
 hipError_t launch_sor_sweep_f32(const GridDims& g, const SorCoef& c, int colour, float* e_r,
                                 float* e_b, const float* r_r, const float* r_b, hipStream_t st) {
-    const long n = (long)(g.nx - 2) * (g.ny - 2) * g.nzh;
-    const unsigned nb = (unsigned)((n + 255) / 256);
     float* mine = colour ? e_b : e_r;
     const float* other = colour ? e_r : e_b;
     const float* rhs = colour ? r_b : r_r;
-    hipLaunchKernelGGL(sor_sweep_f32_kernel, dim3(nb), dim3(256), 0, st, g, (float)c.dx2i,
-                       (float)c.dy2i, (float)c.dz2i, (float)c.factor, (float)c.omega, colour, mine,
-                       other, rhs);
+    hipLaunchKernelGGL(sor_sweep_f32_kernel, sor_grid(g), dim3(256), 0, st, g, (float)c.dx2i,
+                       (float)c.dy2i, (float)c.dz2i, (float)(1.0 / c.factor), (float)c.omega,
+                       colour, mine, other, rhs);
     return hipGetLastError();
 }
 
