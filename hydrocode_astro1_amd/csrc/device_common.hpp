@@ -295,6 +295,61 @@ __device__ __forceinline__ void hllc(const Cell& L, const Cell& R, double gamma,
     F[4] = sup ? f4 : f4 + S * (se - ke);
 }
 
+// The same flux with both cells resident in LDS as planes of doubles
+// (field f of a cell at cell_address[f * FS]): the side is chosen from the
+// normal velocity, pressure, sound speed and density of L and R, and only the
+// chosen cell's fields are then read, through one selected address, instead
+// of selecting every field between two register copies.  Bit-identical to
+// hllc<DIR>.  Field order: d mx my mz e u v w p c.
+template <int DIR, int FS>
+__device__ __forceinline__ void hllc_pick(double dL, double unL, double pL, double cL, double dR,
+                                          double unR, double pR, double cR,
+                                          const double* __restrict__ aL,
+                                          const double* __restrict__ aR, double gamma,
+                                          double F[5]) {
+    const double rL = rho_floor(dL), rR = rho_floor(dR);
+    const double SL = tmin(unL - cL, unR - cR);
+    const double SR = tmax(unL + cL, unR + cR);
+    const double num = pR - pL + rL * unL * (SL - unL) - rR * unR * (SR - unR);
+    const double den = rL * (SL - unL) - rR * (SR - unR);
+    const double Ss = num / den;
+
+    const bool sup_l = (SL >= 0.0);
+    const bool sup_r = !sup_l && (SR <= 0.0);
+    const bool left = sup_l || (!sup_r && (Ss >= 0.0));
+
+    const double* k = left ? aL : aR;
+    const double kd = k[0], kmx = k[FS], kmy = k[2 * FS], kmz = k[3 * FS], ke = k[4 * FS];
+    const double ku = k[5 * FS], kv = k[6 * FS], kw = k[7 * FS], kp = k[8 * FS];
+    const double kr = left ? rL : rR;
+    const double kn = DIR == 0 ? ku : (DIR == 1 ? kv : kw);
+    const double S = left ? SL : SR;
+
+    double f0, f1, f2, f3, f4;
+    if (DIR == 0) {
+        f0 = kmx; f1 = kmx * ku + kp; f2 = kmy * ku; f3 = kmz * ku;
+    } else if (DIR == 1) {
+        f0 = kmy; f1 = kmx * kv; f2 = kmy * kv + kp; f3 = kmz * kv;
+    } else {
+        f0 = kmz; f1 = kmx * kw; f2 = kmy * kw; f3 = kmz * kw + kp;
+    }
+    f4 = (ke + kp) * kn;
+
+    const double factor = (S - kn) / (S - Ss);
+    const double sd = kr * factor;
+    const double smx = sd * (DIR == 0 ? Ss : ku);
+    const double smy = sd * (DIR == 1 ? Ss : kv);
+    const double smz = sd * (DIR == 2 ? Ss : kw);
+    const double se = sd * (ke / kr + (Ss - kn) * (Ss + kp / (kr * (S - kn))));
+
+    const bool sup = sup_l || sup_r;
+    F[0] = sup ? f0 : f0 + S * (sd - kd);
+    F[1] = sup ? f1 : f1 + S * (smx - kmx);
+    F[2] = sup ? f2 : f2 + S * (smy - kmy);
+    F[3] = sup ? f3 : f3 + S * (smz - kmz);
+    F[4] = sup ? f4 : f4 + S * (se - ke);
+}
+
 // ---------------------------------------------------------------------------
 // double-double sums (compensated): two_sum is Knuth's error-free addition;
 // dd_add is the accurate pair addition of Joldes, Muller and Popescu (2017,

@@ -65,7 +65,7 @@ struct HydroDevice {
     double *g_zero = nullptr;          // zero g planes when gravity is off
     double *aos = nullptr;             // staging for AoS transfers
     long step = 0;                     // steps since the last upload
-    int cx = 32;                       // x planes per block of the step kernel
+    int cx = 0;                        // x planes per step-kernel block (0: chosen per grid)
     bool uploaded = false;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;

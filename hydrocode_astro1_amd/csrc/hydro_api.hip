@@ -74,6 +74,8 @@ int dev_create(GodunovHLLC3D* s) {
     d->n = (long)s->nx * s->ny * s->nz;
     d->stream = hyd_stream();
     if (!d->stream) return -2;
+    // x planes per step-kernel block; unset or 0 lets the launcher choose
+    if (const char* e = getenv("HYDRO_MI355X_CX")) d->cx = atoi(e) > 0 ? atoi(e) : 0;
     for (int b = 0; b < 2; ++b)
         for (int f = 0; f < 5; ++f) HYD_CHECK(hipMalloc((void**)&d->U[b][f], sizeof(double) * d->n));
     HYD_CHECK(hipMalloc((void**)&d->face, sizeof(double) * hyd::face_buffer_len(d->nx, d->ny, d->nz)));

@@ -34,8 +34,10 @@
 namespace hyd {
 
 namespace {
+// Rows per block: 10 cell rows + 2 halo rows = 12 wavefronts, 152 KiB of
+// LDS, 3 waves per SIMD (measured fastest of 4, 6, 8, 10 at 256^3).
 #ifndef HYD_TJ
-#define HYD_TJ 8
+#define HYD_TJ 10
 #endif
 constexpr int TK = 64;            // lanes (k) per row
 constexpr int TJ = HYD_TJ;        // cell rows (j) per block
@@ -44,6 +46,7 @@ constexpr int NT = NW * 64;       // threads per block
 constexpr int LR = TJ + 2;        // LDS rows: j0-1 .. j0+TJ
 constexpr int LC = TK + 2;        // LDS cols: k0-1 .. k0+64
 constexpr int NF = 10;            // LDS fields per cell: d mx my mz e u v w p c
+constexpr int FS = LR * LC;       // LDS stride between fields of one cell
 }  // namespace
 
 __device__ __forceinline__ double dt_from_speed(unsigned long long bits, double dx, double dy,
@@ -90,6 +93,17 @@ __device__ __forceinline__ Cell lds_get(const LdsCells& s, int r, int c) {
     q.e = s[4][r][c]; q.u = s[5][r][c]; q.v = s[6][r][c]; q.w = s[7][r][c];
     q.p = s[8][r][c]; q.c = s[9][r][c];
     return q;
+}
+
+// HLLC flux between two cells that are both in an LDS plane buffer
+template <int DIR>
+__device__ __forceinline__ void hllc_lds(const LdsCells& s, int rl, int cl, int rr, int cr,
+                                         double gamma, double F[5]) {
+    constexpr int UN = 5 + DIR;  // field of the normal velocity
+    const double* aL = &s[0][rl][cl];
+    const double* aR = &s[0][rr][cr];
+    hllc_pick<DIR, FS>(aL[0], aL[UN * FS], aL[8 * FS], aL[9 * FS], aR[0], aR[UN * FS], aR[8 * FS],
+                       aR[9 * FS], aL, aR, gamma, F);
 }
 
 __device__ __forceinline__ long face_index(int nx, int ny, int nz, int i, int j, int k, int f) {
@@ -200,16 +214,21 @@ __global__ void __launch_bounds__(NT) hydro_step_kernel(HydroStepArgs a) {
             if (has_next) load_raw(a, cidx + off + sx, valid, rn);  // prefetch plane i+1
 
             const Cell cur = lds_get(s_cell[pb], row + 1, lane + 1);
+            const double* me = &s_cell[pb][0][row + 1][lane + 1];
             {
                 double fy[5];
-                hllc<1>(cur, lds_get(s_cell[pb], row + 2, lane + 1), gamma, fy);
+                const double* nb = &s_cell[pb][0][row + 2][lane + 1];
+                hllc_pick<1, FS>(cur.d, cur.v, cur.p, cur.c, nb[0], nb[6 * FS], nb[8 * FS],
+                                 nb[9 * FS], me, nb, gamma, fy);
 #pragma unroll
                 for (int f = 0; f < 5; ++f) s_fy[f][row + 1][lane] = fy[f];
             }
             double dz[5];
             {
                 double fz[5];
-                hllc<2>(cur, lds_get(s_cell[pb], row + 1, lane + 2), gamma, fz);
+                const double* nb = &s_cell[pb][0][row + 1][lane + 2];
+                hllc_pick<2, FS>(cur.d, cur.w, cur.p, cur.c, nb[0], nb[7 * FS], nb[8 * FS],
+                                 nb[9 * FS], me, nb, gamma, fz);
 #pragma unroll
                 for (int f = 0; f < 5; ++f) {
                     double l = __shfl_up(fz[f], 1, 64);
@@ -222,7 +241,8 @@ __global__ void __launch_bounds__(NT) hydro_step_kernel(HydroStepArgs a) {
                 const Cell nxt = to_cell(rn, gamma);
                 lds_put(s_cell[pb ^ 1], row + 1, lane + 1, nxt);
                 double fxr[5];
-                hllc<0>(cur, nxt, gamma, fxr);
+                hllc_pick<0, FS>(cur.d, cur.u, cur.p, cur.c, nxt.d, nxt.u, nxt.p, nxt.c, me,
+                                 &s_cell[pb ^ 1][0][row + 1][lane + 1], gamma, fxr);
 #pragma unroll
                 for (int f = 0; f < 5; ++f) {
                     tx[f] = -ax * (fxr[f] - fxl[f]);
@@ -278,8 +298,7 @@ __global__ void __launch_bounds__(NT) hydro_step_kernel(HydroStepArgs a) {
         // F_z(k0-1, k0) of every cell row one plane ahead.
         if (lane < TJ) {
             double fh[5];
-            hllc<2>(lds_get(s_cell[ib & 1], lane + 1, 0), lds_get(s_cell[ib & 1], lane + 1, 1),
-                    gamma, fh);
+            hllc_lds<2>(s_cell[ib & 1], lane + 1, 0, lane + 1, 1, gamma, fh);
 #pragma unroll
             for (int f = 0; f < 5; ++f) s_fzl[ib & 1][f][lane] = fh[f];
         }
@@ -295,8 +314,7 @@ __global__ void __launch_bounds__(NT) hydro_step_kernel(HydroStepArgs a) {
             }
             {
                 double fy[5];
-                hllc<1>(lds_get(s_cell[pb], 0, lane + 1), lds_get(s_cell[pb], 1, lane + 1), gamma,
-                        fy);
+                hllc_lds<1>(s_cell[pb], 0, lane + 1, 1, lane + 1, gamma, fy);
 #pragma unroll
                 for (int f = 0; f < 5; ++f) s_fy[f][0][lane] = fy[f];
             }
@@ -307,8 +325,7 @@ __global__ void __launch_bounds__(NT) hydro_step_kernel(HydroStepArgs a) {
             __syncthreads();  // B2: plane i+1 complete
             if (has_next && lane < TJ) {
                 double fh[5];
-                hllc<2>(lds_get(s_cell[pb ^ 1], lane + 1, 0), lds_get(s_cell[pb ^ 1], lane + 1, 1),
-                        gamma, fh);
+                hllc_lds<2>(s_cell[pb ^ 1], lane + 1, 0, lane + 1, 1, gamma, fh);
 #pragma unroll
                 for (int f = 0; f < 5; ++f) s_fzl[pb ^ 1][f][lane] = fh[f];
             }
@@ -481,10 +498,39 @@ hipError_t launch_check_division(const double* a, const double* b, long n,
 }
 
 // ---------------------------------------------------------------- launchers
+// Planes per block along x when the caller leaves it open.  A block fills
+// a CU's LDS, so the grid runs in rounds of one block per CU: pick the chunk
+// count that minimises rounds x (planes per chunk + 1 recomputed plane).
+static int choose_cx(int nx, long tiles) {
+    static int ncu = 0;
+    if (ncu == 0) {
+        int dev = 0, v = 0;
+        if (hipGetDevice(&dev) == hipSuccess &&
+            hipDeviceGetAttribute(&v, hipDeviceAttributeMultiprocessorCount, dev) == hipSuccess &&
+            v > 0)
+            ncu = v;
+        else
+            ncu = 256;
+    }
+    int best_cx = nx;
+    long best = -1;
+    for (int cx = 8; cx <= 96; ++cx) {
+        const long nxc = (nx + cx - 1) / cx;
+        const long rounds = (tiles * nxc + ncu - 1) / ncu;
+        const long cost = rounds * (cx + 1);
+        if (best < 0 || cost < best) {
+            best = cost;
+            best_cx = cx;
+        }
+    }
+    return best_cx < nx ? best_cx : nx;
+}
+
 hipError_t launch_hydro_step(const HydroStepArgs& a, hipStream_t st) {
     const int ntj = (a.ny + TJ - 1) / TJ, ntk = (a.nz + TK - 1) / TK;
-    const int nxc = (a.nx + a.cx - 1) / a.cx;
     HydroStepArgs b = a;
+    if (b.cx <= 0) b.cx = choose_cx(a.nx, (long)ntj * ntk);
+    const int nxc = (a.nx + b.cx - 1) / b.cx;
     b.ntj = ntj; b.ntk = ntk; b.nxc = nxc;
     const long nblk = (long)ntj * ntk * nxc;
     hipLaunchKernelGGL(hydro_step_kernel, dim3((unsigned)nblk), dim3(NT), 0, st, b);
@@ -494,7 +540,7 @@ hipError_t launch_hydro_step(const HydroStepArgs& a, hipStream_t st) {
 hipError_t launch_hydro_bc(const HydroBCArgs& a, hipStream_t st) {
     const long tot = 2L * a.ny * a.nz + 2L * (a.nx - 2) * a.nz + 2L * (a.nx - 2) * (a.ny - 2);
     long nb = (tot + 255) / 256;
-    if (nb > 256) nb = 256;  // grid stride: few blocks, one atomic each
+    if (nb > 2048) nb = 2048;  // grid stride; one (pre-checked) atomic per block
     hipLaunchKernelGGL(hydro_bc_kernel, dim3((unsigned)nb), dim3(256), 0, st, a);
     return hipGetLastError();
 }

@@ -62,9 +62,12 @@ def test_random_state_each_bc_five_steps_bitwise(bc):
     assert same_bits(d.state(), o.state())
 
 
-def test_tile_edges_bitwise():
-    """A grid that is not a multiple of the 8 x 64 tile in j and k and spans
-    several x chunks, with strong shocks and floors."""
+@pytest.mark.parametrize("cx", ["0", "5", "64"])
+def test_tile_edges_bitwise(cx, monkeypatch):
+    """A grid that is not a multiple of the 10 x 64 tile in j and k, with x
+    chunks of the automatic size, 5 planes and one chunk (cx 64 > nx / 2),
+    with strong shocks and floors."""
+    monkeypatch.setenv("HYDRO_MI355X_CX", cx)
     shape = (70, 19, 131)
     rng = np.random.default_rng(3)
     X, Y, Z = ics.centres(shape)
