@@ -11,6 +11,7 @@
 #include <string.h>
 
 #include <algorithm>
+#include <vector>
 
 #include "../../include/hydro_mi355x.h"
 #include "../../include/input_deck.h"
@@ -44,7 +45,8 @@ void dev_free(GravDevice* d) {
     if (!d) return;
     void* ptrs[] = {d->phi_r, d->phi_b, d->rhs_r, d->rhs_b, d->gx,  d->gy, d->gz,
                     d->rho_tmp, d->nat_tmp, d->mom, d->mom_scratch, d->wlm, d->ctl, d->acc,
-                    d->e_r, d->e_b, d->r_r, d->r_b, d->guess_r, d->guess_b};
+                    d->e_r, d->e_b, d->r_r, d->r_b, d->guess_r, d->guess_b,
+                    d->phi2_r, d->phi2_b, d->e2_r, d->e2_b};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     if (d->ctl_host) (void)hipHostFree(d->ctl_host);
@@ -66,6 +68,7 @@ int dev_create(GravitySolver3D* s) {
     d->nsplit = (long)s->nx * s->ny * g.nzh;
     d->stream = hyd_stream();
     if (!d->stream) return -2;
+    if (const char* e = getenv("HYDRO_MI355X_SOR_FUSED")) d->fused = atoi(e) != 0;
     if (dev_alloc(&d->phi_r, d->nsplit) || dev_alloc(&d->phi_b, d->nsplit) ||
         dev_alloc(&d->rhs_r, d->nsplit) || dev_alloc(&d->rhs_b, d->nsplit) ||
         dev_alloc(&d->gx, d->n) || dev_alloc(&d->gy, d->n) || dev_alloc(&d->gz, d->n) ||
@@ -146,48 +149,103 @@ int apply_boundary(GravitySolver3D* s) {
     return 0;
 }
 
+// One launch of the FP64 iteration: a fused pair (buffers in -> out) or a
+// single iteration in place (in == out).  Kept so that the host can tell,
+// after the fact, which buffer holds the state after a given iteration.
+struct SorLaunch {
+    int first;         // first iteration
+    int count;         // 1 or 2
+    int in_b, out_b;   // buffer ids: 0 the solve's starting pair, 1 the spare
+};
+
 // Red-black SOR in FP64 with the reference's iteration structure
 // (gravity_solver_3d.c:566-746): red, black, boundary, check.  The boundary
 // values depend only on the density, so they are written once, after the
-// first iteration, exactly where the reference first writes them.
+// first iteration, exactly where the reference first writes them.  After
+// that, iterations run either as two colour sweeps in place or, with
+// d->fused, two per pass (sor2) from one copy of the potential into the
+// other; a pass whose first iteration already met the tolerance is undone by
+// recomputing that iteration alone from the pass's input, so the result and
+// the iteration count are those of one-iteration sweeps either way.
 int sor_fp64(GravitySolver3D* s) {
     GravDevice* d = s->dev;
     const SorCoef c = coefficients(s);
     const bool rel = (s->criterion == SOR_CRIT_REL_RESID);
     const double tol = s->tolerance;
+    const double stol = rel ? -1.0 : tol;  // device-side stop rule only for max-update
     HYD_CHECK(hyd::launch_sor_reset(d->ctl, d->stream));
     if (s->max_iterations <= 0) {
         s->last_iterations = 0;
         s->last_residual = 0.0;
         return -1;
     }
-    int it = 0, batch = 8;
+    bool fused = d->fused;
+    if (fused && !d->phi2_r && (dev_alloc(&d->phi2_r, d->nsplit) || dev_alloc(&d->phi2_b, d->nsplit)))
+        return -2;
+    double* buf_r[2] = {d->phi_r, d->phi2_r};
+    double* buf_b[2] = {d->phi_b, d->phi2_b};
+    int cur = 0;
+    std::vector<SorLaunch> launches;
+
+    auto single = [&](int it) -> int {
+        if (!rel) HYD_CHECK(hyd::launch_sor_prepare(d->ctl, it, d->stream));
+        HYD_CHECK(hyd::launch_sor_sweep(d->g, c, 0, it, stol, buf_r[cur], buf_b[cur], d->rhs_r,
+                                        d->rhs_b, d->ctl, d->stream));
+        HYD_CHECK(hyd::launch_sor_sweep(d->g, c, 1, it, stol, buf_r[cur], buf_b[cur], d->rhs_r,
+                                        d->rhs_b, d->ctl, d->stream));
+        launches.push_back({it, 1, cur, cur});
+        return 0;
+    };
+    // iteration 0, then the boundary values
+    if (single(0)) return -2;
+    d->phi_r = buf_r[cur];
+    d->phi_b = buf_b[cur];
+    if (apply_boundary(s)) return -2;
+    int it = 1;
+    if (fused && s->max_iterations > 1) {
+        // the spare copy carries the same boundary values
+        HYD_CHECK(hipMemcpyAsync(buf_r[1], buf_r[0], sizeof(double) * d->nsplit,
+                                 hipMemcpyDeviceToDevice, d->stream));
+        HYD_CHECK(hipMemcpyAsync(buf_b[1], buf_b[0], sizeof(double) * d->nsplit,
+                                 hipMemcpyDeviceToDevice, d->stream));
+    }
+    auto finish = [&](int keep) {  // buffer `keep` becomes the solver's potential
+        d->phi_r = buf_r[keep];
+        d->phi_b = buf_b[keep];
+        if (fused) {
+            d->phi2_r = buf_r[1 - keep];
+            d->phi2_b = buf_b[1 - keep];
+        }
+    };
+    int batch = 8;
     double rel_prev = -1.0;
     int it_prev = 0;
     while (true) {
         const int nb = std::min(batch, s->max_iterations - it);
-        for (int q = 0; q < nb; ++q, ++it) {
-            // in the relative-residual mode the sweeps never stop on their own
-            const double stol = rel ? -1.0 : tol;
-            HYD_CHECK(hyd::launch_sor_sweep(d->g, c, 0, it, stol, d->phi_r, d->phi_b, d->rhs_r,
-                                            d->rhs_b, d->ctl, d->stream));
-            HYD_CHECK(hyd::launch_sor_sweep(d->g, c, 1, it, stol, d->phi_r, d->phi_b, d->rhs_r,
-                                            d->rhs_b, d->ctl, d->stream));
-            if (it == 0 && apply_boundary(s)) return -2;
+        for (int q = 0; q < nb;) {
+            if (fused && nb - q >= 2) {
+                HYD_CHECK(hyd::launch_sor2(d->g, c, buf_r[cur], buf_b[cur], buf_r[1 - cur],
+                                           buf_b[1 - cur], d->rhs_r, d->rhs_b,
+                                           rel ? nullptr : d->ctl, it, tol, d->stream));
+                launches.push_back({it, 2, cur, 1 - cur});
+                cur = 1 - cur;
+                it += 2;
+                q += 2;
+            } else {
+                if (single(it)) return -2;
+                ++it;
+                ++q;
+            }
         }
         if (rel) {
+            finish(cur);
             double r;
             if (rel_residual_dev(s, nullptr, nullptr, &r)) return -2;
             s->last_rel_residual = r;
-            if (r <= tol) {
+            if (r <= tol || it >= s->max_iterations) {
                 s->last_iterations = it;
                 s->last_residual = r;
-                return 0;
-            }
-            if (it >= s->max_iterations) {
-                s->last_iterations = it;
-                s->last_residual = r;
-                return -1;
+                return r <= tol ? 0 : -1;
             }
             int b = batch;
             if (rel_prev > 0.0 && r < rel_prev) {
@@ -201,31 +259,60 @@ int sor_fp64(GravitySolver3D* s) {
             batch = std::max(8, std::min(b, 4096));
             continue;
         }
+        // max-update criterion: the stop check of iteration `it`, then read back
+        HYD_CHECK(hyd::launch_sor_check(d->ctl, it, tol, d->stream));
         HYD_CHECK(hipMemcpyAsync(d->ctl_host, d->ctl, sizeof(SorCtl), hipMemcpyDeviceToHost,
                                  d->stream));
         HYD_CHECK(hipStreamSynchronize(d->stream));
         const SorCtl& h = *d->ctl_host;
         if (h.converged) {
-            s->last_iterations = h.iters;
+            const int det = h.iters;  // first iteration that did not run
+            const double rm2 = det >= 2 ? bitsd(h.res[(det - 2) & 3]) : 1.0;
+            // the launch that ran iteration det - 1
+            const SorLaunch* L = nullptr;
+            for (auto r = launches.rbegin(); r != launches.rend(); ++r)
+                if (r->first <= det - 1 && det - 1 < r->first + r->count) {
+                    L = &*r;
+                    break;
+                }
+            if (!L) return -2;
+            if (L->count == 2 && L->first == det - 2 && rm2 < tol) {
+                // iteration det - 2 met tol inside a pair: redo it alone
+                cur = L->in_b;
+                HYD_CHECK(hyd::launch_sor_sweep(d->g, c, 0, det - 2, -1.0, buf_r[cur], buf_b[cur],
+                                                d->rhs_r, d->rhs_b, d->ctl, d->stream));
+                HYD_CHECK(hyd::launch_sor_sweep(d->g, c, 1, det - 2, -1.0, buf_r[cur], buf_b[cur],
+                                                d->rhs_r, d->rhs_b, d->ctl, d->stream));
+                finish(cur);
+                s->last_iterations = det - 1;
+                s->last_residual = rm2;
+                return 0;
+            }
+            finish(L->out_b);
+            s->last_iterations = det;
             s->last_residual = h.last_res;
             return 0;
         }
-        const double r1 = bitsd(h.res[(it - 1) % 3]);
+        finish(cur);
+        const double r1 = bitsd(h.res[(it - 1) & 3]);
         if (it >= s->max_iterations) {
             s->last_iterations = it;
             s->last_residual = r1;
-            if (r1 < tol) return 0;  // converged on the last allowed iteration
             return -1;
         }
-        const double r2 = it >= 2 ? bitsd(h.res[(it - 2) % 3]) : 0.0;
+        const double r2 = it >= 2 ? bitsd(h.res[(it - 2) & 3]) : 0.0;
         batch = next_batch(r1, r2, tol, batch);
+        launches.erase(launches.begin(),
+                       launches.end() - std::min<size_t>(launches.size(), 2));
     }
 }
 
 int ensure_dc_buffers(GravDevice* d) {
-    if (d->e_r) return 0;
-    if (dev_alloc(&d->e_r, d->nsplit) || dev_alloc(&d->e_b, d->nsplit) ||
-        dev_alloc(&d->r_r, d->nsplit) || dev_alloc(&d->r_b, d->nsplit))
+    if (!d->e_r && (dev_alloc(&d->e_r, d->nsplit) || dev_alloc(&d->e_b, d->nsplit) ||
+                    dev_alloc(&d->r_r, d->nsplit) || dev_alloc(&d->r_b, d->nsplit)))
+        return -2;
+    // the second correction copy keeps e = 0 on the boundary for good
+    if (d->fused && !d->e2_r && (dev_alloc(&d->e2_r, d->nsplit) || dev_alloc(&d->e2_b, d->nsplit)))
         return -2;
     return 0;
 }
@@ -244,11 +331,20 @@ int sor_fp32_dc(GravitySolver3D* s) {
     const double tol = s->tolerance;
     for (int outer = 0; outer < 64 && rel > tol && sweeps < s->max_iterations; ++outer) {
         const int nk = std::min(k, s->max_iterations - sweeps);
-        for (int q = 0; q < nk; ++q) {
-            HYD_CHECK(hyd::launch_sor_sweep_f32(d->g, c, 0, d->e_r, d->e_b, d->r_r, d->r_b,
-                                                d->stream));
-            HYD_CHECK(hyd::launch_sor_sweep_f32(d->g, c, 1, d->e_r, d->e_b, d->r_r, d->r_b,
-                                                d->stream));
+        for (int q = 0; q < nk;) {
+            if (d->fused && nk - q >= 2) {
+                HYD_CHECK(hyd::launch_sor2_f32(d->g, c, d->e_r, d->e_b, d->e2_r, d->e2_b, d->r_r,
+                                               d->r_b, d->stream));
+                std::swap(d->e_r, d->e2_r);
+                std::swap(d->e_b, d->e2_b);
+                q += 2;
+            } else {
+                HYD_CHECK(hyd::launch_sor_sweep_f32(d->g, c, 0, d->e_r, d->e_b, d->r_r, d->r_b,
+                                                    d->stream));
+                HYD_CHECK(hyd::launch_sor_sweep_f32(d->g, c, 1, d->e_r, d->e_b, d->r_r, d->r_b,
+                                                    d->stream));
+                ++q;
+            }
         }
         sweeps += nk;
         HYD_CHECK(hyd::launch_correct(d->g, d->phi_r, d->phi_b, d->e_r, d->e_b, d->stream));

@@ -51,28 +51,45 @@ __global__ void split_to_natural_kernel(GridDims g, const double* __restrict__ r
 // k = 2m + ((i+j+colour)&1).  No integer division in the index math.
 constexpr int SOR_ROWS = 4;
 
+// True when iteration `it` must not run; `writer` records the stop.
+__device__ __forceinline__ bool sor_stop_check(SorCtl* ctl, int it, double tol, bool writer) {
+    if (ctl->converged) return true;
+    double prev = 0.0;
+    bool stop = false;
+    if (it > 0) {
+        prev = bitsd(ctl->res[(it - 1) & 3]);
+        stop = prev < tol;
+    }
+    if (!stop && it > 1) {
+        const double prev2 = bitsd(ctl->res[(it - 2) & 3]);
+        if (prev2 < tol) {
+            stop = true;
+            prev = prev2;
+        }
+    }
+    if (stop && writer) {
+        ctl->converged = 1;
+        ctl->iters = it;
+        ctl->last_res = prev;
+    }
+    return stop;
+}
+
 __global__ void __launch_bounds__(256) sor_sweep_kernel(GridDims g, SorCoef c, int colour, int it,
                                                         double tol, double* __restrict__ mine,
                                                         const double* __restrict__ other,
                                                         const double* __restrict__ rhs,
                                                         SorCtl* ctl) {
     // stop rule of gravity_solver_3d.c:715: iteration it runs only while the
-    // residual of iteration it-1 is still >= tol
-    if (ctl->converged) return;
-    if (it > 0) {
-        const double prev = bitsd(ctl->res[(it - 1) % 3]);
-        if (prev < tol) {
-            if (colour == 0 && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 &&
-                threadIdx.x == 0) {
-                ctl->converged = 1;
-                ctl->iters = it;
-                ctl->last_res = prev;
-            }
-            return;
-        }
+    // residual of iteration it-1 is still >= tol (and, after a fused pair,
+    // that of it-2)
+    // (tol < 0: no stop rule and no residual, the relative-residual mode)
+    if (tol >= 0.0) {
+        const bool writer = colour == 0 && blockIdx.x == 0 && blockIdx.y == 0 &&
+                            blockIdx.z == 0 && threadIdx.x == 0;
+        if (sor_stop_check(ctl, it, tol, writer)) return;
+        if (writer) ctl->res[(it + 1) & 3] = 0ull;
     }
-    if (colour == 0 && blockIdx.x == 0 && blockIdx.y == 0 && blockIdx.z == 0 && threadIdx.x == 0)
-        ctl->res[(it + 1) % 3] = 0ull;
 
     const int m = blockIdx.x * 64 + (threadIdx.x & 63);
     const int j = 1 + blockIdx.y * SOR_ROWS + (threadIdx.x >> 6);
@@ -95,18 +112,26 @@ __global__ void __launch_bounds__(256) sor_sweep_kernel(GridDims g, SorCoef c, i
     rb = block_max_u64(rb);
     // most blocks see a running max already above theirs: skip their atomic
     if (threadIdx.x == 0 && rb != 0ull) {
-        const unsigned long long cur = __atomic_load_n(&ctl->res[it % 3], __ATOMIC_RELAXED);
-        if (rb > cur) atomicMax(&ctl->res[it % 3], rb);
+        const unsigned long long cur = __atomic_load_n(&ctl->res[it & 3], __ATOMIC_RELAXED);
+        if (rb > cur) atomicMax(&ctl->res[it & 3], rb);
     }
 }
 
 __global__ void sor_reset_kernel(SorCtl* ctl) {
-    ctl->res[0] = 0ull;
-    ctl->res[1] = 0ull;
-    ctl->res[2] = 0ull;
+    for (int q = 0; q < 4; ++q) ctl->res[q] = 0ull;
     ctl->converged = 0;
     ctl->iters = 0;
     ctl->last_res = 0.0;
+}
+
+__global__ void sor_prepare_kernel(SorCtl* ctl, int it) {
+    if (ctl->converged) return;
+    ctl->res[it & 3] = 0ull;
+    ctl->res[(it + 1) & 3] = 0ull;
+}
+
+__global__ void sor_check_kernel(SorCtl* ctl, int it, double tol) {
+    (void)sor_stop_check(ctl, it, tol, true);
 }
 
 // ---------------------------------------------------- multipole boundary --
@@ -611,6 +636,16 @@ hipError_t launch_sor_sweep(const GridDims& g, const SorCoef& c, int colour, int
     const double* rhs = colour ? rhs_b : rhs_r;
     hipLaunchKernelGGL(sor_sweep_kernel, sor_grid(g), dim3(256), 0, st, g, c, colour, it, tol,
                        mine, other, rhs, ctl);
+    return hipGetLastError();
+}
+
+hipError_t launch_sor_prepare(SorCtl* ctl, int it, hipStream_t st) {
+    hipLaunchKernelGGL(sor_prepare_kernel, dim3(1), dim3(1), 0, st, ctl, it);
+    return hipGetLastError();
+}
+
+hipError_t launch_sor_check(SorCtl* ctl, int it, double tol, hipStream_t st) {
+    hipLaunchKernelGGL(sor_check_kernel, dim3(1), dim3(1), 0, st, ctl, it, tol);
     return hipGetLastError();
 }
 

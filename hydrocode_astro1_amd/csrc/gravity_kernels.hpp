@@ -17,13 +17,17 @@ struct GridDims {
     double dx, dy, dz;
 };
 
-// Per-solve control of the SOR iteration on the device.
+// Per-solve control of the SOR iteration on the device.  The residual of
+// iteration t is accumulated in res[t & 3]; a launch clears the slots it is
+// about to fill.  A launch of iteration t runs only while the residuals of
+// iterations t-1 and t-2 are both >= tol; the first launch that sees one
+// below it sets converged and iters = t, and the host works out which
+// iteration met the tolerance (t-1, or t-2 when t-2 and t-1 ran as a pair).
 struct SorCtl {
-    unsigned long long res[3];  // max |phi_GS - phi_old| per iteration (ordered bits)
-    int converged;              // set by the first iteration that sees res < tol
-    int iters;                  // iteration count at convergence
+    unsigned long long res[4];  // max |phi_GS - phi_old| per iteration (ordered bits)
+    int converged;
+    int iters;                  // first iteration that did not run
     double last_res;
-    double pad;
 };
 
 struct SorCoef {
@@ -63,6 +67,21 @@ hipError_t launch_sor_sweep(const GridDims& g, const SorCoef& c, int colour, int
 hipError_t launch_sor_boundary(const GridDims& g, int bc, double G, int lmax, const double* mom,
                                const double* wlm, double* phi_r, double* phi_b, hipStream_t st);
 hipError_t launch_sor_reset(SorCtl* ctl, hipStream_t st);
+// Clear the residual slots of iterations it and it + 1 (unless converged).
+hipError_t launch_sor_prepare(SorCtl* ctl, int it, hipStream_t st);
+// The stop check of iteration it alone (after the last launch of a batch).
+hipError_t launch_sor_check(SorCtl* ctl, int it, double tol, hipStream_t st);
+// Two iterations in one pass, out of place: (in_r, in_b) -> (out_r, out_b)
+// on the interior (boundary cells of out must already hold the boundary
+// values).  With ctl, the residuals of iterations it and it + 1 go to their
+// slots and the pass is skipped once an earlier iteration met tol.
+hipError_t launch_sor2(const GridDims& g, const SorCoef& c, const double* in_r,
+                       const double* in_b, double* out_r, double* out_b, const double* rhs_r,
+                       const double* rhs_b, SorCtl* ctl, int it, double tol, hipStream_t st);
+// The same for the FP32 correction equation L e = r.
+hipError_t launch_sor2_f32(const GridDims& g, const SorCoef& c, const float* in_r,
+                           const float* in_b, float* out_r, float* out_b, const float* rhs_r,
+                           const float* rhs_b, hipStream_t st);
 
 // g = -grad phi with the boundary copies of gravity_solver_3d.c:820-897.
 hipError_t launch_acceleration(const GridDims& g, const double* phi_r, const double* phi_b,

@@ -47,6 +47,12 @@ struct GravDevice {
     double *acc_host = nullptr;       // pinned readback
     float *e_r = nullptr, *e_b = nullptr, *r_r = nullptr, *r_b = nullptr;
     double *guess_r = nullptr, *guess_b = nullptr;  // learned guess (split layout)
+    double *phi2_r = nullptr, *phi2_b = nullptr;    // second potential for fused passes
+    float *e2_r = nullptr, *e2_b = nullptr;         // second correction for fused passes
+    // two SOR iterations per pass (sor2), HYDRO_MI355X_SOR_FUSED=1; off by
+    // default: at 256^3 it measured 107 us per FP64 iteration against 105 for
+    // two colour sweeps, and 82 against 70 in FP32
+    bool fused = false;
     bool owns_nn = false;  // the network was created from an input deck
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
