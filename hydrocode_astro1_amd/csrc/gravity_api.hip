@@ -44,7 +44,7 @@ int dev_alloc(T** p, long count) {
 void dev_free(GravDevice* d) {
     if (!d) return;
     void* ptrs[] = {d->phi_r, d->phi_b, d->rhs_r, d->rhs_b, d->gx,  d->gy, d->gz,
-                    d->rho_tmp, d->nat_tmp, d->mom, d->mom_scratch, d->wlm, d->ctl, d->acc,
+                    d->rho_tmp, d->nat_tmp, d->mom, d->mom_scratch, d->wlm, d->ctl, d->acc, d->res_part,
                     d->e_r, d->e_b, d->r_r, d->r_b, d->guess_r, d->guess_b,
                     d->phi2_r, d->phi2_b, d->e2_r, d->e2_b};
     for (void* p : ptrs)
@@ -73,7 +73,8 @@ int dev_create(GravitySolver3D* s) {
         dev_alloc(&d->rhs_r, d->nsplit) || dev_alloc(&d->rhs_b, d->nsplit) ||
         dev_alloc(&d->gx, d->n) || dev_alloc(&d->gy, d->n) || dev_alloc(&d->gz, d->n) ||
         dev_alloc(&d->mom, 13 + hyd::mp_n_hi(hyd::MP_LMAX_MAX)) || dev_alloc(&d->wlm, kW * kW) ||
-        dev_alloc(&d->ctl, 1) || dev_alloc(&d->acc, 2))
+        dev_alloc(&d->ctl, 1) || dev_alloc(&d->acc, 2) ||
+        dev_alloc(&d->res_part, hyd::residual_scratch_len(d->g)))
         return -2;
     d->mom_scratch_len = hyd::moments_scratch_len(d->n, hyd::MP_LMAX_MAX);
     if (dev_alloc(&d->mom_scratch, d->mom_scratch_len)) return -2;
@@ -130,8 +131,8 @@ int next_batch(double r1, double r2, double tol, int prev) {
 int rel_residual_dev(GravitySolver3D* s, float* rr, float* rb, double* rel) {
     GravDevice* d = s->dev;
     SorCoef c = coefficients(s);
-    HYD_CHECK(hipMemsetAsync(d->acc, 0, 2 * sizeof(double), d->stream));
-    HYD_CHECK(hyd::launch_residual(d->g, c, d->phi_r, d->phi_b, d->rhs_r, d->rhs_b, rr, rb, d->acc,
+    HYD_CHECK(hyd::launch_residual(d->g, c, d->phi_r, d->phi_b, d->rhs_r, d->rhs_b, rr, rb,
+                                   d->res_part, d->acc,
                                    d->stream));
     HYD_CHECK(hipMemcpyAsync(d->acc_host, d->acc, 2 * sizeof(double), hipMemcpyDeviceToHost,
                              d->stream));

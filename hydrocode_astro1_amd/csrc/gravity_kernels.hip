@@ -475,6 +475,10 @@ __global__ void moments_finish_kernel(const double* __restrict__ sums, int nq, i
 }
 
 // ---------------------------------------------------- defect correction --
+// r = f - L phi on interior cells (0 on the boundary), both colours of
+// split entry m of row (i, j); the block's sums of r^2 and f^2 go to
+// part[block] and residual_finish_kernel adds the blocks in a fixed order
+// (no atomics: the result is reproducible).  Planes are grid-strided.
 __global__ void __launch_bounds__(256) residual_kernel(GridDims g, SorCoef c,
                                                        const double* __restrict__ pr,
                                                        const double* __restrict__ pb,
@@ -482,45 +486,77 @@ __global__ void __launch_bounds__(256) residual_kernel(GridDims g, SorCoef c,
                                                        const double* __restrict__ fb,
                                                        float* __restrict__ rr,
                                                        float* __restrict__ rbk,
-                                                       double* __restrict__ acc) {
-    const long n = (long)g.nx * g.ny * g.nzh * 2;
+                                                       double* __restrict__ part) {
+    const int m = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * SOR_ROWS + (threadIdx.x >> 6);
     double s_rr = 0.0, s_ff = 0.0;
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-         t += (long)gridDim.x * blockDim.x) {
-        const int colour = (int)(t & 1);
-        const long e = t >> 1;
-        const int m = (int)(e % g.nzh);
-        const long row = e / g.nzh;
-        const int j = (int)(row % g.ny), i = (int)(row / g.ny);
-        const int k = 2 * m + ((i + j + colour) & 1);
-        if (k >= g.nz) continue;
-        const double* mine = colour ? pb : pr;
-        const double* other = colour ? pr : pb;
-        float* rout = colour ? rbk : rr;
-        const long base = ((long)i * g.ny + j) * g.nzh;
-        if (i < 1 || i > g.nx - 2 || j < 1 || j > g.ny - 2 || k < 1 || k > g.nz - 2) {
-            if (rout) rout[base + m] = 0.0f;
-            continue;
-        }
+    if (m < g.nzh && j < g.ny) {
         const long sxh = (long)g.ny * g.nzh;
-        const double lap = (other[base - sxh + m] + other[base + sxh + m]) * c.dx2i +
-                           (other[base - g.nzh + m] + other[base + g.nzh + m]) * c.dy2i +
-                           (other[base + ((k - 1) >> 1)] + other[base + ((k + 1) >> 1)]) * c.dz2i -
-                           c.factor * mine[base + m];
-        const double f = (colour ? fb : fr)[base + m];
-        const double r = f - lap;
-        if (rout) rout[base + m] = (float)r;
-        s_rr += r * r;
-        s_ff += f * f;
+        for (int i = blockIdx.z; i < g.nx; i += gridDim.z) {
+            const long base = ((long)i * g.ny + j) * g.nzh;
+#pragma unroll
+            for (int colour = 0; colour < 2; ++colour) {
+                const int k = 2 * m + ((i + j + colour) & 1);
+                if (k >= g.nz) continue;
+                float* rout = colour ? rbk : rr;
+                if (i < 1 || i > g.nx - 2 || j < 1 || j > g.ny - 2 || k < 1 || k > g.nz - 2) {
+                    if (rout) rout[base + m] = 0.0f;
+                    continue;
+                }
+                const double* mine = colour ? pb : pr;
+                const double* other = colour ? pr : pb;
+                const double lap = (other[base - sxh + m] + other[base + sxh + m]) * c.dx2i +
+                                   (other[base - g.nzh + m] + other[base + g.nzh + m]) * c.dy2i +
+                                   (other[base + ((k - 1) >> 1)] + other[base + ((k + 1) >> 1)]) *
+                                       c.dz2i -
+                                   c.factor * mine[base + m];
+                const double f = (colour ? fb : fr)[base + m];
+                const double r = f - lap;
+                if (rout) rout[base + m] = (float)r;
+                s_rr += r * r;
+                s_ff += f * f;
+            }
+        }
     }
+    __shared__ double sp[2][4];
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
         s_rr += __shfl_xor(s_rr, o, 64);
         s_ff += __shfl_xor(s_ff, o, 64);
     }
     if ((threadIdx.x & 63) == 0) {
-        atomicAdd(&acc[0], s_rr);
-        atomicAdd(&acc[1], s_ff);
+        sp[0][threadIdx.x >> 6] = s_rr;
+        sp[1][threadIdx.x >> 6] = s_ff;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const long b = ((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+        part[2 * b] = (sp[0][0] + sp[0][1]) + (sp[0][2] + sp[0][3]);
+        part[2 * b + 1] = (sp[1][0] + sp[1][1]) + (sp[1][2] + sp[1][3]);
+    }
+}
+
+__global__ void __launch_bounds__(256) residual_finish_kernel(const double* __restrict__ part,
+                                                              long nblk, double* acc) {
+    double a = 0.0, b = 0.0;
+    for (long q = threadIdx.x; q < nblk; q += 256) {
+        a += part[2 * q];
+        b += part[2 * q + 1];
+    }
+    __shared__ double s[2][256];
+    s[0][threadIdx.x] = a;
+    s[1][threadIdx.x] = b;
+    __syncthreads();
+    for (int w = 128; w > 0; w >>= 1) {
+        if ((int)threadIdx.x < w) {
+            s[0][threadIdx.x] += s[0][threadIdx.x + w];
+            s[1][threadIdx.x] += s[1][threadIdx.x + w];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        acc[0] = s[0][0];
+        acc[1] = s[1][0];
     }
 }
 
@@ -545,6 +581,58 @@ __global__ void __launch_bounds__(256) sor_sweep_f32_kernel(GridDims g, float dx
     const float old = mine[base + m];
     const float gs = (lap - rhs[base + m]) * inv_factor;
     mine[base + m] = old + omega * (gs - old);
+}
+
+// The same sweep, four split entries per thread with 16-byte loads (rows
+// whose entry count is a multiple of 4).  Cells off the interior keep their
+// value (the store writes it back unchanged).  Row r of the block's 256
+// threads: groups of 4 entries along the row, then rows, then planes.
+__global__ void __launch_bounds__(256) sor_sweep_f32x4_kernel(GridDims g, float dx2i, float dy2i,
+                                                              float dz2i, float inv_factor,
+                                                              float omega, int colour,
+                                                              float* __restrict__ mine,
+                                                              const float* __restrict__ other,
+                                                              const float* __restrict__ rhs,
+                                                              unsigned groups_per_row,
+                                                              unsigned total) {
+    const unsigned t = blockIdx.x * 256u + threadIdx.x;
+    if (t >= total) return;
+    const unsigned row = t / groups_per_row, q = t - row * groups_per_row;
+    const int j = 1 + (int)(row % (unsigned)(g.ny - 2));
+    const int i = 1 + (int)(row / (unsigned)(g.ny - 2));
+    const int m0 = 4 * (int)q;
+    const int par = (i + j + colour) & 1;  // cell k = 2m + par
+    const long base = ((long)i * g.ny + j) * g.nzh + m0;
+    const long sxh = (long)g.ny * g.nzh;
+    const float4 xo = *reinterpret_cast<const float4*>(mine + base);
+    const float4 xr = *reinterpret_cast<const float4*>(rhs + base);
+    const float4 im = *reinterpret_cast<const float4*>(other + base - sxh);
+    const float4 ip = *reinterpret_cast<const float4*>(other + base + sxh);
+    const float4 jm = *reinterpret_cast<const float4*>(other + base - g.nzh);
+    const float4 jp = *reinterpret_cast<const float4*>(other + base + g.nzh);
+    const float4 kc = *reinterpret_cast<const float4*>(other + base);
+    // the entry beyond the group on the side the z neighbours reach
+    float edge = 0.0f;
+    if (par == 0 && m0 > 0) edge = other[base - 1];
+    if (par == 1 && m0 + 4 < g.nzh) edge = other[base + 4];
+    const float o[6] = {par ? kc.x : edge, kc.x, kc.y, kc.z, kc.w, par ? edge : kc.w};
+    // cell e: k - 1 -> o[e + par], k + 1 -> o[e + par + 1] (o[0] = entry m0 - 1)
+    const float old[4] = {xo.x, xo.y, xo.z, xo.w};
+    const float rr[4] = {xr.x, xr.y, xr.z, xr.w};
+    const float vim[4] = {im.x, im.y, im.z, im.w}, vip[4] = {ip.x, ip.y, ip.z, ip.w};
+    const float vjm[4] = {jm.x, jm.y, jm.z, jm.w}, vjp[4] = {jp.x, jp.y, jp.z, jp.w};
+    float out[4];
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int k = 2 * (m0 + e) + par;
+        const float okm = par ? o[e + 1] : o[e];
+        const float okp = par ? o[e + 2] : o[e + 1];
+        const float lap = (vim[e] + vip[e]) * dx2i + (vjm[e] + vjp[e]) * dy2i + (okm + okp) * dz2i;
+        const float gs = (lap - rr[e]) * inv_factor;
+        const float nv = old[e] + omega * (gs - old[e]);
+        out[e] = (k >= 1 && k <= g.nz - 2) ? nv : old[e];
+    }
+    *reinterpret_cast<float4*>(mine + base) = make_float4(out[0], out[1], out[2], out[3]);
 }
 
 __global__ void correct_kernel(GridDims g, double* __restrict__ pr, double* __restrict__ pb,
@@ -671,12 +759,24 @@ hipError_t launch_acceleration(const GridDims& g, const double* phi_r, const dou
     return hipGetLastError();
 }
 
+static dim3 residual_grid(const GridDims& g) {
+    return dim3((unsigned)((g.nzh + 63) / 64), (unsigned)((g.ny + SOR_ROWS - 1) / SOR_ROWS),
+                (unsigned)(g.nx < 64 ? g.nx : 64));
+}
+
+long residual_scratch_len(const GridDims& g) {
+    const dim3 d = residual_grid(g);
+    return 2L * d.x * d.y * d.z;
+}
+
 hipError_t launch_residual(const GridDims& g, const SorCoef& c, const double* phi_r,
                            const double* phi_b, const double* rhs_r, const double* rhs_b,
-                           float* r_r, float* r_b, double* acc, hipStream_t st) {
-    const long n = (long)g.nx * g.ny * g.nzh * 2;
-    hipLaunchKernelGGL(residual_kernel, dim3(grid_for(n, 8192)), dim3(256), 0, st, g, c, phi_r,
-                       phi_b, rhs_r, rhs_b, r_r, r_b, acc);
+                           float* r_r, float* r_b, double* part, double* acc, hipStream_t st) {
+    const dim3 grid = residual_grid(g);
+    hipLaunchKernelGGL(residual_kernel, grid, dim3(256), 0, st, g, c, phi_r, phi_b, rhs_r, rhs_b,
+                       r_r, r_b, part);
+    hipLaunchKernelGGL(residual_finish_kernel, dim3(1), dim3(256), 0, st, part,
+                       (long)grid.x * grid.y * grid.z, acc);
     return hipGetLastError();
 }
 
@@ -685,9 +785,17 @@ hipError_t launch_sor_sweep_f32(const GridDims& g, const SorCoef& c, int colour,
     float* mine = colour ? e_b : e_r;
     const float* other = colour ? e_r : e_b;
     const float* rhs = colour ? r_b : r_r;
-    hipLaunchKernelGGL(sor_sweep_f32_kernel, sor_grid(g), dim3(256), 0, st, g, (float)c.dx2i,
-                       (float)c.dy2i, (float)c.dz2i, (float)(1.0 / c.factor), (float)c.omega,
-                       colour, mine, other, rhs);
+    if (g.nzh % 4 == 0 && (long)(g.nx - 2) * (g.ny - 2) * (g.nzh / 4) < (1L << 31)) {
+        const unsigned gpr = (unsigned)(g.nzh / 4);
+        const unsigned total = (unsigned)((long)(g.nx - 2) * (g.ny - 2) * gpr);
+        hipLaunchKernelGGL(sor_sweep_f32x4_kernel, dim3((total + 255) / 256), dim3(256), 0, st, g,
+                           (float)c.dx2i, (float)c.dy2i, (float)c.dz2i, (float)(1.0 / c.factor),
+                           (float)c.omega, colour, mine, other, rhs, gpr, total);
+    } else {
+        hipLaunchKernelGGL(sor_sweep_f32_kernel, sor_grid(g), dim3(256), 0, st, g,
+                           (float)c.dx2i, (float)c.dy2i, (float)c.dz2i, (float)(1.0 / c.factor),
+                           (float)c.omega, colour, mine, other, rhs);
+    }
     return hipGetLastError();
 }
 

@@ -88,11 +88,14 @@ hipError_t launch_acceleration(const GridDims& g, const double* phi_r, const dou
                                double* gx, double* gy, double* gz, hipStream_t st);
 
 // ---- defect correction ---------------------------------------------------
-// r = f - L phi (FP64 arithmetic, FP32 store) on interior cells; 0 on the
-// boundary.  Accumulates sum r^2 and sum f^2 into acc[0], acc[1].
+// r = f - L phi (FP64 arithmetic, FP32 store, r_r/r_b may be null) on
+// interior cells; 0 on the boundary.  Writes sum r^2 and sum f^2 to acc[0],
+// acc[1], summed per block into part (residual_scratch_len doubles) and then
+// in a fixed order.
+long residual_scratch_len(const GridDims& g);
 hipError_t launch_residual(const GridDims& g, const SorCoef& c, const double* phi_r,
                            const double* phi_b, const double* rhs_r, const double* rhs_b,
-                           float* r_r, float* r_b, double* acc, hipStream_t st);
+                           float* r_r, float* r_b, double* part, double* acc, hipStream_t st);
 // FP32 red-black sweep of L e = r, e = 0 on the boundary.
 hipError_t launch_sor_sweep_f32(const GridDims& g, const SorCoef& c, int colour, float* e_r,
                                 float* e_b, const float* r_r, const float* r_b, hipStream_t st);

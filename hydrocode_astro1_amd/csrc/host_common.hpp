@@ -44,6 +44,7 @@ struct GravDevice {
     hyd::SorCtl *ctl = nullptr;
     hyd::SorCtl *ctl_host = nullptr;  // pinned readback
     double *acc = nullptr;            // residual sums (device)
+    double *res_part = nullptr;       // per-block residual sums
     double *acc_host = nullptr;       // pinned readback
     float *e_r = nullptr, *e_b = nullptr, *r_r = nullptr, *r_b = nullptr;
     double *guess_r = nullptr, *guess_b = nullptr;  // learned guess (split layout)
