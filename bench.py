@@ -178,6 +178,35 @@ def poisson_block(H, ics, n):
     return out
 
 
+def selfgravity_block(H, ics, n, steps=10):
+    """The full step with self-gravity on the same 256^3 state: the
+    reference's Poisson settings (red-black SOR, omega 1.5, max |update| <
+    1e-6, at most 10000 iterations) with the l = 6 multipole boundary; the
+    first step solves cold, the next `steps` warm-start from the previous
+    potential."""
+    U = ics.explosion_n3(n)
+    s = H.Solver(n, n, n, G=1.0, gamma=5.0 / 3.0, cfl=0.4, bc=(H.BC_OUTFLOW,) * 6)
+    gr = s.gravity
+    gr.set_lmax(6)
+    gr.set_params(1.5, 10000, 1e-6)
+    s.set_state(U)
+    t0 = time.perf_counter()
+    s.step()
+    cold_ms = 1e3 * (time.perf_counter() - t0)
+    cold_it = gr.diagnostics()[0]
+    its, ms = [], []
+    for _ in range(steps):
+        t0 = time.perf_counter()
+        s.step()  # returns after reading dt back: the step is complete
+        ms.append(1e3 * (time.perf_counter() - t0))
+        its.append(gr.diagnostics()[0])
+    s.close()
+    return {"grid": n, "sor": {"omega": 1.5, "tol_max_update": 1e-6, "max_iter": 10000},
+            "multipole_lmax": 6, "first_step": {"ms": cold_ms, "sor_iterations": cold_it},
+            "warm_steps": steps, "warm_ms_per_step": sum(ms) / len(ms),
+            "warm_sor_iterations": its}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -186,6 +215,9 @@ def main():
     ap.add_argument("--n", type=int, default=256)
     ap.add_argument("--no-poisson", action="store_true")
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--poisson-probe", type=int, default=0, metavar="ITERS",
+                    help="only time ITERS iterations of plain FP64 SOR and of FP32 defect "
+                         "correction on the 256^3 explosion density (twice each), then exit")
     ap.add_argument("--calibrate", action="store_true",
                     help="only run the device layout-conversion copy (40 B read + 40 B written "
                          "per cell) used to scale HBM byte counters, then exit")
@@ -203,6 +235,23 @@ def main():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
     n = args.n
+    if args.poisson_probe > 0:
+        import numpy as np
+        rho = np.ascontiguousarray(ics.explosion_n3(n)[..., 0])
+        g = H.Gravity(n, n, n, G=1.0)
+        g.set_lmax(6)
+        g.set_criterion(H.SOR_CRIT_REL_RESID)
+        out = []
+        for mode in (H.POISSON_FP64_SOR, H.POISSON_FP32_DC) * 2:
+            g.set_mode(mode)
+            g.set_params(1.5, args.poisson_probe, 1e-14)  # the cap ends each solve
+            g.set_phi(np.zeros((n, n, n)))
+            g.solve_potential(rho)
+            it = g.diagnostics()[0]
+            out.append({"mode": int(mode), "iterations": it, "ms": g.s.last_solve_ms,
+                        "us_per_iteration": 1e3 * g.s.last_solve_ms / max(it, 1)})
+        print(json.dumps({"poisson_probe": out, "grid": n}))
+        return
     if args.calibrate:
         import numpy as np
         s = H.Solver(n, n, n, G=0.0)
@@ -256,6 +305,10 @@ def main():
             res["poisson"] = poisson_block(H, ics, n)
         except Exception as e:  # report, never hide the headline
             res["poisson"] = {"error": repr(e)}
+        try:
+            res["selfgravity_step"] = selfgravity_block(H, ics, n)
+        except Exception as e:
+            res["selfgravity_step"] = {"error": repr(e)}
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(U, n)
     print(json.dumps(res))

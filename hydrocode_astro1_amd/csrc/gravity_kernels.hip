@@ -293,13 +293,6 @@ __global__ void acceleration_kernel(GridDims g, const double* __restrict__ pr,
 // not depend on the launch shape beyond that tree, and it is accurate to a
 // few units in the 106th bit before the final rounding.
 
-__device__ __forceinline__ void block_dd_store(DD v, int q, double* __restrict__ part, long nblk,
-                                               DD (*s)[4]) {
-    v = wave_dd_tree(v);
-    const int w = threadIdx.x >> 6;
-    if ((threadIdx.x & 63) == 0) s[q][w] = v;
-}
-
 __device__ __forceinline__ void cell_coord(const GridDims& g, long t, int& i, int& j, int& k) {
     k = (int)(t % g.nz);
     const long ij = t / g.nz;
@@ -307,11 +300,65 @@ __device__ __forceinline__ void cell_coord(const GridDims& g, long t, int& i, in
     i = (int)(ij / g.ny);
 }
 
+constexpr int MQC = 16;         // quantities per LDS round
+constexpr int MROW = 256 + 16;  // one padding entry per 16 cells: the 16-leaf
+                                // subtrees of 16 threads fall on distinct banks
+
+__device__ __forceinline__ int mslot(int c) { return c + (c >> 4); }
+
+// The pairwise double-double tree over the block's 256 consecutive cells of
+// each of NQ terms: 16 threads per quantity each sum a 16-leaf subtree, then
+// one thread sums the 16 subtrees; the block's sum of quantity q goes to
+// part[q][blockIdx.x].  The same tree as a lane butterfly followed by the
+// wavefronts pairwise, at one double-double addition per leaf.
+template <int NQ>
+__device__ __forceinline__ void block_tree_terms(const double (&term)[NQ], double* __restrict__ part,
+                                                 double (*s_term)[MROW], DD (*s_part)[16]) {
+    const int tid = threadIdx.x;
+#pragma unroll
+    for (int c0 = 0; c0 < NQ; c0 += MQC) {
+#pragma unroll
+        for (int qq = 0; qq < MQC; ++qq)
+            if (c0 + qq < NQ) s_term[qq][mslot(tid)] = term[c0 + qq];
+        __syncthreads();
+        {
+            const int qq = tid >> 4, st = tid & 15;
+            if (c0 + qq < NQ) {
+                const double* leaf = &s_term[qq][mslot(st * 16)];
+                DD v[8];
+#pragma unroll
+                for (int e = 0; e < 8; ++e) v[e] = dd_add(DD{leaf[2 * e], 0.0}, DD{leaf[2 * e + 1], 0.0});
+#pragma unroll
+                for (int e = 0; e < 4; ++e) v[e] = dd_add(v[2 * e], v[2 * e + 1]);
+                v[0] = dd_add(v[0], v[1]);
+                v[1] = dd_add(v[2], v[3]);
+                s_part[qq][st] = dd_add(v[0], v[1]);
+            }
+        }
+        __syncthreads();
+        if (tid < MQC && c0 + tid < NQ) {
+            DD v[8];
+#pragma unroll
+            for (int e = 0; e < 8; ++e) v[e] = dd_add(s_part[tid][2 * e], s_part[tid][2 * e + 1]);
+#pragma unroll
+            for (int e = 0; e < 4; ++e) v[e] = dd_add(v[2 * e], v[2 * e + 1]);
+            v[0] = dd_add(v[0], v[1]);
+            v[1] = dd_add(v[2], v[3]);
+            const DD r = dd_add(v[0], v[1]);
+            const long o = 2 * ((long)(c0 + tid) * gridDim.x + blockIdx.x);
+            part[o] = r.hi;
+            part[o + 1] = r.lo;
+        }
+        __syncthreads();  // the next round reuses s_term and s_part
+    }
+}
+
 // pass 1: M and the first moments (gravity_solver_3d.c:182-198)
 __global__ void __launch_bounds__(256) moments_pass1_kernel(GridDims g,
                                                             const double* __restrict__ rho,
                                                             double* __restrict__ part) {
-    __shared__ DD s[4][4];
+    __shared__ double s_term[MQC][MROW];
+    __shared__ DD s_part[MQC][16];
     const long n = (long)g.nx * g.ny * g.nz;
     const long t = (long)blockIdx.x * 256 + threadIdx.x;
     double term[4] = {0.0, 0.0, 0.0, 0.0};
@@ -328,26 +375,26 @@ __global__ void __launch_bounds__(256) moments_pass1_kernel(GridDims g,
             term[0] = dm; term[1] = dm * x; term[2] = dm * y; term[3] = dm * z;
         }
     }
-#pragma unroll
-    for (int q = 0; q < 4; ++q) block_dd_store(DD{term[q], 0.0}, q, part, gridDim.x, s);
-    __syncthreads();
-    if (threadIdx.x < 4) {
-        const int q = threadIdx.x;
-        const DD v = dd_add(dd_add(s[q][0], s[q][1]), dd_add(s[q][2], s[q][3]));
-        part[2 * ((long)q * gridDim.x + blockIdx.x)] = v.hi;
-        part[2 * ((long)q * gridDim.x + blockIdx.x) + 1] = v.lo;
-    }
+    block_tree_terms<4>(term, part, s_term, s_part);
+}
+
+constexpr int mp_nhi_c(int lmax) {
+    int n = 0;
+    for (int l = 3; l <= lmax; ++l) n += 2 * (l + 1);
+    return n;
 }
 
 // pass 2: Q_ij (gravity_solver_3d.c:226-258) and A_lm, B_lm for l >= 3
-__global__ void __launch_bounds__(256) moments_pass2_kernel(GridDims g, int lmax,
+template <int LMAX>
+__global__ void __launch_bounds__(256) moments_pass2_kernel(GridDims g,
                                                             const double* __restrict__ rho,
                                                             const double* __restrict__ mom,
                                                             double* __restrict__ part) {
-    __shared__ DD s[9 + 2 * 42][4];
+    constexpr int NQ = 9 + mp_nhi_c(LMAX);
+    __shared__ double s_term[MQC][MROW];
+    __shared__ DD s_part[MQC][16];
     const long n = (long)g.nx * g.ny * g.nz;
     const long t = (long)blockIdx.x * 256 + threadIdx.x;
-    const int nq = 9 + [&] { int c = 0; for (int l = 3; l <= lmax; ++l) c += 2 * (l + 1); return c; }();
     const double r = (t < n) ? rho[t] : 0.0;
     const bool in = (t < n) && (r > 0.0);
     double x = 0.0, y = 0.0, z = 0.0, dm = 0.0, r2 = 0.0;
@@ -361,26 +408,25 @@ __global__ void __launch_bounds__(256) moments_pass2_kernel(GridDims g, int lmax
         dm = r * dV;
         r2 = x * x + y * y + z * z;
     }
-    {
-        double term[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
-        if (in) {
-            term[0] = dm * (3.0 * x * x - r2);
-            term[1] = dm * (3.0 * x * y);
-            term[2] = dm * (3.0 * x * z);
-            term[3] = dm * (3.0 * y * x);
-            term[4] = dm * (3.0 * y * y - r2);
-            term[5] = dm * (3.0 * y * z);
-            term[6] = dm * (3.0 * z * x);
-            term[7] = dm * (3.0 * z * y);
-            term[8] = dm * (3.0 * z * z - r2);
-        }
+    double term[NQ];
 #pragma unroll
-        for (int q = 0; q < 9; ++q) block_dd_store(DD{term[q], 0.0}, q, part, gridDim.x, s);
+    for (int q = 0; q < NQ; ++q) term[q] = 0.0;
+    if (in) {
+        term[0] = dm * (3.0 * x * x - r2);
+        term[1] = dm * (3.0 * x * y);
+        term[2] = dm * (3.0 * x * z);
+        term[3] = dm * (3.0 * y * x);
+        term[4] = dm * (3.0 * y * y - r2);
+        term[5] = dm * (3.0 * y * z);
+        term[6] = dm * (3.0 * z * x);
+        term[7] = dm * (3.0 * z * y);
+        term[8] = dm * (3.0 * z * z - r2);
     }
-    if (lmax >= 3) {
+    if (LMAX >= 3) {
         // walk the recurrence per m; emit each (l, m) with l >= 3 at its slot
         double cr = 1.0, ci = 0.0;
-        for (int m = 0; m <= lmax; ++m) {
+#pragma unroll
+        for (int m = 0; m <= LMAX; ++m) {
             if (m > 0) {
                 const double f = (double)(2 * m - 1);
                 const double nr = f * (x * cr - y * ci);
@@ -390,7 +436,8 @@ __global__ void __launch_bounds__(256) moments_pass2_kernel(GridDims g, int lmax
             }
             double c2 = cr, s2 = ci;  // R_{l-2}, R_{l-1} as the walk advances
             double c1 = 0.0, s1 = 0.0;
-            for (int l = m; l <= lmax; ++l) {
+#pragma unroll
+            for (int l = m; l <= LMAX; ++l) {
                 double cl, sl;
                 if (l == m) {
                     cl = cr; sl = ci;
@@ -408,21 +455,14 @@ __global__ void __launch_bounds__(256) moments_pass2_kernel(GridDims g, int lmax
                     c2 = c1; s2 = s1; c1 = cl; s1 = sl;
                 }
                 if (l >= 3) {
-                    int q = 9;
-                    for (int ll = 3; ll < l; ++ll) q += 2 * (ll + 1);
-                    q += 2 * m;
-                    block_dd_store(DD{in ? dm * cl : 0.0, 0.0}, q, part, gridDim.x, s);
-                    block_dd_store(DD{in ? dm * sl : 0.0, 0.0}, q + 1, part, gridDim.x, s);
+                    const int q = 9 + mp_nhi_c(l - 1) + 2 * m;
+                    term[q] = in ? dm * cl : 0.0;
+                    term[q + 1] = in ? dm * sl : 0.0;
                 }
             }
         }
     }
-    __syncthreads();
-    for (int q = threadIdx.x; q < nq; q += 256) {
-        const DD v = dd_add(dd_add(s[q][0], s[q][1]), dd_add(s[q][2], s[q][3]));
-        part[2 * ((long)q * gridDim.x + blockIdx.x)] = v.hi;
-        part[2 * ((long)q * gridDim.x + blockIdx.x) + 1] = v.lo;
-    }
+    block_tree_terms<NQ>(term, part, s_term, s_part);
 }
 
 // Fold 256 consecutive partials of quantity q (zero padded) pairwise.
@@ -693,8 +733,16 @@ hipError_t launch_moments(const GridDims& g, const double* rho, int lmax, double
     double* s1 = reduce_partials(scratch, nb, 4, st);
     hipLaunchKernelGGL(moments_finish_kernel, dim3(1), dim3(64), 0, st, s1, 4, 1, g, mom_out);
     const int nq = mp_n_pass2(lmax);
-    hipLaunchKernelGGL(moments_pass2_kernel, dim3((unsigned)nb), dim3(256), 0, st, g, lmax, rho,
-                       mom_out, scratch);
+    switch (lmax) {
+#define MP_CASE(L)                                                                               \
+    case L:                                                                                      \
+        hipLaunchKernelGGL(moments_pass2_kernel<L>, dim3((unsigned)nb), dim3(256), 0, st, g, rho, \
+                           mom_out, scratch);                                                    \
+        break;
+        MP_CASE(2) MP_CASE(3) MP_CASE(4) MP_CASE(5) MP_CASE(6) MP_CASE(7) MP_CASE(8)
+#undef MP_CASE
+        default: return hipErrorInvalidValue;
+    }
     double* s2 = reduce_partials(scratch, nb, nq, st);
     hipLaunchKernelGGL(moments_finish_kernel, dim3(1), dim3(64), 0, st, s2, nq, 2, g, mom_out);
     return hipGetLastError();

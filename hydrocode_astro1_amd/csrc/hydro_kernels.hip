@@ -372,25 +372,29 @@ __device__ __forceinline__ void maybe_reflect(bool on, double q[5], double gamma
     if (on) reflect<DIR>(q[0], q[1], q[2], q[3], q[4], gamma);
 }
 
+// One thread per boundary cell: blockIdx.z is the face (x-min, x-max,
+// y-min, y-max, z-min, z-max), blockIdx.y the outer index and the threads of
+// blockIdx.x the inner one (k on the x and y faces, j on the z faces).  Each
+// face owns the cells not owned by an earlier axis: x faces all of theirs, y
+// faces i in 1..nx-2, z faces i, j in 1..nx-2, 1..ny-2.
 __global__ void __launch_bounds__(256) hydro_bc_kernel(HydroBCArgs a) {
     const int nx = a.nx, ny = a.ny, nz = a.nz;
-    const long nX = 2L * ny * nz, nY = 2L * (nx - 2) * nz, nZ = 2L * (nx - 2) * (ny - 2);
+    const int face = blockIdx.z, axis = face >> 1, hi_side = face & 1;
+    const int outer = blockIdx.y, inner = blockIdx.x * 256 + threadIdx.x;
     unsigned long long smax = 0ull;
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < nX + nY + nZ;
-         t += (long)gridDim.x * blockDim.x) {
-        int i, j, k;
-        if (t < nX) {
-            long side = t / ((long)ny * nz), rem = t % ((long)ny * nz);
-            i = side ? nx - 1 : 0; j = (int)(rem / nz); k = (int)(rem % nz);
-        } else if (t < nX + nY) {
-            long u = t - nX, per = (long)(nx - 2) * nz;
-            long side = u / per, rem = u % per;
-            i = 1 + (int)(rem / nz); j = side ? ny - 1 : 0; k = (int)(rem % nz);
-        } else {
-            long u = t - nX - nY, per = (long)(nx - 2) * (ny - 2);
-            long side = u / per, rem = u % per;
-            i = 1 + (int)(rem / (ny - 2)); j = 1 + (int)(rem % (ny - 2)); k = side ? nz - 1 : 0;
-        }
+    int i = 0, j = 0, k = 0;
+    bool on = false;
+    if (axis == 0) {
+        i = hi_side ? nx - 1 : 0; j = outer; k = inner;
+        on = (j < ny) && (k < nz);
+    } else if (axis == 1) {
+        i = 1 + outer; j = hi_side ? ny - 1 : 0; k = inner;
+        on = (i <= nx - 2) && (k < nz);
+    } else {
+        i = 1 + outer; j = 1 + inner; k = hi_side ? nz - 1 : 0;
+        on = (i <= nx - 2) && (j <= ny - 2);
+    }
+    if (on) {
         const int n3[3] = {nx, ny, nz};
         const int c3[3] = {i, j, k};
         int s3[3];
@@ -538,10 +542,11 @@ hipError_t launch_hydro_step(const HydroStepArgs& a, hipStream_t st) {
 }
 
 hipError_t launch_hydro_bc(const HydroBCArgs& a, hipStream_t st) {
-    const long tot = 2L * a.ny * a.nz + 2L * (a.nx - 2) * a.nz + 2L * (a.nx - 2) * (a.ny - 2);
-    long nb = (tot + 255) / 256;
-    if (nb > 2048) nb = 2048;  // grid stride; one (pre-checked) atomic per block
-    hipLaunchKernelGGL(hydro_bc_kernel, dim3((unsigned)nb), dim3(256), 0, st, a);
+    // widest inner and outer extents over the three face kinds
+    const int inner = a.nz > a.ny - 2 ? a.nz : a.ny - 2;
+    const int outer = a.ny > a.nx - 2 ? a.ny : a.nx - 2;
+    hipLaunchKernelGGL(hydro_bc_kernel, dim3((unsigned)((inner + 255) / 256), (unsigned)outer, 6),
+                       dim3(256), 0, st, a);
     return hipGetLastError();
 }
 

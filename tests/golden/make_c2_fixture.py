@@ -1,0 +1,68 @@
+"""Fixture for the 128^3 n=1 polytrope workload (G = 1, gamma = 2): the
+oracle run (reference semantics, moments summed by the device's pairwise
+double-double tree, boundary moments hoisted out of the SOR loop, which the
+oracle tests show is bit-identical to the reference) for STEPS steps.
+
+Stored: per-step dt and SOR iteration counts, a SHA-256 of the final state
+bytes, per-field L1 norms and sums, and the same run with the reference's
+sequential moment sums (final-state L1 norms) for the relative-L1 bar.
+
+    python tests/golden/make_c2_fixture.py [steps]      (about an hour on one core)
+"""
+import hashlib
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from oracle_lib import MOM_DDTREE, MOM_SEQUENTIAL, Grid, OracleSolver  # noqa: E402
+from hydrocode_astro1_amd import ics  # noqa: E402
+
+N = 128
+TOL = 1e-8
+OMEGA = 1.5
+
+
+def run(mode, steps, log):
+    U0 = ics.polytrope_n1(N)
+    g = Grid(N, N, N, G=1.0, gamma=2.0, cfl=0.4, bc=(2,) * 6)
+    o = OracleSolver(g, omega=OMEGA, max_iter=200000, tol=TOL, lmax=2, moment_mode=mode)
+    o.set_state(U0)
+    dts, its = [], []
+    t0 = time.time()
+    for s in range(steps):
+        o.step()
+        it, res, st = o.diag()
+        dts.append(o.dt)
+        its.append(it)
+        log(f"mode {mode} step {s + 1}/{steps}: dt {o.dt:.6e}, SOR {it} iterations, "
+            f"status {st}, {time.time() - t0:.0f}s")
+    return o.state(), np.array(dts), np.array(its)
+
+
+def main():
+    steps = int(sys.argv[1]) if len(sys.argv) > 1 else 100
+    log = lambda m: print(m, flush=True)
+    U, dts, its = run(MOM_DDTREE, steps, log)
+    Us, dts_s, its_s = run(MOM_SEQUENTIAL, steps, log)
+    out = dict(
+        steps=steps, n=N, tol=TOL, omega=OMEGA, dt=dts, iters=its,
+        sha256=np.frombuffer(hashlib.sha256(np.ascontiguousarray(U).tobytes()).digest(),
+                             dtype=np.uint8),
+        l1=np.abs(U).reshape(-1, 5).sum(axis=0), sums=U.reshape(-1, 5).sum(axis=0),
+        seq_dt=dts_s, seq_iters=its_s, seq_l1=np.abs(Us).reshape(-1, 5).sum(axis=0),
+        seq_diff_l1=np.abs(U - Us).reshape(-1, 5).sum(axis=0),
+        rho_slice=np.ascontiguousarray(U[N // 2, :, :, 0]),
+    )
+    np.savez_compressed(os.path.join(HERE, f"c2_polytrope_{N}.npz"), **out)
+    log(f"wrote c2_polytrope_{N}.npz; ddtree vs sequential relative L1 per field: "
+        f"{out['seq_diff_l1'] / out['seq_l1']}")
+
+
+if __name__ == "__main__":
+    main()
