@@ -189,7 +189,10 @@ int sor_fp64(GravitySolver3D* s) {
     std::vector<SorLaunch> launches;
 
     auto single = [&](int it) -> int {
-        if (!rel) HYD_CHECK(hyd::launch_sor_prepare(d->ctl, it, d->stream));
+        // a single sweep clears the slot of the iteration after it; only a
+        // fused pair leaves the slot of the next iteration to be cleared
+        if (!rel && !launches.empty() && launches.back().count == 2)
+            HYD_CHECK(hyd::launch_sor_prepare(d->ctl, it, d->stream));
         HYD_CHECK(hyd::launch_sor_sweep(d->g, c, 0, it, stol, buf_r[cur], buf_b[cur], d->rhs_r,
                                         d->rhs_b, d->ctl, d->stream));
         HYD_CHECK(hyd::launch_sor_sweep(d->g, c, 1, it, stol, buf_r[cur], buf_b[cur], d->rhs_r,
@@ -218,7 +221,12 @@ int sor_fp64(GravitySolver3D* s) {
             d->phi2_b = buf_b[1 - keep];
         }
     };
-    int batch = 8;
+    // Under the max-update rule a warm start usually needs about as many
+    // iterations as the last solve, and iterations past convergence return
+    // at once: one batch of that size ends most solves with a single host
+    // round trip.  (The relative-residual rule stops only between batches.)
+    int batch = (!rel && s->last_iterations > 1 && s->last_iterations < 4096) ? s->last_iterations
+                                                                             : 8;
     double rel_prev = -1.0;
     int it_prev = 0;
     while (true) {

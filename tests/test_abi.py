@@ -219,3 +219,17 @@ def test_initial_state_file_roundtrip(tmp_path):
     path = str(tmp_path / "s.bin")
     ics.write_ic_file(path, U)
     assert np.array_equal(ics.read_ic_file(path), U)
+
+
+@pytest.mark.skipif(not os.path.exists(os.path.join(REF, "main_3d.c")),
+                    reason="reference sources not present")
+def test_reference_cli_builds_against_the_drop_in(tmp_path):
+    """The reference's unmodified command-line driver compiles against
+    include/ and links against the library (drop-in at the source level)."""
+    exe = tmp_path / "hydro3d"
+    cmd = ["gcc", "-std=c11", "-D_DEFAULT_SOURCE", "-I", INC, "-o", str(exe),
+           os.path.join(REF, "main_3d.c"), "-L", os.path.dirname(H.LIB_PATH), "-lhydro_mi355x",
+           "-Wl,-rpath," + os.path.dirname(H.LIB_PATH), "-lm"]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+    assert exe.exists()

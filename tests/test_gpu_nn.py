@@ -5,7 +5,7 @@ import numpy as np
 import pytest
 
 import hydrocode_astro1_amd as H
-from hydrocode_astro1_amd import nn
+from hydrocode_astro1_amd import ics, nn
 import nn_reference as R
 
 pytestmark = pytest.mark.gpu
@@ -144,3 +144,30 @@ def test_guess_or_warm_start(warm):
         assert s.last_guess == 0  # the converged potential beats any guess
     assert s.last_rel_residual <= 1e-8
     assert rel_l2(g.phi(), phi_ref) < 1e-7
+
+
+def test_deck_attaches_the_network(tmp_path):
+    """poisson_mode = nn_fp32dc with nn_guess = <weights>: the solver made
+    from the deck owns a loaded network and its first step keeps the better
+    start and converges."""
+    n = 32
+    deck = tmp_path / "deck.txt"
+    deck.write_text(f"nx = {n}\nny = {n}\nnz = {n}\nG_const = 1.0\ngamma = 2.0\n"
+                    f"sor_tol = 1e-8\nsor_criterion = rel_resid\npoisson_mode = nn_fp32dc\n"
+                    f"nn_guess = {nn.DEFAULT_WEIGHTS}\noutput_enable = false\n")
+    L = H.lib()
+    d = L.input_deck_create(str(deck).encode())
+    h = L.solver3d_create_from_deck(d)
+    L.input_deck_destroy(d)
+    assert h
+    s = H.Solver(0, 0, 0, handle=h)
+    g = s.s.gravity_solver.contents
+    assert g.poisson_mode == H.POISSON_NN_FP32_DC and g.nn_model
+    X, Y, Z = ics.centres((n, n, n))
+    rho = 1e-3 + np.exp(-((X - 0.5) ** 2 + (Y - 0.5) ** 2 + (Z - 0.5) ** 2) / 0.01)
+    U = ics.prim_to_cons(rho, 0 * X, 0 * X, 0 * X, 0.1 * rho ** 2, 2.0)
+    s.set_state(U)
+    assert s.step() == 0
+    assert g.last_status == 0 and g.last_guess in (0, 1)
+    assert g.last_rel_residual <= 1e-8
+    s.close()
