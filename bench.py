@@ -87,6 +87,23 @@ def cpu_baseline(U, n, budget_s=20.0):
             "ms_per_step": 1e3 * el / steps}
 
 
+def network_timing(H, model, n=128, reps=5):
+    """Device time of the guess network at 128^3 (where it runs) and its
+    MFMA use: flops = 2 * 27 * cin * cout per output voxel and layer; a
+    training pass (forward + input and weight gradients) is 3x that."""
+    import ctypes as C
+    f, fb = C.c_double(0.0), C.c_double(0.0)
+    if H.lib().nn_benchmark(model.h, n, reps, C.byref(f), C.byref(fb)) != 0:
+        return {"timing_error": True}
+    flops = H.lib().nn_forward_flops(n)
+    peak = 2.5e15  # MI355X dense BF16 MFMA
+    return {"grid": n, "forward_ms": f.value, "forward_backward_ms": fb.value,
+            "forward_gflop": flops / 1e9,
+            "forward_tflops": flops / (f.value * 1e-3) / 1e12,
+            "forward_mfma_frac": flops / (f.value * 1e-3) / peak,
+            "training_tflops": 3 * flops / (fb.value * 1e-3) / 1e12}
+
+
 def poisson_block(H, ics, n):
     """256^3 Poisson solves to ||f - L phi|| / ||f|| <= 1e-8 (interior).
 
@@ -122,6 +139,7 @@ def poisson_block(H, ics, n):
         gr.set_nn_model(model)
         out["network"] = {"weights": os.path.relpath(nn.DEFAULT_WEIGHTS, ROOT),
                           "parameters": model.num_params()}
+        out["network"].update(network_timing(H, model))
     else:
         out["network"] = {"weights": None, "note": "no trained weights; network arms skipped"}
 
@@ -218,6 +236,8 @@ def main():
     ap.add_argument("--poisson-probe", type=int, default=0, metavar="ITERS",
                     help="only time ITERS iterations of plain FP64 SOR and of FP32 defect "
                          "correction on the 256^3 explosion density (twice each), then exit")
+    ap.add_argument("--selfgravity-probe", type=int, default=0, metavar="STEPS",
+                    help="only run the self-gravity step block with STEPS warm steps, then exit")
     ap.add_argument("--calibrate", action="store_true",
                     help="only run the device layout-conversion copy (40 B read + 40 B written "
                          "per cell) used to scale HBM byte counters, then exit")
@@ -235,6 +255,10 @@ def main():
         print("no GPU visible", file=sys.stderr)
         sys.exit(1)
     n = args.n
+    if args.selfgravity_probe > 0:
+        print(json.dumps({"selfgravity_step": selfgravity_block(H, ics, n,
+                                                                args.selfgravity_probe)}))
+        return
     if args.poisson_probe > 0:
         import numpy as np
         rho = np.ascontiguousarray(ics.explosion_n3(n)[..., 0])

@@ -143,6 +143,8 @@ EXPORTS = [
     ("nn_train", C.c_int, [C.c_void_p, C.c_void_p, _ip, C.c_int, C.c_int, C.c_float, _fp]),
     ("nn_evaluate", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, _fp]),
     ("nn_probe_mfma", C.c_int, [_fp]),
+    ("nn_benchmark", C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _dp]),
+    ("nn_forward_flops", C.c_double, [C.c_int]),
 ]
 
 _lib = None

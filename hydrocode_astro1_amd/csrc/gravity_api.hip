@@ -225,8 +225,9 @@ int sor_fp64(GravitySolver3D* s) {
     // iterations as the last solve, and iterations past convergence return
     // at once: one batch of that size ends most solves with a single host
     // round trip.  (The relative-residual rule stops only between batches.)
-    int batch = (!rel && s->last_iterations > 1 && s->last_iterations < 4096) ? s->last_iterations
-                                                                             : 8;
+    // (Capped: after a cold solve the next warm one is far shorter, and each
+    // launch past convergence still costs a few microseconds.)
+    int batch = (!rel && s->last_iterations > 1) ? std::min(s->last_iterations, 128) : 8;
     double rel_prev = -1.0;
     int it_prev = 0;
     while (true) {

@@ -651,6 +651,68 @@ int nn_evaluate(void* hm, void* hd, int first, int count, float* mse) {
     return 0;
 }
 
+// Device time of the network on an n^3 grid (features of a synthetic
+// density already on the device): the average of `reps` forward passes and
+// of `reps` forward + backward passes (loss against a zero target), in ms.
+int nn_benchmark(void* hm, int n, int reps, double* fwd_ms, double* fwdbwd_ms) {
+    using namespace nnh;
+    Model* M = (Model*)hm;
+    if (!grid_ok(n) || reps < 1) return -1;
+    if (alloc_acts(M->acts, n)) return -2;
+    const long nv = vox(n);
+    float *rho = nullptr, *tgt = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = 0;
+    if (hipMalloc((void**)&rho, sizeof(float) * nv) || hipMalloc((void**)&tgt, sizeof(float) * nv) ||
+        hipEventCreate(&e0) || hipEventCreate(&e1))
+        rc = -2;
+    // a smooth positive density and the moments of a unit mass at the centre
+    const double mom[4] = {1.0, 0.5, 0.5, 0.5};
+    const double box[4] = {0.0, 0.0, 0.0, 1.0};
+    std::vector<float> h(nv);
+    for (long v = 0; v < nv; ++v) h[v] = 1.0f + 0.5f * (float)((v * 2654435761u) % 1000) / 1000.0f;
+    if (!rc && (hipMemcpy(rho, h.data(), sizeof(float) * nv, hipMemcpyHostToDevice) ||
+                hipMemset(tgt, 0, sizeof(float) * nv) ||
+                hipMemcpy(M->mom_tmp, mom, sizeof(mom), hipMemcpyHostToDevice)))
+        rc = -2;
+    if (!rc && (features(*M, rho, n, M->mom_tmp, box) || forward(*M, n))) rc = -2;  // warm-up
+    float ms = 0.f;
+    if (!rc && hipEventRecord(e0, M->st)) rc = -2;
+    for (int r = 0; !rc && r < reps; ++r)
+        if (forward(*M, n)) rc = -2;
+    if (!rc && (hipEventRecord(e1, M->st) || hipEventSynchronize(e1) ||
+                hipEventElapsedTime(&ms, e0, e1)))
+        rc = -2;
+    if (!rc) *fwd_ms = ms / reps;
+    if (!rc && hipEventRecord(e0, M->st)) rc = -2;
+    for (int r = 0; !rc && r < reps; ++r) {
+        if (forward(*M, n) || mse_loss(M->acts.out, tgt, nv, 1.0f / nv, M->acts.gout,
+                                       M->acts.loss, M->st) ||
+            backward(*M, n))
+            rc = -2;
+    }
+    if (!rc && (hipEventRecord(e1, M->st) || hipEventSynchronize(e1) ||
+                hipEventElapsedTime(&ms, e0, e1)))
+        rc = -2;
+    if (!rc) *fwdbwd_ms = ms / reps;
+    if (rho) (void)hipFree(rho);
+    if (tgt) (void)hipFree(tgt);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    return rc;
+}
+
+// Multiply-add flops of one forward pass on an n^3 grid counting real
+// channels only: 2 * 27 * cin * cout per output voxel of each convolution.
+double nn_forward_flops(int n) {
+    using namespace nnh;
+    const int lvl[kNL] = {0, 0, 1, 1, 2, 2, 3, 3, 2, 1, 0, 0};  // output grid level
+    double f = 0.0;
+    for (int l = 0; l < kNL; ++l)
+        f += 2.0 * 27.0 * kLayers[l].cin * kLayers[l].cout * (double)vox(n >> lvl[l]);
+    return f;
+}
+
 int nn_probe_mfma(float* out256) {
     hipStream_t st = hyd_stream();
     float* d = nullptr;

@@ -30,69 +30,82 @@ __device__ __forceinline__ void tap_offset(int t, int& dx, int& dy, int& dz) {
 }
 
 // ------------------------------------------------------------- forward --
-// One wavefront computes 64 consecutive voxels (four 16-voxel M tiles) for
-// all output channels (one or two 16-channel N tiles).
+// The convolution with the input tile staged in LDS.  A block owns a
+// 4 (x) x 4 (y) x 16 (z) brick of output voxels; wavefront w takes x slice w
+// and its four y rows as four 16-voxel M tiles along z.  The brick's input,
+// widened by one voxel on every side (upsampled on the fly for the decoder
+// layers, zero outside the grid), is read from HBM once into LDS, and every
+// A fragment then comes from LDS instead of a 27-fold global gather.
+constexpr int BX = 4, BY = 4, BZ = 16;
+constexpr int LX = BX + 2, LY = BY + 2, LZ = BZ + 2;
+
 template <int CINP>
-__global__ void __launch_bounds__(256) conv_fwd_kernel(ConvArgs a) {
+__global__ void __launch_bounds__(256) conv_fwd_lds_kernel(ConvArgs a) {
     constexpr int KSTEPS = (27 * CINP + 15) / 16;
+    constexpr int G4 = CINP / 4;  // 4-channel groups per voxel
+    __shared__ v4s s_in[LX * LY * LZ * G4];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int n = a.n;
-    const long nvox = (long)n * n * n;
-    const long vbase = ((long)blockIdx.x * 4 + wave) * 64;
-    if (vbase >= nvox) return;
-    const int ntiles = (a.cout + 15) / 16;
+    const int nbz = (n + BZ - 1) / BZ, nby = (n + BY - 1) / BY;
+    const int bz = blockIdx.x % nbz, by = (blockIdx.x / nbz) % nby, bx = blockIdx.x / (nbz * nby);
+    const int x0 = bx * BX, y0 = by * BY, z0 = bz * BZ;
     const int nin = a.up ? n / 2 : n;
-    const int g = lane >> 4, m = lane & 15;
 
-    int vi[4], vj[4], vk[4];
-    bool vin[4];
-#pragma unroll
-    for (int mt = 0; mt < 4; ++mt) {
-        const long v = vbase + mt * 16 + m;
-        vin[mt] = v < nvox;
-        vk[mt] = (int)(v % n);
-        vj[mt] = (int)((v / n) % n);
-        vi[mt] = (int)(v / ((long)n * n));
+    // stage the input brick: (x, y, z, group) entries, z and group fastest
+    const v4s* xin = reinterpret_cast<const v4s*>(a.x);
+    for (int e = threadIdx.x; e < LX * LY * LZ * G4; e += 256) {
+        const int q = e % G4, vz = (e / G4) % LZ, vy = (e / (G4 * LZ)) % LY,
+                  vx = e / (G4 * LZ * LY);
+        const int ii = x0 - 1 + vx, jj = y0 - 1 + vy, kz = z0 - 1 + vz;
+        v4s val = v4s{0, 0, 0, 0};
+        if (ii >= 0 && ii < n && jj >= 0 && jj < n && kz >= 0 && kz < n) {
+            const int si = a.up ? ii >> 1 : ii, sj = a.up ? jj >> 1 : jj, sk = a.up ? kz >> 1 : kz;
+            val = xin[(((long)si * nin + sj) * nin + sk) * G4 + q];
+        }
+        s_in[e] = val;
     }
+    __syncthreads();
+
+    const int g = lane >> 4, m = lane & 15;
+    const int ntiles = (a.cout + 15) / 16;
     v4f acc[4][2];
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt)
         for (int nt = 0; nt < 2; ++nt) acc[mt][nt] = v4f{0.f, 0.f, 0.f, 0.f};
-
     const v4s* wp = reinterpret_cast<const v4s*>(a.wp);
     for (int ks = 0; ks < KSTEPS; ++ks) {
         const int kk = (ks * 4 + g) * 4;
         const int t = kk / CINP, ci = kk % CINP;
         int dx = 0, dy = 0, dz = 0;
         if (t < 27) tap_offset(t, dx, dy, dz);
-        v4s b0 = wp[(0 * KSTEPS + ks) * 64 + lane];
+        const v4s b0 = wp[(0 * KSTEPS + ks) * 64 + lane];
         v4s b1 = v4s{0, 0, 0, 0};
         if (ntiles > 1) b1 = wp[(1 * KSTEPS + ks) * 64 + lane];
 #pragma unroll
         for (int mt = 0; mt < 4; ++mt) {
+            // voxel (x0 + wave, y0 + mt, z0 + m) shifted by the tap, in the brick
             v4s av = v4s{0, 0, 0, 0};
-            const int ii = vi[mt] + dx, jj = vj[mt] + dy, kz = vk[mt] + dz;
-            if (t < 27 && vin[mt] && ii >= 0 && ii < n && jj >= 0 && jj < n && kz >= 0 && kz < n) {
-                const int si = a.up ? ii >> 1 : ii, sj = a.up ? jj >> 1 : jj,
-                          sk = a.up ? kz >> 1 : kz;
-                const long idx = (((long)si * nin + sj) * nin + sk) * CINP + ci;
-                av = *reinterpret_cast<const v4s*>(a.x + idx);
-            }
+            if (t < 27)
+                av = s_in[(((wave + 1 + dx) * LY + (mt + 1 + dy)) * LZ + (m + 1 + dz)) * G4 +
+                          ci / 4];
             acc[mt][0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, b0, acc[mt][0], 0, 0, 0);
             if (ntiles > 1)
                 acc[mt][1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, b1, acc[mt][1], 0, 0, 0);
         }
     }
-    // epilogue: bias, ReLU, optional skip add, stores
+    // epilogue: bias, ReLU, optional skip add, stores (D row = z offset)
+    const int xi = x0 + wave;
 #pragma unroll
     for (int mt = 0; mt < 4; ++mt)
         for (int nt = 0; nt < 2; ++nt) {
             if (nt >= ntiles) break;
             const int c = nt * 16 + m;
+            const int yj = y0 + mt;
 #pragma unroll
             for (int i = 0; i < 4; ++i) {
-                const long v = vbase + mt * 16 + g * 4 + i;
-                if (v >= nvox || c >= a.coutp) continue;
+                const int zk = z0 + g * 4 + i;
+                if (xi >= n || yj >= n || zk >= n || c >= a.coutp) continue;
+                const long v = ((long)xi * n + yj) * n + zk;
                 float val = acc[mt][nt][i];
                 if (c < a.cout) {
                     if (a.bias) val += a.bias[c];
@@ -114,14 +127,14 @@ __global__ void __launch_bounds__(256) conv_fwd_kernel(ConvArgs a) {
 hipError_t conv_forward(const ConvArgs& a, hipStream_t st) {
     if (a.cout < 1 || a.cout > 32 || a.coutp < a.cout || a.n < 1 || (a.up && a.n % 2))
         return hipErrorInvalidValue;  // one wavefront covers at most two 16-channel tiles
-    const long nvox = (long)a.n * a.n * a.n;
-    const unsigned nb = (unsigned)((nvox + 255) / 256);
+    const long nbricks = (long)((a.n + BX - 1) / BX) * ((a.n + BY - 1) / BY) * ((a.n + BZ - 1) / BZ);
+    const unsigned nb = (unsigned)nbricks;
     switch (a.cinp) {
-        case 4: hipLaunchKernelGGL(conv_fwd_kernel<4>, dim3(nb), dim3(256), 0, st, a); break;
-        case 8: hipLaunchKernelGGL(conv_fwd_kernel<8>, dim3(nb), dim3(256), 0, st, a); break;
-        case 12: hipLaunchKernelGGL(conv_fwd_kernel<12>, dim3(nb), dim3(256), 0, st, a); break;
-        case 16: hipLaunchKernelGGL(conv_fwd_kernel<16>, dim3(nb), dim3(256), 0, st, a); break;
-        case 24: hipLaunchKernelGGL(conv_fwd_kernel<24>, dim3(nb), dim3(256), 0, st, a); break;
+        case 4: hipLaunchKernelGGL(conv_fwd_lds_kernel<4>, dim3(nb), dim3(256), 0, st, a); break;
+        case 8: hipLaunchKernelGGL(conv_fwd_lds_kernel<8>, dim3(nb), dim3(256), 0, st, a); break;
+        case 12: hipLaunchKernelGGL(conv_fwd_lds_kernel<12>, dim3(nb), dim3(256), 0, st, a); break;
+        case 16: hipLaunchKernelGGL(conv_fwd_lds_kernel<16>, dim3(nb), dim3(256), 0, st, a); break;
+        case 24: hipLaunchKernelGGL(conv_fwd_lds_kernel<24>, dim3(nb), dim3(256), 0, st, a); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();

@@ -129,6 +129,11 @@ int nn_dataset_get(void *dataset, int index, float *rho, float *phi, double *mom
 int nn_train(void *model, void *dataset, const int *order, int steps, int batch, float lr,
              float *loss_log);
 int nn_evaluate(void *model, void *dataset, int first, int count, float *mse);
+/* Average device ms of a forward pass and of a forward + backward pass of
+ * the network on an n^3 grid (device-resident synthetic input), and the
+ * flops of one forward pass (2 * 27 * cin * cout per output voxel). */
+int nn_benchmark(void *model, int n, int reps, double *fwd_ms, double *fwdbwd_ms);
+double nn_forward_flops(int n);
 /* MFMA operand-layout probe: D = A I with A[m][k] = 16 m + k (256 floats). */
 int nn_probe_mfma(float *out256);
 /* Attach a network to a solver for POISSON_NN_FP32_DC (NULL detaches; the
