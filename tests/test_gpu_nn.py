@@ -55,16 +55,21 @@ def test_forward_backward_match_torch(seed):
     assert np.max(np.abs(feat[..., 1] - 0.1 * mono)) <= 2 ** -8 * np.max(np.abs(0.1 * mono))
 
     lp = m.layer_params(p)
-    rloss, rout, rgrads = R.loss_and_grads(lp, feat, mono, target)
+    # the reference rounds gradients to BF16 where the device stores them;
+    # what is left is FP32 sums taken in another order that land on the
+    # other side of a BF16 rounding boundary (measured: output 4e-4, weight
+    # and bias gradients at most 1.1e-2 relative L2; without the gradient
+    # rounding the coarsest layers differ by 6%)
+    rloss, rout, rgrads = R.loss_and_grads(lp, feat, mono, target, grad_rounding=True)
     assert rel_l2(out, rout) < 1e-2
-    assert loss == pytest.approx(rloss, rel=3e-2)
+    assert loss == pytest.approx(rloss, rel=3e-3)
     glp = m.layer_params(grads)
     for l, ((gw, gb), (rw, rb)) in enumerate(zip(glp, rgrads)):
-        assert rel_l2(gw, rw) < 6e-2, f"layer {l} weight gradient"
-        assert rel_l2(gb, rb) < 6e-2, f"layer {l} bias gradient"
+        assert rel_l2(gw, rw) < 3e-2, f"layer {l} weight gradient"
+        assert rel_l2(gb, rb) < 3e-2, f"layer {l} bias gradient"
         cos = float(np.dot(gw.ravel(), rw.ravel()) /
                     (np.linalg.norm(gw) * np.linalg.norm(rw) + 1e-30))
-        assert cos > 0.998, f"layer {l} weight gradient direction"
+        assert cos > 0.9995, f"layer {l} weight gradient direction"
 
 
 @pytest.fixture(scope="module")

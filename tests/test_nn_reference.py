@@ -84,3 +84,20 @@ def test_training_fields_are_seeded():
     assert np.all((blobs[:, 4] >= 0.03) & (blobs[:, 4] <= 0.15))
     # lognormal factor: std of delta equals sigma
     assert np.sqrt(0.5 * np.sum(modes[:, 4] ** 2)) == pytest.approx(sigma, rel=1e-12)
+
+
+def test_gradient_rounding_is_identity_forward_bf16_backward():
+    """gbf16 leaves values alone and rounds the incoming gradient to the
+    nearest BF16 (ties to even): 1 + 2^-8 is a tie that rounds down to 1,
+    1 + 3 * 2^-8 rounds up to 1 + 2^-6."""
+    x = torch.tensor([0.5, -2.0, 3.0], requires_grad=True)
+    y = R.gbf16(x)
+    assert torch.equal(y, x)
+    c = torch.tensor([1.0 + 2.0 ** -8, -(1.0 + 3 * 2.0 ** -8), 0.1])
+    (y * c).sum().backward()
+    expect = torch.tensor([1.0, -(1.0 + 2.0 ** -6), 0.10009765625])
+    assert torch.equal(x.grad, expect)
+    # disabled: the gradient passes unchanged
+    x.grad = None
+    (R.gbf16(x, enabled=False) * c).sum().backward()
+    assert torch.equal(x.grad, c)
