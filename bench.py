@@ -14,8 +14,10 @@ its duration is the HIP-event average of that kernel on the library's stream.
 The cpu_baseline block times the reference C solver (built from its own
 sources, one core) on a bounded sample of the same workload.  Unless
 --no-poisson, a "poisson" block reports 256^3 Poisson solves to a 1e-8
-relative residual: plain FP64 SOR vs FP32 defect correction vs the learned
-guess + FP32 defect correction, from a warm and from a cold start.
+relative residual: plain FP64 SOR vs FP32 defect correction (inner solves by
+red-black SOR sweeps, or by multigrid V-cycles of red-black sweeps) with and
+without the learned guess, from a warm and from a cold start, and a
+"selfgravity_step" block times whole steps with self-gravity.
 """
 import argparse
 import json
@@ -143,13 +145,14 @@ def poisson_block(H, ics, n):
     else:
         out["network"] = {"weights": None, "note": "no trained weights; network arms skipped"}
 
-    def arm(mode, phi_start, rho):
+    def arm(mode, phi_start, rho, inner=H.DC_INNER_SOR):
         gr.set_mode(mode)
+        gr.set_dc_inner(inner)
         gr.set_phi(phi_start)
         st = gr.solve_potential(rho)
         it, _ = gr.diagnostics()
-        r = {"status": st, "iterations": it, "ms": gr.s.last_solve_ms,
-             "rel_residual": gr.s.last_rel_residual}
+        r = {"status": st, "ms": gr.s.last_solve_ms, "rel_residual": gr.s.last_rel_residual}
+        r["v_cycles" if inner == H.DC_INNER_MULTIGRID else "iterations"] = it
         if mode == H.POISSON_NN_FP32_DC:
             r["guess_used"] = bool(gr.s.last_guess == 1)
             r["guess_rel_residual"] = gr.s.last_guess_rel_residual
@@ -157,15 +160,23 @@ def poisson_block(H, ics, n):
         return r, gr.phi()
 
     warm, phis = {}, {}
-    arms = [("plain_fp64_sor", H.POISSON_FP64_SOR), ("fp32_dc", H.POISSON_FP32_DC)]
+    MG = H.DC_INNER_MULTIGRID
+    arms = [("plain_fp64_sor", H.POISSON_FP64_SOR, H.DC_INNER_SOR),
+            ("fp32_dc", H.POISSON_FP32_DC, H.DC_INNER_SOR),
+            ("fp32_dc_multigrid", H.POISSON_FP32_DC, MG)]
     if model is not None:
-        arms.append(("nn_fp32_dc", H.POISSON_NN_FP32_DC))
-    for name, mode in arms:
-        warm[name], phis[name] = arm(mode, phi0, rho1)
+        arms += [("nn_fp32_dc", H.POISSON_NN_FP32_DC, H.DC_INNER_SOR),
+                 ("nn_fp32_dc_multigrid", H.POISSON_NN_FP32_DC, MG)]
+    for name, mode, inner in arms:
+        warm[name], phis[name] = arm(mode, phi0, rho1, inner)
     coldres, cphis = {"plain_fp64_sor": cold}, {}
+    coldres["fp32_dc_multigrid"], cphis["fp32_dc_multigrid"] = arm(
+        H.POISSON_FP32_DC, np.zeros_like(phi0), rho0, MG)
     if model is not None:
-        coldres["nn_fp32_dc"], cphis["nn_fp32_dc"] = arm(H.POISSON_NN_FP32_DC,
-                                                         np.zeros_like(phi0), rho0)
+        for name, inner in (("nn_fp32_dc", H.DC_INNER_SOR), ("nn_fp32_dc_multigrid", MG)):
+            coldres[name], cphis[name] = arm(H.POISSON_NN_FP32_DC, np.zeros_like(phi0), rho0,
+                                             inner)
+    gr.set_dc_inner(H.DC_INNER_SOR)
     # tight FP64 solutions as the accuracy yardstick
     gr.set_mode(H.POISSON_FP64_SOR)
     gr.set_params(1.5, 400000, 1e-11)
@@ -185,44 +196,64 @@ def poisson_block(H, ics, n):
     out["warm_start"] = warm
     out["cold_start"] = coldres
     plain = warm["plain_fp64_sor"]["ms"]
-    if warm["fp32_dc"]["ms"] > 0:
-        out["speedup_fp32dc_vs_plain"] = plain / warm["fp32_dc"]["ms"]
-    if "nn_fp32_dc" in warm and warm["nn_fp32_dc"]["ms"] > 0:
-        out["speedup_nn_fp32dc_vs_plain"] = plain / warm["nn_fp32_dc"]["ms"]
-    if "nn_fp32_dc" in coldres and coldres["nn_fp32_dc"]["ms"] > 0:
-        out["speedup_cold_nn_fp32dc_vs_plain"] = cold["ms"] / coldres["nn_fp32_dc"]["ms"]
+    out["speedup_warm_vs_plain"] = {k: plain / v["ms"] for k, v in warm.items()
+                                    if k != "plain_fp64_sor" and v["ms"] > 0}
+    out["speedup_cold_vs_plain"] = {k: cold["ms"] / v["ms"] for k, v in coldres.items()
+                                    if k != "plain_fp64_sor" and v["ms"] > 0}
     gr.set_nn_model(None)
     s.close()
     return out
 
 
 def selfgravity_block(H, ics, n, steps=10):
-    """The full step with self-gravity on the same 256^3 state: the
-    reference's Poisson settings (red-black SOR, omega 1.5, max |update| <
-    1e-6, at most 10000 iterations) with the l = 6 multipole boundary; the
-    first step solves cold, the next `steps` warm-start from the previous
-    potential."""
+    """The full step with self-gravity on the same 256^3 state, l = 6
+    multipole boundary.  "reference_sor": the reference's Poisson settings
+    (red-black SOR, omega 1.5, max |update| < 1e-6, at most 10000
+    iterations).  "nn_fp32_dc_multigrid": relative residual 1e-8, the learned
+    guess (kept only when its residual beats the warm start) and FP32 defect
+    correction with multigrid inner solves.  The first step solves cold, the
+    next `steps` warm-start from the previous potential."""
+    from hydrocode_astro1_amd import nn
+
     U = ics.explosion_n3(n)
-    s = H.Solver(n, n, n, G=1.0, gamma=5.0 / 3.0, cfl=0.4, bc=(H.BC_OUTFLOW,) * 6)
-    gr = s.gravity
-    gr.set_lmax(6)
-    gr.set_params(1.5, 10000, 1e-6)
-    s.set_state(U)
-    t0 = time.perf_counter()
-    s.step()
-    cold_ms = 1e3 * (time.perf_counter() - t0)
-    cold_it = gr.diagnostics()[0]
-    its, ms = [], []
-    for _ in range(steps):
+
+    def run(setup):
+        s = H.Solver(n, n, n, G=1.0, gamma=5.0 / 3.0, cfl=0.4, bc=(H.BC_OUTFLOW,) * 6)
+        gr = s.gravity
+        gr.set_lmax(6)
+        info = setup(gr)
+        s.set_state(U)
         t0 = time.perf_counter()
-        s.step()  # returns after reading dt back: the step is complete
-        ms.append(1e3 * (time.perf_counter() - t0))
-        its.append(gr.diagnostics()[0])
-    s.close()
-    return {"grid": n, "sor": {"omega": 1.5, "tol_max_update": 1e-6, "max_iter": 10000},
-            "multipole_lmax": 6, "first_step": {"ms": cold_ms, "sor_iterations": cold_it},
-            "warm_steps": steps, "warm_ms_per_step": sum(ms) / len(ms),
-            "warm_sor_iterations": its}
+        s.step()
+        cold_ms = 1e3 * (time.perf_counter() - t0)
+        cold_it = gr.diagnostics()[0]
+        its, ms = [], []
+        for _ in range(steps):
+            t0 = time.perf_counter()
+            s.step()  # returns after reading dt back: the step is complete
+            ms.append(1e3 * (time.perf_counter() - t0))
+            its.append(gr.diagnostics()[0])
+        gr.set_nn_model(None)
+        s.close()
+        info.update({"first_step": {"ms": cold_ms, "iterations": cold_it},
+                     "warm_ms_per_step": sum(ms) / len(ms), "warm_iterations": its})
+        return info
+
+    def reference(gr):
+        gr.set_params(1.5, 10000, 1e-6)
+        return {"sor": {"omega": 1.5, "tol_max_update": 1e-6, "max_iter": 10000}}
+
+    def learned_multigrid(gr):
+        gr.set_params(1.5, 1000, 1e-8)
+        gr.set_criterion(H.SOR_CRIT_REL_RESID)
+        gr.set_mode(H.POISSON_NN_FP32_DC)
+        gr.set_dc_inner(H.DC_INNER_MULTIGRID)
+        if os.path.exists(nn.DEFAULT_WEIGHTS):
+            gr.set_nn_model(nn.Model(path=nn.DEFAULT_WEIGHTS))
+        return {"tol_rel_residual": 1e-8, "iterations_are": "V-cycles"}
+
+    return {"grid": n, "multipole_lmax": 6, "warm_steps": steps,
+            "reference_sor": run(reference), "nn_fp32_dc_multigrid": run(learned_multigrid)}
 
 
 def main():
@@ -242,6 +273,23 @@ def main():
                     help="only run the device layout-conversion copy (40 B read + 40 B written "
                          "per cell) used to scale HBM byte counters, then exit")
     args = ap.parse_args()
+    # the library prints the reference's progress messages on stdout: send
+    # them to stderr while the work runs, so that stdout carries one JSON line
+    sys.stdout.flush()
+    saved = os.dup(1)
+    os.dup2(2, 1)
+    try:
+        out = run(args)
+    finally:
+        import ctypes
+        ctypes.CDLL(None).fflush(None)
+        sys.stdout.flush()
+        os.dup2(saved, 1)
+        os.close(saved)
+    print(json.dumps(out))
+
+
+def run(args):
     if args.gpus != 1:
         print(f"bench.py measures one GPU; --gpus {args.gpus} is not supported (the path "
               f"would shard as x slabs with halo planes and a dt all-reduce; that is not "
@@ -256,9 +304,7 @@ def main():
         sys.exit(1)
     n = args.n
     if args.selfgravity_probe > 0:
-        print(json.dumps({"selfgravity_step": selfgravity_block(H, ics, n,
-                                                                args.selfgravity_probe)}))
-        return
+        return {"selfgravity_step": selfgravity_block(H, ics, n, args.selfgravity_probe)}
     if args.poisson_probe > 0:
         import numpy as np
         rho = np.ascontiguousarray(ics.explosion_n3(n)[..., 0])
@@ -274,17 +320,15 @@ def main():
             it = g.diagnostics()[0]
             out.append({"mode": int(mode), "iterations": it, "ms": g.s.last_solve_ms,
                         "us_per_iteration": 1e3 * g.s.last_solve_ms / max(it, 1)})
-        print(json.dumps({"poisson_probe": out, "grid": n}))
-        return
+        return {"poisson_probe": out, "grid": n}
     if args.calibrate:
         import numpy as np
         s = H.Solver(n, n, n, G=0.0)
         s.set_state(np.ones((n, n, n, 5)))
         for _ in range(3):
             s.state()  # soa_to_aos_kernel: reads and writes 40 B per cell
-        print(json.dumps({"calibration": "soa_to_aos_kernel", "cells": n ** 3,
-                          "bytes_read": 40 * n ** 3, "bytes_written": 40 * n ** 3}))
-        return
+        return {"calibration": "soa_to_aos_kernel", "cells": n ** 3,
+                "bytes_read": 40 * n ** 3, "bytes_written": 40 * n ** 3}
     t0 = time.perf_counter()
     U = ics.explosion_n3(n)
     print(f"[bench] initial state {n}^3 built in {time.perf_counter() - t0:.1f}s", file=sys.stderr,
@@ -335,7 +379,7 @@ def main():
             res["selfgravity_step"] = {"error": repr(e)}
     if not args.no_cpu:
         res["cpu_baseline"] = cpu_baseline(U, n)
-    print(json.dumps(res))
+    return res
 
 
 if __name__ == "__main__":

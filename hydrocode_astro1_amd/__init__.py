@@ -27,6 +27,7 @@ BC_PERIODIC, BC_REFLECTIVE, BC_OUTFLOW = 0, 1, 2
 GRAV_BC_PERIODIC, GRAV_BC_ISOLATED, GRAV_BC_DIRICHLET = 0, 1, 2
 POISSON_FP64_SOR, POISSON_FP32_DC, POISSON_NN_FP32_DC = 0, 1, 2
 SOR_CRIT_MAX_UPDATE, SOR_CRIT_REL_RESID = 0, 1
+DC_INNER_SOR, DC_INNER_MULTIGRID = 0, 1
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -52,6 +53,7 @@ class GravitySolver3D(C.Structure):
         ("last_rel_residual", C.c_double), ("last_solve_ms", C.c_double),
         ("nn_model", C.c_void_p), ("last_guess", C.c_int),
         ("last_guess_rel_residual", C.c_double), ("last_warm_rel_residual", C.c_double),
+        ("dc_inner", C.c_int),
     ]
 
 
@@ -116,6 +118,7 @@ EXPORTS = [
     ("hydro_check_division", C.c_long, [_dp, _dp, C.c_long]),
     ("gravity_set_multipole_lmax", C.c_int, [_GP, C.c_int]),
     ("gravity_set_poisson_mode", C.c_int, [_GP, C.c_int]),
+    ("gravity_set_dc_inner", C.c_int, [_GP, C.c_int]),
     ("gravity_set_criterion", C.c_int, [_GP, C.c_int]),
     ("gravity_sync_to_device", C.c_int, [_GP]),
     ("gravity_sync_to_host", C.c_int, [_GP]),
@@ -219,6 +222,11 @@ class Gravity:
     def set_mode(self, mode):
         if self.L.gravity_set_poisson_mode(self.h, mode) != 0:
             raise ValueError(f"bad mode {mode}")
+
+    def set_dc_inner(self, inner):
+        """DC_INNER_SOR or DC_INNER_MULTIGRID for the FP32 modes."""
+        if self.L.gravity_set_dc_inner(self.h, inner) != 0:
+            raise ValueError(f"bad inner solver {inner} for this grid")
 
     def set_nn_model(self, model):
         """Attach an nn.Model for POISSON_NN_FP32_DC (None detaches)."""

@@ -49,6 +49,11 @@ void dev_free(GravDevice* d) {
                     d->phi2_r, d->phi2_b, d->e2_r, d->e2_b};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
+    for (auto* v : {&d->mg_u, &d->mg_f, &d->mg_q})
+        for (float* p : *v)
+            if (p) (void)hipFree(p);
+    if (d->mg_qr) (void)hipFree(d->mg_qr);
+    if (d->mg_qb) (void)hipFree(d->mg_qb);
     if (d->ctl_host) (void)hipHostFree(d->ctl_host);
     if (d->acc_host) (void)hipHostFree(d->acc_host);
     if (d->ev0) (void)hipEventDestroy(d->ev0);
@@ -69,6 +74,16 @@ int dev_create(GravitySolver3D* s) {
     d->stream = hyd_stream();
     if (!d->stream) return -2;
     if (const char* e = getenv("HYDRO_MI355X_SOR_FUSED")) d->fused = atoi(e) != 0;
+    if (const char* e = getenv("HYDRO_MI355X_MG_SMOOTH")) {
+        int a = 0, b = 0;
+        float w = 0.0f;
+        if (sscanf(e, "%d,%d,%f", &a, &b, &w) == 3 && a >= 0 && b >= 0 && a + b >= 1 &&
+            w > 0.0f && w < 2.0f) {
+            d->mg_nu1 = a;
+            d->mg_nu2 = b;
+            d->mg_omega = w;
+        }
+    }
     if (dev_alloc(&d->phi_r, d->nsplit) || dev_alloc(&d->phi_b, d->nsplit) ||
         dev_alloc(&d->rhs_r, d->nsplit) || dev_alloc(&d->rhs_b, d->nsplit) ||
         dev_alloc(&d->gx, d->n) || dev_alloc(&d->gy, d->n) || dev_alloc(&d->gz, d->n) ||
@@ -379,6 +394,149 @@ int sor_fp32_dc(GravitySolver3D* s) {
     return rel <= tol ? 0 : -1;
 }
 
+bool mg_supported(const GravitySolver3D* s) {
+    return s->nx % 2 == 0 && s->ny % 2 == 0 && s->nz % 2 == 0 && s->nx >= 8 && s->ny >= 8 &&
+           s->nz >= 8;
+}
+
+// The coarse levels: level l has n / 2^l + 1 points per direction at fine
+// cells 0 and c 2^l - 1 (c >= 1), halving while every direction has an even
+// interval count and at least 5 points.  Per direction, with H = 2^l fine
+// cells: the uniform coefficient 1 / (H h)^2; at index 1 (first interval
+// (H - 1) h) the three-point weights 2 / ((hl + hr) hl) and 2 / ((hl + hr) hr);
+// the interpolation weight (H - 1) / (2H - 1) of the next level's point 1 at
+// index 1; and, with Hf = H / 2 the finer spacing, the restriction weights
+// (Hf - 1, 2 Hf, Hf) / (4 Hf - 1) of the finer points around point 1 (the
+// transpose of interpolation scaled by dual-cell lengths).
+hyd::MgAxis mg_axis(double h, int l) {
+    const double H = (double)(1 << l), Hf = H / 2.0;
+    const double hl = (H - 1.0) * h, hr = H * h, sc = 2.0 / (hl + hr);
+    hyd::MgAxis A;
+    A.a = (float)(1.0 / (hr * hr));
+    A.am1 = (float)(sc / hl);
+    A.ap1 = (float)(sc / hr);
+    A.pw1 = (float)((H - 1.0) / (2.0 * H - 1.0));
+    A.r1[0] = (float)((Hf - 1.0) / (4.0 * Hf - 1.0));
+    A.r1[1] = (float)(2.0 * Hf / (4.0 * Hf - 1.0));
+    A.r1[2] = (float)(Hf / (4.0 * Hf - 1.0));
+    return A;
+}
+
+int ensure_mg(GravitySolver3D* s) {
+    GravDevice* d = s->dev;
+    if (!d->mg.empty()) return 0;
+    hyd::MgLevel L;
+    L.nx = s->nx / 2 + 1;
+    L.ny = s->ny / 2 + 1;
+    L.nz = s->nz / 2 + 1;
+    for (int l = 1;; ++l) {
+        L.ax[0] = mg_axis(s->dx, l);
+        L.ax[1] = mg_axis(s->dy, l);
+        L.ax[2] = mg_axis(s->dz, l);
+        d->mg.push_back(L);
+        const bool halve = (L.nx - 1) % 2 == 0 && (L.ny - 1) % 2 == 0 && (L.nz - 1) % 2 == 0 &&
+                           std::min(L.nx, std::min(L.ny, L.nz)) >= 5;
+        if (!halve) break;
+        L.nx = (L.nx - 1) / 2 + 1;
+        L.ny = (L.ny - 1) / 2 + 1;
+        L.nz = (L.nz - 1) / 2 + 1;
+    }
+    for (const hyd::MgLevel& l : d->mg) {
+        const long n = (long)l.nx * l.ny * l.nz;
+        float *u = nullptr, *f = nullptr, *q = nullptr;
+        if (dev_alloc(&u, n) || dev_alloc(&f, n) || dev_alloc(&q, n)) return -2;
+        d->mg_u.push_back(u);
+        d->mg_f.push_back(f);
+        d->mg_q.push_back(q);
+    }
+    if (dev_alloc(&d->mg_qr, d->nsplit) || dev_alloc(&d->mg_qb, d->nsplit)) return -2;
+    return 0;
+}
+
+int mg_smooth(GravDevice* d, size_t l, int sweeps, float omega) {
+    for (int q = 0; q < sweeps; ++q) {
+        HYD_CHECK(hyd::launch_mg_smooth(d->mg[l], 0, omega, d->mg_u[l], d->mg_f[l], d->stream));
+        HYD_CHECK(hyd::launch_mg_smooth(d->mg[l], 1, omega, d->mg_u[l], d->mg_f[l], d->stream));
+    }
+    return 0;
+}
+
+// V-cycle from coarse level l (index into d->mg) down, on u_l with f_l.
+int mg_coarse_cycle(GravDevice* d, size_t l) {
+    const hyd::MgLevel& L = d->mg[l];
+    if (l + 1 == d->mg.size())  // coarsest: Gauss-Seidel sweeps
+        return mg_smooth(d, l, std::max(L.nx, std::max(L.ny, L.nz)), 1.0f);
+    if (mg_smooth(d, l, d->mg_nu1, d->mg_omega)) return -2;
+    HYD_CHECK(hyd::launch_mg_residual(L, d->mg_u[l], d->mg_f[l], d->mg_q[l], d->stream));
+    HYD_CHECK(hyd::launch_mg_restrict(L, d->mg_q[l], d->mg[l + 1], d->mg_f[l + 1],
+                                      d->mg_u[l + 1], d->stream));
+    if (mg_coarse_cycle(d, l + 1)) return -2;
+    HYD_CHECK(hyd::launch_mg_prolong(d->mg[l + 1], d->mg_u[l + 1], L, d->mg_u[l], d->stream));
+    return mg_smooth(d, l, d->mg_nu2, d->mg_omega);
+}
+
+// One V-cycle on the solver's grid: L e = r (FP32, split layout), e += ...
+int mg_vcycle(GravitySolver3D* s, const SorCoef& c) {
+    GravDevice* d = s->dev;
+    SorCoef cs = c;
+    cs.omega = d->mg_omega;
+    auto sweep = [&](int n) -> int {
+        for (int q = 0; q < n; ++q) {
+            HYD_CHECK(hyd::launch_sor_sweep_f32(d->g, cs, 0, d->e_r, d->e_b, d->r_r, d->r_b,
+                                                d->stream));
+            HYD_CHECK(hyd::launch_sor_sweep_f32(d->g, cs, 1, d->e_r, d->e_b, d->r_r, d->r_b,
+                                                d->stream));
+        }
+        return 0;
+    };
+    if (sweep(d->mg_nu1)) return -2;
+    HYD_CHECK(hyd::launch_mg_fine_residual(d->g, c, d->e_r, d->e_b, d->r_r, d->r_b, d->mg_qr,
+                                           d->mg_qb, d->stream));
+    HYD_CHECK(hyd::launch_mg_restrict_fine(d->g, d->mg_qr, d->mg_qb, d->mg[0], d->mg_f[0],
+                                           d->mg_u[0], d->stream));
+    if (mg_coarse_cycle(d, 0)) return -2;
+    HYD_CHECK(hyd::launch_mg_prolong_fine(d->g, d->mg[0], d->mg_u[0], d->e_r, d->e_b, d->stream));
+    return sweep(d->mg_nu2);
+}
+
+// Defect correction with multigrid inner solves: the outer loop is the one
+// of sor_fp32_dc (FP64 residual, phi += e, relative-residual stop), the
+// inner solve a number of V-cycles sized from the previous round's
+// contraction to aim at a 1e-4 reduction.  last_iterations counts V-cycles.
+int dc_multigrid(GravitySolver3D* s) {
+    GravDevice* d = s->dev;
+    if (ensure_dc_buffers(d) || ensure_mg(s)) return -2;
+    const SorCoef c = coefficients(s);
+    if (apply_boundary(s)) return -2;
+    double rel;
+    if (rel_residual_dev(s, d->r_r, d->r_b, &rel)) return -2;
+    int cycles = 0, k = 3;
+    const double tol = s->tolerance;
+    for (int outer = 0; outer < 64 && rel > tol && cycles < s->max_iterations; ++outer) {
+        const int nk = std::min(k, s->max_iterations - cycles);
+        for (int q = 0; q < nk; ++q)
+            if (mg_vcycle(s, c)) return -2;
+        cycles += nk;
+        HYD_CHECK(hyd::launch_correct(d->g, d->phi_r, d->phi_b, d->e_r, d->e_b, d->stream));
+        double rnew;
+        if (rel_residual_dev(s, d->r_r, d->r_b, &rnew)) return -2;
+        const double red = rnew / rel;
+        if (red > 0.0 && red < 1.0) {
+            const double rate = pow(red, 1.0 / nk);
+            const double want = std::max(1e-4, 0.5 * tol / rnew);
+            k = (int)ceil(log(want) / log(rate));
+            k = std::max(1, std::min(k, 16));
+        } else {
+            k = std::min(2 * k, 16);
+        }
+        rel = rnew;
+    }
+    s->last_iterations = cycles;
+    s->last_residual = rel;
+    s->last_rel_residual = rel;
+    return rel <= tol ? 0 : -1;
+}
+
 // Starting potential of POISSON_NN_FP32_DC: the network's guess replaces
 // the warm start (the potential left by the previous solve) only when its
 // relative residual, with this density's boundary values, is smaller.
@@ -428,7 +586,7 @@ int potential_device(GravitySolver3D* s, const double* d_rho) {
         st = sor_fp64(s);
     } else {
         if (s->poisson_mode == POISSON_NN_FP32_DC && choose_guess(s, d_rho)) return -2;
-        st = sor_fp32_dc(s);
+        st = s->dc_inner == DC_INNER_MULTIGRID ? dc_multigrid(s) : sor_fp32_dc(s);
     }
     HYD_CHECK(hipEventRecord(d->ev1, d->stream));
     HYD_CHECK(hipEventSynchronize(d->ev1));
@@ -510,6 +668,7 @@ GravitySolver3D* gravity_solver_create(int nx, int ny, int nz, double xmin, doub
     s->multipole_lmax = 2;
     s->poisson_mode = POISSON_FP64_SOR;
     s->criterion = SOR_CRIT_MAX_UPDATE;
+    s->dc_inner = DC_INNER_SOR;
     const long n = (long)nx * ny * nz;
     s->phi = (double*)calloc(n, sizeof(double));
     s->rho = NULL;
@@ -608,6 +767,17 @@ int gravity_set_poisson_mode(GravitySolver3D* s, int mode) {
     if (mode != POISSON_FP64_SOR && mode != POISSON_FP32_DC && mode != POISSON_NN_FP32_DC)
         return -1;
     s->poisson_mode = mode;
+    return 0;
+}
+
+int gravity_set_dc_inner(GravitySolver3D* s, int inner) {
+    if (inner != DC_INNER_SOR && inner != DC_INNER_MULTIGRID) return -1;
+    if (inner == DC_INNER_MULTIGRID && !mg_supported(s)) {
+        fprintf(stderr, "gravity_set_dc_inner: multigrid needs even nx, ny, nz >= 8 (got %d x %d x %d)\n",
+                s->nx, s->ny, s->nz);
+        return -1;
+    }
+    s->dc_inner = inner;
     return 0;
 }
 

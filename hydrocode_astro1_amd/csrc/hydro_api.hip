@@ -343,6 +343,9 @@ GodunovHLLC3D* solver3d_create_from_deck(const InputDeck* deck) {
                                   deck->sor_tol);
         gravity_set_multipole_lmax(s->gravity_solver, deck->multipole_lmax);
         gravity_set_poisson_mode(s->gravity_solver, deck->poisson_mode);
+        if (deck->dc_inner != DC_INNER_SOR &&
+            gravity_set_dc_inner(s->gravity_solver, deck->dc_inner) != 0)
+            fprintf(stderr, "Warning: dc_inner ignored; the FP32 modes use SOR sweeps.\n");
         gravity_set_criterion(s->gravity_solver, deck->sor_criterion);
         if (deck->nn_guess[0]) {
             void* m = nn_model_create(0);

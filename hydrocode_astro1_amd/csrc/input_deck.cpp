@@ -148,6 +148,7 @@ InputDeck* input_deck_create(const char* filename) {
     d->sor_criterion = SOR_CRIT_MAX_UPDATE;
     d->multipole_lmax = 2;
     d->poisson_mode = POISSON_FP64_SOR;
+    d->dc_inner = DC_INNER_SOR;
     d->output_enable = true;
 
     if (filename == NULL) {
@@ -186,6 +187,12 @@ InputDeck* input_deck_create(const char* filename) {
                 d->poisson_mode = POISSON_NN_FP32_DC;
             else
                 d->poisson_mode = POISSON_FP64_SOR;
+            continue;
+        }
+        if (strcmp(k, "dc_inner") == 0) {
+            d->dc_inner = (strcmp(v, "multigrid") == 0 || strcmp(v, "1") == 0)
+                              ? DC_INNER_MULTIGRID
+                              : DC_INNER_SOR;
             continue;
         }
         if (strcmp(k, "sor_criterion") == 0) {
@@ -235,6 +242,8 @@ void input_deck_print(const InputDeck* d) {
            d->poisson_mode == POISSON_FP32_DC      ? "fp32dc"
            : d->poisson_mode == POISSON_NN_FP32_DC ? "nn_fp32dc"
                                                    : "fp64");
+    if (d->poisson_mode != POISSON_FP64_SOR)
+        printf("  dc_inner = %s\n", d->dc_inner == DC_INNER_MULTIGRID ? "multigrid" : "sor");
     printf("\nOutput:\n  Directory: %s\n", d->output_dir);
     printf("==============================================\n\n");
 }

@@ -67,6 +67,9 @@ typedef struct GravitySolver3D {
                             start kept, -1 no guess possible              */
     double last_guess_rel_residual; /* residual of the network guess         */
     double last_warm_rel_residual;  /* residual of the warm start            */
+    int dc_inner;        /* DcInner from input_deck.h: inner solver of the FP32
+                            modes; with DC_INNER_MULTIGRID, last_iterations
+                            counts V-cycles                               */
 } GravitySolver3D;
 
 /* gravity_solver_3d.h:64-68 */

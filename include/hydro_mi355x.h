@@ -72,6 +72,9 @@ long hydro_check_division(const double *a, const double *b, long n);
 int gravity_set_multipole_lmax(GravitySolver3D *solver, int lmax);
 /* POISSON_FP64_SOR, POISSON_FP32_DC or POISSON_NN_FP32_DC (input_deck.h). */
 int gravity_set_poisson_mode(GravitySolver3D *solver, int mode);
+/* DC_INNER_SOR or DC_INNER_MULTIGRID (input_deck.h) for the FP32 modes;
+ * multigrid needs even nx, ny, nz >= 8 (returns -1 otherwise). */
+int gravity_set_dc_inner(GravitySolver3D *solver, int inner);
 /* SOR_CRIT_MAX_UPDATE (the reference) or SOR_CRIT_REL_RESID. */
 int gravity_set_criterion(GravitySolver3D *solver, int criterion);
 /* Make the device potential equal to solver->phi (a caller-provided

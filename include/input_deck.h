@@ -51,6 +51,13 @@ typedef enum {
                                residual beats the warm start), then FP32_DC */
 } PoissonMode;
 
+/* Inner solver of the FP32 defect correction, added by this build. */
+typedef enum {
+    DC_INNER_SOR = 0,       /* red-black SOR sweeps on the solver's grid */
+    DC_INNER_MULTIGRID = 1  /* V-cycles of red-black sweeps over a grid
+                               hierarchy (even nx, ny, nz) */
+} DcInner;
+
 /* SOR stopping rules added by this build. */
 typedef enum {
     SOR_CRIT_MAX_UPDATE = 0, /* max |phi_GS - phi_old| < tol (the reference) */
@@ -141,6 +148,7 @@ typedef struct {
     char nn_guess[256];       /* weights file for the learned guess, ""   */
     bool output_enable;       /* false skips CSV/VTK writing              */
     char ic_file[256];        /* binary initial state written by this build */
+    int dc_inner;             /* DcInner of the FP32 modes, default SOR   */
 } InputDeck;
 
 /* input_deck.h:127-131 */
