@@ -267,6 +267,9 @@ def main():
     ap.add_argument("--poisson-probe", type=int, default=0, metavar="ITERS",
                     help="only time ITERS iterations of plain FP64 SOR and of FP32 defect "
                          "correction on the 256^3 explosion density (twice each), then exit")
+    ap.add_argument("--multigrid-probe", type=int, default=0, metavar="REPS",
+                    help="only run REPS cold FP32 defect-correction solves with multigrid inner "
+                         "solves (relative residual 1e-8) on the 256^3 explosion density, then exit")
     ap.add_argument("--selfgravity-probe", type=int, default=0, metavar="STEPS",
                     help="only run the self-gravity step block with STEPS warm steps, then exit")
     ap.add_argument("--calibrate", action="store_true",
@@ -305,6 +308,22 @@ def run(args):
     n = args.n
     if args.selfgravity_probe > 0:
         return {"selfgravity_step": selfgravity_block(H, ics, n, args.selfgravity_probe)}
+    if args.multigrid_probe > 0:
+        import numpy as np
+        rho = np.ascontiguousarray(ics.explosion_n3(n)[..., 0])
+        g = H.Gravity(n, n, n, G=1.0)
+        g.set_lmax(6)
+        g.set_criterion(H.SOR_CRIT_REL_RESID)
+        g.set_mode(H.POISSON_FP32_DC)
+        g.set_dc_inner(H.DC_INNER_MULTIGRID)
+        g.set_params(1.5, 1000, 1e-8)
+        out = []
+        for _ in range(args.multigrid_probe):
+            g.set_phi(np.zeros((n, n, n)))
+            st = g.solve_potential(rho)
+            out.append({"status": st, "v_cycles": g.diagnostics()[0], "ms": g.s.last_solve_ms,
+                        "rel_residual": g.s.last_rel_residual})
+        return {"multigrid_probe": out, "grid": n}
     if args.poisson_probe > 0:
         import numpy as np
         rho = np.ascontiguousarray(ics.explosion_n3(n)[..., 0])
