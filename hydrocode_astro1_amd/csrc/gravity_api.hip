@@ -54,6 +54,7 @@ void dev_free(GravDevice* d) {
             if (p) (void)hipFree(p);
     if (d->mg_qr) (void)hipFree(d->mg_qr);
     if (d->mg_qb) (void)hipFree(d->mg_qb);
+    if (d->mg_tail_dev) (void)hipFree(d->mg_tail_dev);
     if (d->ctl_host) (void)hipHostFree(d->ctl_host);
     if (d->acc_host) (void)hipHostFree(d->acc_host);
     if (d->ev0) (void)hipEventDestroy(d->ev0);
@@ -74,6 +75,7 @@ int dev_create(GravitySolver3D* s) {
     d->stream = hyd_stream();
     if (!d->stream) return -2;
     if (const char* e = getenv("HYDRO_MI355X_SOR_FUSED")) d->fused = atoi(e) != 0;
+    if (const char* e = getenv("HYDRO_MI355X_MG_TAIL")) d->mg_use_tail = atoi(e) != 0;
     if (const char* e = getenv("HYDRO_MI355X_MG_SMOOTH")) {
         int a = 0, b = 0;
         float w = 0.0f;
@@ -450,6 +452,43 @@ int ensure_mg(GravitySolver3D* s) {
         d->mg_q.push_back(q);
     }
     if (dev_alloc(&d->mg_qr, d->nsplit) || dev_alloc(&d->mg_qb, d->nsplit)) return -2;
+    // the tail: the longest run of coarsest levels whose first level has at
+    // most kTailPoints points and whose u, f, q fit the workgroup's LDS
+    constexpr long kTailPoints = 6000;
+    const int nl = (int)d->mg.size();
+    d->mg_tail_first = -1;
+    long floats = 0;
+    for (int l = nl - 1; l >= 0 && d->mg_use_tail && nl - l <= hyd::MG_TAIL_MAX; --l) {
+        const long pts = (long)d->mg[l].nx * d->mg[l].ny * d->mg[l].nz;
+        if (pts > kTailPoints || sizeof(float) * (size_t)(floats + 3 * pts) > hyd::MG_TAIL_LDS_MAX)
+            break;
+        floats += 3 * pts;
+        d->mg_tail_first = l;
+    }
+    if (d->mg_tail_first >= 0 && d->mg_tail_first < nl - 1) {
+        hyd::MgTail tab{};
+        tab.count = nl - d->mg_tail_first;
+        tab.nu1 = d->mg_nu1;
+        tab.nu2 = d->mg_nu2;
+        tab.omega = d->mg_omega;
+        int off = 0;
+        for (int q = 0; q < tab.count; ++q) {
+            const hyd::MgLevel& L = d->mg[d->mg_tail_first + q];
+            const int pts = L.nx * L.ny * L.nz;
+            tab.lv[q] = L;
+            tab.off_u[q] = off;
+            tab.off_f[q] = off + pts;
+            tab.off_q[q] = off + 2 * pts;
+            off += 3 * pts;
+        }
+        d->mg_tail_floats = off;
+        if (dev_alloc(&d->mg_tail_dev, 1)) return -2;
+        HYD_CHECK(hipMemcpyAsync(d->mg_tail_dev, &tab, sizeof tab, hipMemcpyHostToDevice,
+                                 d->stream));
+        HYD_CHECK(hipStreamSynchronize(d->stream));
+    } else {
+        d->mg_tail_first = -1;  // a single level gains nothing from the tail
+    }
     return 0;
 }
 
@@ -463,6 +502,11 @@ int mg_smooth(GravDevice* d, size_t l, int sweeps, float omega) {
 
 // V-cycle from coarse level l (index into d->mg) down, on u_l with f_l.
 int mg_coarse_cycle(GravDevice* d, size_t l) {
+    if ((int)l == d->mg_tail_first) {
+        HYD_CHECK(hyd::launch_mg_tail(d->mg_tail_dev, d->mg_tail_floats, d->mg_f[l], d->mg_u[l],
+                                      d->stream));
+        return 0;
+    }
     const hyd::MgLevel& L = d->mg[l];
     if (l + 1 == d->mg.size())  // coarsest: Gauss-Seidel sweeps
         return mg_smooth(d, l, std::max(L.nx, std::max(L.ny, L.nz)), 1.0f);

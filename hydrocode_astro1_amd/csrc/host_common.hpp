@@ -67,6 +67,11 @@ struct GravDevice {
     std::vector<hyd::MgLevel> mg;          // levels 1, 2, ... (coarser along the list)
     std::vector<float*> mg_u, mg_f, mg_q;  // correction, right-hand side, residual
     float *mg_qr = nullptr, *mg_qb = nullptr;  // level-0 residual (split layout)
+    // levels from mg_tail_first down run in one workgroup in LDS (-1: none);
+    // HYDRO_MI355X_MG_TAIL=0 runs them one phase per launch
+    bool mg_use_tail = true;
+    int mg_tail_first = -1, mg_tail_floats = 0;
+    hyd::MgTail* mg_tail_dev = nullptr;
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool g_valid = false;  // device g has been computed at least once

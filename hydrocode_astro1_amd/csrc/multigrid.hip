@@ -150,16 +150,14 @@ __global__ void __launch_bounds__(256) mg_prolong_fine_kernel(GridDims g, MgLeve
 }
 
 // ------------------------------------------------------- coarse levels --
-__global__ void __launch_bounds__(256) mg_smooth_kernel(MgLevel L, int colour, float omega,
-                                                        float* __restrict__ u,
-                                                        const float* __restrict__ f) {
-    const int m = blockIdx.x * 64 + (threadIdx.x & 63);
-    const int j = 1 + blockIdx.y * MG_ROWS + (threadIdx.x >> 6);
-    const int i = 1 + blockIdx.z;
-    const int k = 2 * m + ((i + j + colour) & 1);
-    if (j > L.ny - 2 || k < 1 || k > L.nz - 2) return;
-    const long sx = (long)L.ny * L.nz;
-    const long t = ((long)i * L.ny + j) * L.nz + k;
+// Point operations shared by the one-phase-per-launch kernels and the
+// one-workgroup tail kernel, so both give the same bits.  T is the index
+// type (long for global arrays, int inside the tail's LDS).
+template <class T>
+__device__ __forceinline__ void smooth_at(const MgLevel& L, int i, int j, int k, float omega,
+                                          float* u, const float* f) {
+    const T sx = (T)L.ny * L.nz;
+    const T t = ((T)i * L.ny + j) * L.nz + k;
     const float off = axis_off(L.ax[0], i, u[t - sx], u[t + sx]) +
                       axis_off(L.ax[1], j, u[t - L.nz], u[t + L.nz]) +
                       axis_off(L.ax[2], k, u[t - 1], u[t + 1]);
@@ -169,6 +167,47 @@ __global__ void __launch_bounds__(256) mg_smooth_kernel(MgLevel L, int colour, f
     u[t] = old + omega * (gs - old);
 }
 
+// f - L u at an interior point
+template <class T>
+__device__ __forceinline__ float residual_at(const MgLevel& L, int i, int j, int k,
+                                             const float* u, const float* f) {
+    const T sx = (T)L.ny * L.nz;
+    const T t = ((T)i * L.ny + j) * L.nz + k;
+    const float off = axis_off(L.ax[0], i, u[t - sx], u[t + sx]) +
+                      axis_off(L.ax[1], j, u[t - L.nz], u[t + L.nz]) +
+                      axis_off(L.ax[2], k, u[t - 1], u[t + 1]);
+    const float diag = axis_diag(L.ax[0], i) + axis_diag(L.ax[1], j) + axis_diag(L.ax[2], k);
+    return f[t] - (off - diag * u[t]);
+}
+
+// restriction of q (level F) into interior point (I, J, K) of level C
+template <class T>
+__device__ __forceinline__ float restrict_at(const MgLevel& F, const float* q, const MgLevel& C,
+                                             int I, int J, int K) {
+    float f = 0.0f;
+    for (int dx = -1; dx <= 1; ++dx) {
+        const float wx = rweight(C.ax[0], I, dx);
+        for (int dy = -1; dy <= 1; ++dy) {
+            const float wxy = wx * rweight(C.ax[1], J, dy);
+            const T row = ((T)(2 * I + dx) * F.ny + (2 * J + dy)) * F.nz + 2 * K;
+#pragma unroll
+            for (int dz = -1; dz <= 1; ++dz) f += (wxy * rweight(C.ax[2], K, dz)) * q[row + dz];
+        }
+    }
+    return f;
+}
+
+__global__ void __launch_bounds__(256) mg_smooth_kernel(MgLevel L, int colour, float omega,
+                                                        float* __restrict__ u,
+                                                        const float* __restrict__ f) {
+    const int m = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = 1 + blockIdx.y * MG_ROWS + (threadIdx.x >> 6);
+    const int i = 1 + blockIdx.z;
+    const int k = 2 * m + ((i + j + colour) & 1);
+    if (j > L.ny - 2 || k < 1 || k > L.nz - 2) return;
+    smooth_at<long>(L, i, j, k, omega, u, f);
+}
+
 __global__ void __launch_bounds__(256) mg_residual_kernel(MgLevel L, const float* __restrict__ u,
                                                           const float* __restrict__ f,
                                                           float* __restrict__ q) {
@@ -176,17 +215,10 @@ __global__ void __launch_bounds__(256) mg_residual_kernel(MgLevel L, const float
     const int j = blockIdx.y * MG_ROWS + (threadIdx.x >> 6);
     const int i = blockIdx.z;
     if (k >= L.nz || j >= L.ny) return;
-    const long sx = (long)L.ny * L.nz;
     const long t = ((long)i * L.ny + j) * L.nz + k;
-    if (i < 1 || i > L.nx - 2 || j < 1 || j > L.ny - 2 || k < 1 || k > L.nz - 2) {
-        q[t] = 0.0f;
-        return;
-    }
-    const float off = axis_off(L.ax[0], i, u[t - sx], u[t + sx]) +
-                      axis_off(L.ax[1], j, u[t - L.nz], u[t + L.nz]) +
-                      axis_off(L.ax[2], k, u[t - 1], u[t + 1]);
-    const float diag = axis_diag(L.ax[0], i) + axis_diag(L.ax[1], j) + axis_diag(L.ax[2], k);
-    q[t] = f[t] - (off - diag * u[t]);
+    const bool inner = i >= 1 && i <= L.nx - 2 && j >= 1 && j <= L.ny - 2 && k >= 1 &&
+                       k <= L.nz - 2;
+    q[t] = inner ? residual_at<long>(L, i, j, k, u, f) : 0.0f;
 }
 
 // Restriction between coarse levels: coarse c gathers fine 2c-1 .. 2c+1.
@@ -198,19 +230,9 @@ __global__ void __launch_bounds__(256) mg_restrict_kernel(MgLevel F, const float
     const int I = blockIdx.z;
     if (K >= C.nz || J >= C.ny) return;
     const long t = ((long)I * C.ny + J) * C.nz + K;
-    float f = 0.0f;
-    if (I >= 1 && I <= C.nx - 2 && J >= 1 && J <= C.ny - 2 && K >= 1 && K <= C.nz - 2) {
-        for (int dx = -1; dx <= 1; ++dx) {
-            const float wx = rweight(C.ax[0], I, dx);
-            for (int dy = -1; dy <= 1; ++dy) {
-                const float wxy = wx * rweight(C.ax[1], J, dy);
-                const long row = ((long)(2 * I + dx) * F.ny + (2 * J + dy)) * F.nz + 2 * K;
-#pragma unroll
-                for (int dz = -1; dz <= 1; ++dz) f += (wxy * rweight(C.ax[2], K, dz)) * q[row + dz];
-            }
-        }
-    }
-    fc[t] = f;
+    const bool inner = I >= 1 && I <= C.nx - 2 && J >= 1 && J <= C.ny - 2 && K >= 1 &&
+                       K <= C.nz - 2;
+    fc[t] = inner ? restrict_at<long>(F, q, C, I, J, K) : 0.0f;
     uc[t] = 0.0f;
 }
 
@@ -222,6 +244,107 @@ __global__ void __launch_bounds__(256) mg_prolong_kernel(MgLevel C, const float*
     if (k < 1 || k > F.nz - 2 || j < 1 || j > F.ny - 2 || i < 1 || i > F.nx - 2) return;
     const long t = ((long)i * F.ny + j) * F.nz + k;
     uf[t] += interpolate(uc, C.ny, C.nz, i, j, k, F.ax[0].pw1, F.ax[1].pw1, F.ax[2].pw1);
+}
+
+// ---------------------------------------------------------------- tail --
+// The V-cycle from level `first` of the tail table down to its coarsest
+// level in one workgroup, with every level's u, f and q in LDS: the same
+// phases and point operations as the launch-per-phase path, separated by
+// barriers instead of launches.  Reads f of the first level from global
+// memory (u starts at 0) and writes its u back.
+constexpr int MG_TAIL_THREADS = 1024;
+
+__device__ void tail_smooth(const MgLevel& L, int colour, float omega, float* u, const float* f) {
+    const int mh = (L.nz + 1) / 2, rows = L.ny - 2;
+    const int total = (L.nx - 2) * rows * mh;
+    for (int t = threadIdx.x; t < total; t += MG_TAIL_THREADS) {
+        const int m = t % mh, r = t / mh;
+        const int j = 1 + r % rows, i = 1 + r / rows;
+        const int k = 2 * m + ((i + j + colour) & 1);
+        if (k >= 1 && k <= L.nz - 2) smooth_at<int>(L, i, j, k, omega, u, f);
+    }
+    __syncthreads();
+}
+
+__device__ void tail_sweeps(const MgLevel& L, int n, float omega, float* u, const float* f) {
+    for (int s = 0; s < n; ++s) {
+        tail_smooth(L, 0, omega, u, f);
+        tail_smooth(L, 1, omega, u, f);
+    }
+}
+
+__global__ void __launch_bounds__(MG_TAIL_THREADS) mg_tail_kernel(const MgTail* __restrict__ tab,
+                                                                  const float* __restrict__ f_in,
+                                                                  float* __restrict__ u_out) {
+    __shared__ float sm[MG_TAIL_LDS_MAX / sizeof(float)];
+    const int nl = tab->count;
+    {
+        const MgLevel& L = tab->lv[0];
+        const int n = L.nx * L.ny * L.nz;
+        float* u = sm + tab->off_u[0];
+        float* f = sm + tab->off_f[0];
+        for (int t = threadIdx.x; t < n; t += MG_TAIL_THREADS) {
+            f[t] = f_in[t];
+            u[t] = 0.0f;
+        }
+        __syncthreads();
+    }
+    for (int l = 0; l + 1 < nl; ++l) {  // down
+        const MgLevel& L = tab->lv[l];
+        const MgLevel& C = tab->lv[l + 1];
+        float* u = sm + tab->off_u[l];
+        float* f = sm + tab->off_f[l];
+        float* q = sm + tab->off_q[l];
+        tail_sweeps(L, tab->nu1, tab->omega, u, f);
+        const int n = L.nx * L.ny * L.nz;
+        for (int t = threadIdx.x; t < n; t += MG_TAIL_THREADS) {
+            const int k = t % L.nz, r = t / L.nz;
+            const int j = r % L.ny, i = r / L.ny;
+            const bool inner = i >= 1 && i <= L.nx - 2 && j >= 1 && j <= L.ny - 2 && k >= 1 &&
+                               k <= L.nz - 2;
+            q[t] = inner ? residual_at<int>(L, i, j, k, u, f) : 0.0f;
+        }
+        __syncthreads();
+        float* uc = sm + tab->off_u[l + 1];
+        float* fc = sm + tab->off_f[l + 1];
+        const int nc = C.nx * C.ny * C.nz;
+        for (int t = threadIdx.x; t < nc; t += MG_TAIL_THREADS) {
+            const int K = t % C.nz, r = t / C.nz;
+            const int J = r % C.ny, I = r / C.ny;
+            const bool inner = I >= 1 && I <= C.nx - 2 && J >= 1 && J <= C.ny - 2 && K >= 1 &&
+                               K <= C.nz - 2;
+            fc[t] = inner ? restrict_at<int>(L, q, C, I, J, K) : 0.0f;
+            uc[t] = 0.0f;
+        }
+        __syncthreads();
+    }
+    {
+        const MgLevel& L = tab->lv[nl - 1];  // coarsest: Gauss-Seidel sweeps
+        tail_sweeps(L, max(L.nx, max(L.ny, L.nz)), 1.0f, sm + tab->off_u[nl - 1],
+                    sm + tab->off_f[nl - 1]);
+    }
+    for (int l = nl - 2; l >= 0; --l) {  // up
+        const MgLevel& L = tab->lv[l];
+        const MgLevel& C = tab->lv[l + 1];
+        float* u = sm + tab->off_u[l];
+        const float* uc = sm + tab->off_u[l + 1];
+        const int mz = L.nz - 2, my = L.ny - 2;
+        const int n = (L.nx - 2) * my * mz;
+        for (int t = threadIdx.x; t < n; t += MG_TAIL_THREADS) {
+            const int k = 1 + t % mz, r = t / mz;
+            const int j = 1 + r % my, i = 1 + r / my;
+            u[(i * L.ny + j) * L.nz + k] +=
+                interpolate(uc, C.ny, C.nz, i, j, k, L.ax[0].pw1, L.ax[1].pw1, L.ax[2].pw1);
+        }
+        __syncthreads();
+        tail_sweeps(L, tab->nu2, tab->omega, u, sm + tab->off_f[l]);
+    }
+    {
+        const MgLevel& L = tab->lv[0];
+        const int n = L.nx * L.ny * L.nz;
+        const float* u = sm + tab->off_u[0];
+        for (int t = threadIdx.x; t < n; t += MG_TAIL_THREADS) u_out[t] = u[t];
+    }
 }
 
 dim3 grid3(int fast, int rows, int planes) {
@@ -279,6 +402,15 @@ hipError_t launch_mg_restrict(const MgLevel& fine, const float* q, const MgLevel
         return hipErrorInvalidValue;
     hipLaunchKernelGGL(mg_restrict_kernel, grid3(coarse.nz, coarse.ny, coarse.nx), dim3(256), 0,
                        st, fine, q, coarse, fc, uc);
+    return hipGetLastError();
+}
+
+hipError_t launch_mg_tail(const MgTail* d_tab, int lds_floats, const float* f_first,
+                          float* u_first, hipStream_t st) {
+    if (lds_floats < 1 || sizeof(float) * (size_t)lds_floats > MG_TAIL_LDS_MAX)
+        return hipErrorInvalidValue;
+    hipLaunchKernelGGL(mg_tail_kernel, dim3(1), dim3(MG_TAIL_THREADS), 0, st, d_tab, f_first,
+                       u_first);
     return hipGetLastError();
 }
 

@@ -32,6 +32,18 @@ struct MgLevel {
     MgAxis ax[3];    // x, y, z
 };
 
+// The coarse levels from one level down, run by a single workgroup in LDS
+// (the levels' u, f, q at the given float offsets of its LDS buffer).
+constexpr int MG_TAIL_MAX = 8;
+constexpr size_t MG_TAIL_LDS_MAX = 150 * 1024;
+struct MgTail {
+    int count;  // levels in the table; the last is the coarsest
+    int nu1, nu2;
+    float omega;
+    MgLevel lv[MG_TAIL_MAX];
+    int off_u[MG_TAIL_MAX], off_f[MG_TAIL_MAX], off_q[MG_TAIL_MAX];
+};
+
 // level 0 (split) <-> level 1
 hipError_t launch_mg_fine_residual(const GridDims& g, const SorCoef& c, const float* e_r,
                                    const float* e_b, const float* r_r, const float* r_b,
@@ -50,5 +62,9 @@ hipError_t launch_mg_restrict(const MgLevel& fine, const float* q, const MgLevel
                               float* fc, float* uc, hipStream_t st);
 hipError_t launch_mg_prolong(const MgLevel& coarse, const float* uc, const MgLevel& fine,
                              float* uf, hipStream_t st);
+// d_tab: an MgTail in device memory; f_first / u_first: the first level's
+// f (read) and u (written) in global memory.
+hipError_t launch_mg_tail(const MgTail* d_tab, int lds_floats, const float* f_first,
+                          float* u_first, hipStream_t st);
 
 }  // namespace hyd

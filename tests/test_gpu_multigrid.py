@@ -78,3 +78,28 @@ def test_isolated_boundary_reaches_tolerance_and_agrees(tight, nn_mode):
         assert g.s.last_guess in (0, 1)
     assert err < 1e-7
     assert g.diagnostics()[0] <= 12
+
+
+@pytest.mark.parametrize("shape", [(32, 32, 32), (64, 48, 40)])
+def test_lds_tail_gives_the_same_bits(shape, monkeypatch):
+    """The coarse levels run in one workgroup in LDS (the default) or one
+    phase per launch (HYDRO_MI355X_MG_TAIL=0): the same point operations in
+    the same phase order, so the potential and the V-cycle count agree bit
+    for bit.  32^3: the tail is every coarse level; 64 x 48 x 40: the last
+    two of three."""
+    rho = blob(shape)
+    out = []
+    for tail in ("1", "0"):
+        print(f"multigrid tail={tail} {shape}", flush=True)
+        monkeypatch.setenv("HYDRO_MI355X_MG_TAIL", tail)
+        g = H.Gravity(*shape, G=1.0)
+        g.set_lmax(4)
+        g.set_criterion(H.SOR_CRIT_REL_RESID)
+        g.set_mode(H.POISSON_FP32_DC)
+        g.set_dc_inner(H.DC_INNER_MULTIGRID)
+        g.set_params(1.5, 1000, 1e-9)
+        assert g.solve_potential(rho) == 0
+        out.append((g.diagnostics()[0], g.phi()))
+        g.close()
+    assert out[0][0] == out[1][0]
+    assert np.array_equal(out[0][1], out[1][1])
