@@ -7,7 +7,10 @@ Stored: per-step dt and SOR iteration counts, a SHA-256 of the final state
 bytes, per-field L1 norms and sums, and the same run with the reference's
 sequential moment sums (final-state L1 norms) for the relative-L1 bar.
 
-    python tests/golden/make_c2_fixture.py [steps]      (about an hour on one core)
+    python tests/golden/make_c2_fixture.py [steps] [lmax]   (about an hour on one core)
+
+lmax 2 (the reference's monopole + quadrupole) writes c2_polytrope_128.npz,
+any other order c2_polytrope_128_l<lmax>.npz.
 """
 import hashlib
 import os
@@ -28,10 +31,10 @@ TOL = 1e-8
 OMEGA = 1.5
 
 
-def run(mode, steps, log):
+def run(mode, steps, log, lmax):
     U0 = ics.polytrope_n1(N)
     g = Grid(N, N, N, G=1.0, gamma=2.0, cfl=0.4, bc=(2,) * 6)
-    o = OracleSolver(g, omega=OMEGA, max_iter=200000, tol=TOL, lmax=2, moment_mode=mode)
+    o = OracleSolver(g, omega=OMEGA, max_iter=200000, tol=TOL, lmax=lmax, moment_mode=mode)
     o.set_state(U0)
     dts, its = [], []
     t0 = time.time()
@@ -47,11 +50,12 @@ def run(mode, steps, log):
 
 def main():
     steps = int(sys.argv[1]) if len(sys.argv) > 1 else 100
+    lmax = int(sys.argv[2]) if len(sys.argv) > 2 else 2
     log = lambda m: print(m, flush=True)
-    U, dts, its = run(MOM_DDTREE, steps, log)
-    Us, dts_s, its_s = run(MOM_SEQUENTIAL, steps, log)
+    U, dts, its = run(MOM_DDTREE, steps, log, lmax)
+    Us, dts_s, its_s = run(MOM_SEQUENTIAL, steps, log, lmax)
     out = dict(
-        steps=steps, n=N, tol=TOL, omega=OMEGA, dt=dts, iters=its,
+        steps=steps, n=N, tol=TOL, omega=OMEGA, lmax=lmax, dt=dts, iters=its,
         sha256=np.frombuffer(hashlib.sha256(np.ascontiguousarray(U).tobytes()).digest(),
                              dtype=np.uint8),
         l1=np.abs(U).reshape(-1, 5).sum(axis=0), sums=U.reshape(-1, 5).sum(axis=0),
@@ -59,8 +63,9 @@ def main():
         seq_diff_l1=np.abs(U - Us).reshape(-1, 5).sum(axis=0),
         rho_slice=np.ascontiguousarray(U[N // 2, :, :, 0]),
     )
-    np.savez_compressed(os.path.join(HERE, f"c2_polytrope_{N}.npz"), **out)
-    log(f"wrote c2_polytrope_{N}.npz; ddtree vs sequential relative L1 per field: "
+    name = f"c2_polytrope_{N}.npz" if lmax == 2 else f"c2_polytrope_{N}_l{lmax}.npz"
+    np.savez_compressed(os.path.join(HERE, name), **out)
+    log(f"wrote {name}; ddtree vs sequential relative L1 per field: "
         f"{out['seq_diff_l1'] / out['seq_l1']}")
 
 

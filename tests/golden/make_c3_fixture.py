@@ -1,0 +1,95 @@
+"""Fixture for the 256^3 stellar-explosion workload with self-gravity (n=3
+polytrope, lognormal k^-2 perturbations, 10|W| core deposit; gamma 5/3,
+G = 1, CFL 0.4, outflow, SOR omega 1.5 / max |update| < 1e-6 / at most
+10000 iterations, multipole boundary to l = 6): the oracle run for STEPS
+steps with moments summed by the device's pairwise double-double tree, and
+the same run with sequential moment sums for the relative-L1 bar.
+
+Stored: SHA-256 of the initial state, per-step dt and SOR iteration
+counts, SHA-256 of the final state, per-field L1 norms and sums, a 32 x 32
+subsample of the final mid-plane density, and the sequential run's dt,
+iteration counts and final L1 distance.
+
+    python tests/golden/make_c3_fixture.py ddtree [steps]      (one run, hours on one core)
+    python tests/golden/make_c3_fixture.py sequential [steps]
+    python tests/golden/make_c3_fixture.py merge
+
+The two runs write their results to $C3_WORK (default /tmp); merge writes
+tests/golden/c3_explosion_256.npz.
+"""
+import hashlib
+import os
+import sys
+import time
+
+import numpy as np
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, os.path.dirname(HERE))
+sys.path.insert(0, os.path.dirname(os.path.dirname(HERE)))
+
+from oracle_lib import MOM_DDTREE, MOM_SEQUENTIAL, Grid, OracleSolver  # noqa: E402
+from hydrocode_astro1_amd import ics  # noqa: E402
+
+N = 256
+LMAX = 6
+WORK = os.environ.get("C3_WORK", "/tmp")
+
+
+def sha(a):
+    return np.frombuffer(hashlib.sha256(np.ascontiguousarray(a).tobytes()).digest(),
+                         dtype=np.uint8)
+
+
+def run(mode, steps):
+    U0 = ics.explosion_n3(N)
+    g = Grid(N, N, N, G=1.0, gamma=5.0 / 3.0, cfl=0.4, bc=(2,) * 6)
+    o = OracleSolver(g, omega=1.5, max_iter=10000, tol=1e-6, lmax=LMAX, moment_mode=mode)
+    o.set_state(U0)
+    ic = sha(U0)
+    del U0
+    dts, its = [], []
+    t0 = time.time()
+    for s in range(steps):
+        o.step()
+        it, res, st = o.diag()
+        dts.append(o.dt)
+        its.append(it)
+        print(f"mode {mode} step {s + 1}/{steps}: dt {o.dt:.6e}, SOR {it} iterations, "
+              f"status {st}, {time.time() - t0:.0f}s", flush=True)
+    U = o.state()
+    return ic, U, np.array(dts), np.array(its)
+
+
+def main():
+    what = sys.argv[1]
+    steps = int(sys.argv[2]) if len(sys.argv) > 2 else 200
+    if what in ("ddtree", "sequential"):
+        mode = MOM_DDTREE if what == "ddtree" else MOM_SEQUENTIAL
+        ic, U, dts, its = run(mode, steps)
+        np.save(os.path.join(WORK, f"c3_{what}_state.npy"), U)
+        np.savez(os.path.join(WORK, f"c3_{what}_log.npz"), ic=ic, dt=dts, iters=its,
+                 steps=steps)
+        return
+    a = np.load(os.path.join(WORK, "c3_ddtree_log.npz"))
+    b = np.load(os.path.join(WORK, "c3_sequential_log.npz"))
+    U = np.load(os.path.join(WORK, "c3_ddtree_state.npy"), mmap_mode="r")
+    Us = np.load(os.path.join(WORK, "c3_sequential_state.npy"), mmap_mode="r")
+    assert np.array_equal(a["ic"], b["ic"]) and int(a["steps"]) == int(b["steps"])
+    l1 = np.abs(U).reshape(-1, 5).sum(axis=0)
+    out = dict(
+        steps=int(a["steps"]), n=N, lmax=LMAX, tol=1e-6, omega=1.5, max_iter=10000,
+        ic_sha256=a["ic"], dt=a["dt"], iters=a["iters"], sha256=sha(U), l1=l1,
+        sums=np.asarray(U).reshape(-1, 5).sum(axis=0),
+        rho_sub=np.ascontiguousarray(U[N // 2, ::8, ::8, 0]),
+        seq_dt=b["dt"], seq_iters=b["iters"], seq_l1=np.abs(Us).reshape(-1, 5).sum(axis=0),
+        seq_diff_l1=np.abs(np.asarray(U) - np.asarray(Us)).reshape(-1, 5).sum(axis=0),
+    )
+    np.savez_compressed(os.path.join(HERE, f"c3_explosion_{N}.npz"), **out)
+    print(f"wrote c3_explosion_{N}.npz; iterations equal: "
+          f"{np.array_equal(out['iters'], out['seq_iters'])}; ddtree vs sequential relative L1 "
+          f"per field: {out['seq_diff_l1'] / out['seq_l1']}")
+
+
+if __name__ == "__main__":
+    main()

@@ -1,6 +1,8 @@
 """The 128^3 n=1 polytrope with self-gravity (G = 1, gamma = 2, SOR tol
 1e-8 under the reference's max-update criterion) on the device, step by
-step against the oracle fixture tests/golden/c2_polytrope_128.npz: dt and
+step against the oracle fixtures tests/golden/c2_polytrope_128.npz
+(multipole boundary to l = 2, the reference's) and c2_polytrope_128_l4.npz
+(to l = 4): dt and
 SOR iteration count of every step exactly, the final state bit for bit
 (SHA-256), and the distance to the run with the reference's sequential
 moment sums: relative L1 <= 1e-12 for density and energy, and for the
@@ -19,15 +21,21 @@ from hydrocode_astro1_amd import ics
 
 pytestmark = pytest.mark.gpu
 
-FIX = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c2_polytrope_128.npz")
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
-@pytest.mark.skipif(not os.path.exists(FIX), reason="fixture not generated")
-def test_polytrope_128_steps_bitwise():
-    f = np.load(FIX)
+@pytest.mark.parametrize("name", ["c2_polytrope_128.npz", "c2_polytrope_128_l4.npz"])
+def test_polytrope_128_steps_bitwise(name):
+    path = os.path.join(GOLDEN, name)
+    if not os.path.exists(path):
+        pytest.skip(f"fixture {name} not generated")
+    f = np.load(path)
     n, steps = int(f["n"]), int(f["steps"])
+    lmax = int(f["lmax"]) if "lmax" in f else 2
+    print(f"polytrope 128^3, l = {lmax}, {steps} steps", flush=True)
     U0 = ics.polytrope_n1(n)
     s = H.Solver(n, n, n, G=1.0, gamma=2.0, cfl=0.4, bc=(H.BC_OUTFLOW,) * 6)
+    s.gravity.set_lmax(lmax)
     s.gravity.set_params(float(f["omega"]), 200000, float(f["tol"]))
     s.set_state(U0)
     for k in range(steps):

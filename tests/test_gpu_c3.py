@@ -1,0 +1,51 @@
+"""The 256^3 stellar-explosion workload with self-gravity (gamma 5/3, G = 1,
+CFL 0.4, outflow, the reference's SOR settings, multipole boundary to
+l = 6) on the device, 200 steps against the oracle fixture
+tests/golden/c3_explosion_256.npz: the initial state's digest, dt and SOR
+iteration count of every step exactly, the final state bit for bit
+(SHA-256), and the fixture's distance to the run with the reference's
+sequential moment sums."""
+import hashlib
+import os
+
+import numpy as np
+import pytest
+
+import hydrocode_astro1_amd as H
+from hydrocode_astro1_amd import ics
+
+pytestmark = pytest.mark.gpu
+
+FIX = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "c3_explosion_256.npz")
+
+
+def sha(a):
+    return np.frombuffer(hashlib.sha256(np.ascontiguousarray(a).tobytes()).digest(),
+                         dtype=np.uint8)
+
+
+@pytest.mark.skipif(not os.path.exists(FIX), reason="fixture not generated")
+def test_explosion_256_steps_bitwise():
+    f = np.load(FIX)
+    n, steps = int(f["n"]), int(f["steps"])
+    U0 = ics.explosion_n3(n)
+    assert np.array_equal(sha(U0), f["ic_sha256"]), "initial state differs from the fixture's"
+    s = H.Solver(n, n, n, G=1.0, gamma=5.0 / 3.0, cfl=0.4, bc=(H.BC_OUTFLOW,) * 6)
+    s.gravity.set_lmax(int(f["lmax"]))
+    s.gravity.set_params(float(f["omega"]), int(f["max_iter"]), float(f["tol"]))
+    s.set_state(U0)
+    del U0
+    for k in range(steps):
+        assert s.step() == 0
+        it, _ = s.gravity.diagnostics()
+        assert it == f["iters"][k], f"step {k + 1}"
+        assert s.dt == f["dt"][k], f"step {k + 1}"
+    U = s.state()
+    assert np.array_equal(U[n // 2, ::8, ::8, 0], f["rho_sub"])
+    assert np.array_equal(sha(U), f["sha256"])
+    s.close()
+    # the device equals the tree-summed run bit for bit, so its distance to
+    # the run with the reference's sequential moment sums is the fixture's
+    d, l1 = f["seq_diff_l1"], f["seq_l1"]
+    assert d[0] <= 1e-12 * l1[0] and d[4] <= 1e-12 * l1[4]
+    assert np.all(d[1:4] <= 1e-12 * l1[0])
