@@ -65,6 +65,7 @@ struct Model {
     long woff[kNL], boff[kNL];  // offsets into params: weights [27][cinp][cout], bias [cout]
     long nparam = 0;
     float *params = nullptr, *grads = nullptr, *m = nullptr, *v = nullptr;
+    float* wg_part = nullptr;        // weight-gradient block partials
     uint16_t* wp[kNL] = {nullptr};   // packed B fragments, forward
     uint16_t* wpt[kNL] = {nullptr};  // packed B fragments, adjoint
     long step = 0;
@@ -191,7 +192,7 @@ int wgrad(Model& M, int l, int n_out, const uint16_t* x, const uint16_t* g) {
     a.cinp = pad4(L.cin);
     a.cout = L.cout;
     a.coutp = pad4(L.cout);
-    a.vchunk = 4096;
+    a.part = M.wg_part;
     HYD_CHECK(conv_wgrad(a, M.st));
     return 0;
 }
@@ -336,6 +337,7 @@ void* nn_model_create(unsigned long long seed) {
     }
     bool ok = hipMalloc((void**)&M->params, sizeof(float) * off) == hipSuccess &&
               hipMalloc((void**)&M->grads, sizeof(float) * off) == hipSuccess &&
+              hipMalloc((void**)&M->wg_part, sizeof(float) * wgrad_scratch_len()) == hipSuccess &&
               hipMalloc((void**)&M->m, sizeof(float) * off) == hipSuccess &&
               hipMalloc((void**)&M->v, sizeof(float) * off) == hipSuccess &&
               hipMalloc((void**)&M->mom_tmp, sizeof(double) * 4) == hipSuccess;
@@ -362,7 +364,7 @@ void nn_model_destroy(void* h) {
     Model* M = (Model*)h;
     if (!M) return;
     free_acts(M->acts);
-    void* ps[] = {M->params, M->grads, M->m, M->v, M->mom_tmp};
+    void* ps[] = {M->params, M->grads, M->m, M->v, M->mom_tmp, M->wg_part};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     for (double* p : M->coarse) (void)hipFree(p);

@@ -141,80 +141,193 @@ hipError_t conv_forward(const ConvArgs& a, hipStream_t st) {
 }
 
 // -------------------------------------------------------- weight gradient --
-// dW[t][ci][co] = sum_v x[v + off_t][ci] g[v][co]: rows (t, ci) x columns co,
-// reduced over voxels 16 at a time.  Block = one wavefront = one 16 x 16 tile
-// over one chunk of voxels; partial tiles are added atomically.
-template <int CINP>
-__global__ void __launch_bounds__(64) conv_wgrad_kernel(WgradArgs a) {
-    const int lane = threadIdx.x & 63;
+// dW[t][ci][co] = sum_v x[v + off_t][ci] g[v][co] as D[co][(t, ci)] =
+// G^T X on MFMA: M = output channels, N = rows (t, ci), K = voxels.  A block
+// walks bricks of 4 x 4 x 16 voxels (the forward kernel's bricks).  Per brick
+// it stages the gradient transposed, gs[co][voxel], and the input with a
+// one-voxel halo as three copies shifted by dz = -1, 0, 1 along z, so that
+// every fragment is one aligned 8-byte LDS read.  Rows past 27 * cinp are
+// zero except the first, which is all ones and yields the bias gradient in
+// the same product.  Each block writes its sums to part[block]; a second
+// kernel adds the blocks in a fixed order (no atomics: the gradient does
+// not depend on scheduling).
+template <int CINP, int MT>
+__global__ void __launch_bounds__(256) conv_wgrad_lds_kernel(WgradArgs a) {
+    constexpr int NROWS = 27 * CINP;
+    constexpr int NT = (NROWS + 1 + 15) / 16;  // row tiles, the ones row included
+    constexpr int PAIRS = NT * MT;
+    constexpr int PW = (PAIRS + 3) / 4;  // tile pairs per wavefront
+    constexpr int G4 = CINP / 4;
+    __shared__ __attribute__((aligned(16))) uint16_t xs[3][CINP][LX][LY][BZ];
+    __shared__ __attribute__((aligned(16))) uint16_t gs[16 * MT][256];
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g4 = lane >> 4, m = lane & 15;
-    const int n = a.n;
-    const long nvox = (long)n * n * n;
-    const int nin = a.up ? n / 2 : n;
-    const int rt = blockIdx.y, ct = blockIdx.z;
-    const int row = rt * 16 + m;  // A row of this lane
-    const int t = row / CINP, ci = row % CINP;
-    int dx = 0, dy = 0, dz = 0;
-    if (t < 27) tap_offset(t, dx, dy, dz);
-    const int col = ct * 16 + m;  // B column of this lane
-    const long v0 = (long)blockIdx.x * a.vchunk;
-    const long v1 = min(nvox, v0 + a.vchunk);
-    v4f acc = v4f{0.f, 0.f, 0.f, 0.f};
-    for (long vb = v0; vb < v1; vb += 16) {
-        v4s av, bv;
+    const int n = a.n, nin = a.up ? n / 2 : n;
+    const int nbx = (n + BX - 1) / BX, nby = (n + BY - 1) / BY, nbz = (n + BZ - 1) / BZ;
+    const int nbricks = nbx * nby * nbz;
+    const v4s* xin = reinterpret_cast<const v4s*>(a.x);
+
+    // per tile pair of this wavefront: LDS element offset of its B row
+    // (tap and channel) and whether the row is real, the ones row or padding
+    int boff[PW], bkind[PW];
+#pragma unroll
+    for (int q = 0; q < PW; ++q) {
+        const int p = wave + 4 * q;
+        const int nt = p / MT;
+        const int r = nt * 16 + m;
+        boff[q] = 0;
+        bkind[q] = (p < PAIRS) ? (r < NROWS ? 0 : (r == NROWS ? 1 : 2)) : 2;
+        if (r < NROWS) {
+            int dx, dy, dz;
+            tap_offset(r / CINP, dx, dy, dz);
+            boff[q] = (((dz + 1) * CINP + r % CINP) * LX + (1 + dx)) * LY * BZ + (1 + dy) * BZ;
+        }
+    }
+    v4f acc[PW];
+#pragma unroll
+    for (int q = 0; q < PW; ++q) acc[q] = v4f{0.f, 0.f, 0.f, 0.f};
+    const v4s ones = v4s{(short)0x3F80, (short)0x3F80, (short)0x3F80, (short)0x3F80};
+
+    for (int b = blockIdx.x; b < nbricks; b += gridDim.x) {
+        const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
+        const int x0 = bx * BX, y0 = by * BY, z0 = bz * BZ;
+        __syncthreads();  // the previous brick's reads are done
+        // input: positions (x0-1 .. x0+4) x (y0-1 .. y0+4) x (z0-1 .. z0+16),
+        // 4-channel groups; position z_in goes to copy s at z = z_in - z0 - s + 1
+        for (int e = threadIdx.x; e < LX * LY * (BZ + 2) * G4; e += 256) {
+            const int q = e % G4, vz = (e / G4) % (BZ + 2), vy = (e / (G4 * (BZ + 2))) % LY,
+                      vx = e / (G4 * (BZ + 2) * LY);
+            const int ii = x0 - 1 + vx, jj = y0 - 1 + vy, kz = z0 - 1 + vz;
+            v4s val = v4s{0, 0, 0, 0};
+            if (ii >= 0 && ii < n && jj >= 0 && jj < n && kz >= 0 && kz < n) {
+                const int si = a.up ? ii >> 1 : ii, sj = a.up ? jj >> 1 : jj,
+                          sk = a.up ? kz >> 1 : kz;
+                val = xin[(((long)si * nin + sj) * nin + sk) * G4 + q];
+            }
+#pragma unroll
+            for (int sft = 0; sft < 3; ++sft) {
+                const int z = vz - sft;  // input z = z0 + z + sft - 1
+                if (z >= 0 && z < BZ) {
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) xs[sft][4 * q + c][vx][vy][z] = (uint16_t)val[c];
+                }
+            }
+        }
+        // gradient of the brick's voxels, transposed; zero outside the grid
+        for (int e = threadIdx.x; e < 256 * (4 * MT); e += 256) {
+            const int q = e % (4 * MT), v = e / (4 * MT);
+            const int vz = v % BZ, vy = (v / BZ) % BY, vx = v / (BZ * BY);
+            const int i = x0 + vx, j = y0 + vy, k = z0 + vz;
+            v4s val = v4s{0, 0, 0, 0};
+            if (4 * q < a.coutp && i < n && j < n && k < n)
+                val = *reinterpret_cast<const v4s*>(a.g + (((long)i * n + j) * n + k) * a.coutp +
+                                                    4 * q);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const int co = 4 * q + c;
+                gs[co][v] = (co < a.cout) ? (uint16_t)val[c] : (uint16_t)0;
+            }
+        }
+        __syncthreads();
+#pragma unroll 4
+        for (int ks = 0; ks < 16; ++ks) {  // 16 voxels: (vx, vy) = (ks / 4, ks % 4), z 0..15
+            const int vbase = ks * 16 + 4 * g4;
+            const int xoff = ((ks >> 2) * LY + (ks & 3)) * BZ + 4 * g4;
+            v4s av[MT];
+#pragma unroll
+            for (int mt = 0; mt < MT; ++mt)
+                av[mt] = *reinterpret_cast<const v4s*>(&gs[mt * 16 + m][vbase]);
+#pragma unroll
+            for (int q = 0; q < PW; ++q) {
+                if (wave + 4 * q >= PAIRS) break;
+                const int mt = (wave + 4 * q) % MT;
+                v4s bv = v4s{0, 0, 0, 0};
+                if (bkind[q] == 0)
+                    bv = *reinterpret_cast<const v4s*>(&xs[0][0][0][0][0] + boff[q] + xoff);
+                else if (bkind[q] == 1)
+                    bv = ones;
+                acc[q] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[mt], bv, acc[q], 0, 0, 0);
+            }
+        }
+    }
+    // D[co][row] of each pair: lane holds rows nt*16 + m, channels mt*16 + 4 g4 + i
+    const long len = (long)NROWS * a.cout + a.cout;
+    float* part = a.part + (long)blockIdx.x * len;
+#pragma unroll
+    for (int q = 0; q < PW; ++q) {
+        const int p = wave + 4 * q;
+        if (p >= PAIRS) break;
+        const int nt = p / MT, mt = p % MT;
+        const int r = nt * 16 + m;
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
-            const long v = vb + g4 * 4 + i;
-            short aval = 0, bval = 0;
-            if (v < v1) {
-                if (t < 27) {
-                    const int vk = (int)(v % n), vj = (int)((v / n) % n),
-                              vi = (int)(v / ((long)n * n));
-                    const int ii = vi + dx, jj = vj + dy, kz = vk + dz;
-                    if (ii >= 0 && ii < n && jj >= 0 && jj < n && kz >= 0 && kz < n) {
-                        const int si = a.up ? ii >> 1 : ii, sj = a.up ? jj >> 1 : jj,
-                                  sk = a.up ? kz >> 1 : kz;
-                        aval = (short)a.x[(((long)si * nin + sj) * nin + sk) * CINP + ci];
-                    }
-                }
-                if (col < a.cout) bval = (short)a.g[v * a.coutp + col];
-            }
-            av[i] = aval;
-            bv[i] = bval;
+            const int co = mt * 16 + 4 * g4 + i;
+            if (co >= a.cout) continue;
+            if (r < NROWS)
+                part[(long)r * a.cout + co] = acc[q][i];
+            else if (r == NROWS)
+                part[(long)NROWS * a.cout + co] = acc[q][i];
         }
-        acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc, 0, 0, 0);
-    }
-#pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int r = rt * 16 + g4 * 4 + i;  // D row held by this lane
-        if (r < 27 * CINP && col < a.cout && acc[i] != 0.f)
-            atomicAdd(&a.dw[(long)r * a.cout + col], acc[i]);
-    }
-    // bias gradient from the first row tile
-    if (a.db && rt == 0) {
-        float s = 0.f;
-        if (col < a.cout)
-            for (long v = v0 + g4; v < v1; v += 4) s += bf2f(a.g[v * a.coutp + col]);
-        s += __shfl_xor(s, 16, 64);
-        s += __shfl_xor(s, 32, 64);
-        if (g4 == 0 && col < a.cout && s != 0.f) atomicAdd(&a.db[col], s);
     }
 }
 
+// out[q] += sum over blocks b of part[b][q]; q < nw goes to dw, the rest to
+// db.  A block takes 64 consecutive q; its 16 rows of 64 threads each sum
+// the blocks b = row, row + 16, ... in order, and the 16 row sums are added
+// in row order, so the result is fixed for a given block count.
+__global__ void __launch_bounds__(1024) wgrad_reduce_kernel(const float* __restrict__ part,
+                                                            int nblk, long len, long nw,
+                                                            float* __restrict__ dw,
+                                                            float* __restrict__ db) {
+    __shared__ float s_row[16][64];
+    const int c = threadIdx.x & 63, rw = threadIdx.x >> 6;
+    const long q = (long)blockIdx.x * 64 + c;
+    float s = 0.f;
+    if (q < len)
+        for (int b = rw; b < nblk; b += 16) s += part[(long)b * len + q];
+    s_row[rw][c] = s;
+    __syncthreads();
+    if (rw == 0 && q < len) {
+        float t = s_row[0][c];
+#pragma unroll
+        for (int r = 1; r < 16; ++r) t += s_row[r][c];
+        if (q < nw)
+            dw[q] += t;
+        else
+            db[q - nw] += t;
+    }
+}
+
+template <int CINP>
+hipError_t launch_wgrad_cin(const WgradArgs& a, int nblk, hipStream_t st) {
+    if (a.cout > 16)
+        hipLaunchKernelGGL((conv_wgrad_lds_kernel<CINP, 2>), dim3(nblk), dim3(256), 0, st, a);
+    else
+        hipLaunchKernelGGL((conv_wgrad_lds_kernel<CINP, 1>), dim3(nblk), dim3(256), 0, st, a);
+    return hipGetLastError();
+}
+
 hipError_t conv_wgrad(const WgradArgs& a, hipStream_t st) {
-    const long nvox = (long)a.n * a.n * a.n;
-    const unsigned nchunk = (unsigned)((nvox + a.vchunk - 1) / a.vchunk);
-    const unsigned rtiles = (unsigned)((27 * a.cinp + 15) / 16);
-    const unsigned ctiles = (unsigned)((a.cout + 15) / 16);
-    dim3 grid(nchunk, rtiles, ctiles);
+    if (a.cout < 1 || a.cout > 32 || a.coutp < a.cout || a.coutp % 4 || a.n < 1 ||
+        (a.up && a.n % 2) || !a.part)
+        return hipErrorInvalidValue;
+    const long nbricks = (long)((a.n + BX - 1) / BX) * ((a.n + BY - 1) / BY) * ((a.n + BZ - 1) / BZ);
+    const int nblk = (int)(nbricks < WGRAD_MAX_BLOCKS ? nbricks : WGRAD_MAX_BLOCKS);
+    const long len = wgrad_part_len(a.cinp, a.cout);
+    if (len > wgrad_part_len(24, 24)) return hipErrorInvalidValue;
+    hipError_t e;
     switch (a.cinp) {
-        case 4: hipLaunchKernelGGL(conv_wgrad_kernel<4>, grid, dim3(64), 0, st, a); break;
-        case 8: hipLaunchKernelGGL(conv_wgrad_kernel<8>, grid, dim3(64), 0, st, a); break;
-        case 12: hipLaunchKernelGGL(conv_wgrad_kernel<12>, grid, dim3(64), 0, st, a); break;
-        case 16: hipLaunchKernelGGL(conv_wgrad_kernel<16>, grid, dim3(64), 0, st, a); break;
-        case 24: hipLaunchKernelGGL(conv_wgrad_kernel<24>, grid, dim3(64), 0, st, a); break;
+        case 4: e = launch_wgrad_cin<4>(a, nblk, st); break;
+        case 8: e = launch_wgrad_cin<8>(a, nblk, st); break;
+        case 12: e = launch_wgrad_cin<12>(a, nblk, st); break;
+        case 16: e = launch_wgrad_cin<16>(a, nblk, st); break;
+        case 24: e = launch_wgrad_cin<24>(a, nblk, st); break;
         default: return hipErrorInvalidValue;
     }
+    if (e != hipSuccess) return e;
+    const long nw = 27L * a.cinp * a.cout;
+    hipLaunchKernelGGL(wgrad_reduce_kernel, dim3((unsigned)((len + 63) / 64)), dim3(1024), 0, st,
+                       a.part, nblk, len, nw, a.dw, a.db);
     return hipGetLastError();
 }
 

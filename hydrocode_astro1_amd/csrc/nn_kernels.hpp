@@ -41,9 +41,14 @@ struct WgradArgs {
     const uint16_t* g;     // gradient w.r.t. the pre-activation [n^3][coutp]
     float* dw;             // [27][cinp][cout] FP32, accumulated
     float* db;             // [cout] FP32, accumulated
+    float* part;           // scratch: wgrad_scratch_len() floats of block partials
     int n, up, cinp, cout, coutp;
-    long vchunk;           // voxels per block
 };
+
+// Blocks of the weight-gradient kernel and the scratch it needs.
+constexpr int WGRAD_MAX_BLOCKS = 512;
+inline long wgrad_part_len(int cinp, int cout) { return 27L * cinp * cout + cout; }
+inline long wgrad_scratch_len() { return WGRAD_MAX_BLOCKS * wgrad_part_len(24, 24); }
 
 hipError_t conv_forward(const ConvArgs& a, hipStream_t st);
 hipError_t conv_wgrad(const WgradArgs& a, hipStream_t st);
