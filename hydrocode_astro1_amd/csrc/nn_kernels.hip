@@ -31,110 +31,129 @@ __device__ __forceinline__ void tap_offset(int t, int& dx, int& dy, int& dz) {
 
 // ------------------------------------------------------------- forward --
 // The convolution with the input tile staged in LDS.  A block owns a
-// 4 (x) x 4 (y) x 16 (z) brick of output voxels; wavefront w takes x slice w
-// and its four y rows as four 16-voxel M tiles along z.  The brick's input,
-// widened by one voxel on every side (upsampled on the fly for the decoder
-// layers, zero outside the grid), is read from HBM once into LDS, and every
-// A fragment then comes from LDS instead of a 27-fold global gather.
+// 4 (x) x 4 (y) x 16 (z) brick of output voxels at a time; wavefront w takes
+// x slice w and its four y rows as four 16-voxel M tiles along z.  The
+// brick's input, widened by one voxel on every side (upsampled on the fly
+// for the decoder layers, zero outside the grid), is read from HBM once into
+// LDS, so every A fragment comes from LDS instead of a 27-fold global gather.
+// The layer's packed B fragments are staged in LDS once per block, and the
+// block walks bricks blockIdx.x, blockIdx.x + gridDim.x, ...
 constexpr int BX = 4, BY = 4, BZ = 16;
 constexpr int LX = BX + 2, LY = BY + 2, LZ = BZ + 2;
+constexpr int CONV_MAX_BLOCKS = 4096;
 
-template <int CINP>
+template <int CINP, int NTL>
 __global__ void __launch_bounds__(256) conv_fwd_lds_kernel(ConvArgs a) {
     constexpr int KSTEPS = (27 * CINP + 15) / 16;
     constexpr int G4 = CINP / 4;  // 4-channel groups per voxel
     __shared__ v4s s_in[LX * LY * LZ * G4];
+    __shared__ v4s s_w[NTL * KSTEPS * 64];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int n = a.n;
-    const int nbz = (n + BZ - 1) / BZ, nby = (n + BY - 1) / BY;
-    const int bz = blockIdx.x % nbz, by = (blockIdx.x / nbz) % nby, bx = blockIdx.x / (nbz * nby);
-    const int x0 = bx * BX, y0 = by * BY, z0 = bz * BZ;
+    const int nbz = (n + BZ - 1) / BZ, nby = (n + BY - 1) / BY, nbx = (n + BX - 1) / BX;
+    const int nbricks = nbx * nby * nbz;
     const int nin = a.up ? n / 2 : n;
-
-    // stage the input brick: (x, y, z, group) entries, z and group fastest
     const v4s* xin = reinterpret_cast<const v4s*>(a.x);
-    for (int e = threadIdx.x; e < LX * LY * LZ * G4; e += 256) {
-        const int q = e % G4, vz = (e / G4) % LZ, vy = (e / (G4 * LZ)) % LY,
-                  vx = e / (G4 * LZ * LY);
-        const int ii = x0 - 1 + vx, jj = y0 - 1 + vy, kz = z0 - 1 + vz;
-        v4s val = v4s{0, 0, 0, 0};
-        if (ii >= 0 && ii < n && jj >= 0 && jj < n && kz >= 0 && kz < n) {
-            const int si = a.up ? ii >> 1 : ii, sj = a.up ? jj >> 1 : jj, sk = a.up ? kz >> 1 : kz;
-            val = xin[(((long)si * nin + sj) * nin + sk) * G4 + q];
-        }
-        s_in[e] = val;
-    }
-    __syncthreads();
-
-    const int g = lane >> 4, m = lane & 15;
-    const int ntiles = (a.cout + 15) / 16;
-    v4f acc[4][2];
-#pragma unroll
-    for (int mt = 0; mt < 4; ++mt)
-        for (int nt = 0; nt < 2; ++nt) acc[mt][nt] = v4f{0.f, 0.f, 0.f, 0.f};
     const v4s* wp = reinterpret_cast<const v4s*>(a.wp);
-    for (int ks = 0; ks < KSTEPS; ++ks) {
-        const int kk = (ks * 4 + g) * 4;
-        const int t = kk / CINP, ci = kk % CINP;
-        int dx = 0, dy = 0, dz = 0;
-        if (t < 27) tap_offset(t, dx, dy, dz);
-        const v4s b0 = wp[(0 * KSTEPS + ks) * 64 + lane];
-        v4s b1 = v4s{0, 0, 0, 0};
-        if (ntiles > 1) b1 = wp[(1 * KSTEPS + ks) * 64 + lane];
-#pragma unroll
-        for (int mt = 0; mt < 4; ++mt) {
-            // voxel (x0 + wave, y0 + mt, z0 + m) shifted by the tap, in the brick
-            v4s av = v4s{0, 0, 0, 0};
-            if (t < 27)
-                av = s_in[(((wave + 1 + dx) * LY + (mt + 1 + dy)) * LZ + (m + 1 + dz)) * G4 +
-                          ci / 4];
-            acc[mt][0] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, b0, acc[mt][0], 0, 0, 0);
-            if (ntiles > 1)
-                acc[mt][1] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, b1, acc[mt][1], 0, 0, 0);
+    for (int e = threadIdx.x; e < NTL * KSTEPS * 64; e += 256) s_w[e] = wp[e];
+    const int g = lane >> 4, m = lane & 15;
+
+    for (int blk = blockIdx.x; blk < nbricks; blk += gridDim.x) {
+        const int bz = blk % nbz, by = (blk / nbz) % nby, bx = blk / (nbz * nby);
+        const int x0 = bx * BX, y0 = by * BY, z0 = bz * BZ;
+        __syncthreads();  // the previous brick's reads are done
+        // stage the input brick: (x, y, z, group) entries, z and group fastest
+        for (int e = threadIdx.x; e < LX * LY * LZ * G4; e += 256) {
+            const int q = e % G4, vz = (e / G4) % LZ, vy = (e / (G4 * LZ)) % LY,
+                      vx = e / (G4 * LZ * LY);
+            const int ii = x0 - 1 + vx, jj = y0 - 1 + vy, kz = z0 - 1 + vz;
+            v4s val = v4s{0, 0, 0, 0};
+            if (ii >= 0 && ii < n && jj >= 0 && jj < n && kz >= 0 && kz < n) {
+                const int si = a.up ? ii >> 1 : ii, sj = a.up ? jj >> 1 : jj,
+                          sk = a.up ? kz >> 1 : kz;
+                val = xin[(((long)si * nin + sj) * nin + sk) * G4 + q];
+            }
+            s_in[e] = val;
         }
-    }
-    // epilogue: bias, ReLU, optional skip add, stores (D row = z offset)
-    const int xi = x0 + wave;
+        __syncthreads();
+
+        v4f acc[4][NTL];
 #pragma unroll
-    for (int mt = 0; mt < 4; ++mt)
-        for (int nt = 0; nt < 2; ++nt) {
-            if (nt >= ntiles) break;
-            const int c = nt * 16 + m;
-            const int yj = y0 + mt;
+        for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) {
-                const int zk = z0 + g * 4 + i;
-                if (xi >= n || yj >= n || zk >= n || c >= a.coutp) continue;
-                const long v = ((long)xi * n + yj) * n + zk;
-                float val = acc[mt][nt][i];
-                if (c < a.cout) {
-                    if (a.bias) val += a.bias[c];
-                    if (a.relu) val = val > 0.f ? val : 0.f;
-                } else {
-                    val = 0.f;
-                }
-                const uint16_t rb = f2bf(val);
-                if (a.yr) a.yr[v * a.coutp + c] = rb;
-                if (a.yd) {
-                    const float s = a.skip ? bf2f(a.skip[v * a.coutp + c]) : 0.f;
-                    a.yd[v * a.coutp + c] = f2bf(bf2f(rb) + s);
-                }
-                if (a.yf && c == 0) a.yf[v] = a.add ? val + a.add[v] : val;
+            for (int nt = 0; nt < NTL; ++nt) acc[mt][nt] = v4f{0.f, 0.f, 0.f, 0.f};
+        for (int ks = 0; ks < KSTEPS; ++ks) {
+            const int kk = (ks * 4 + g) * 4;
+            const int t = kk / CINP, ci = kk % CINP;
+            int dx = 0, dy = 0, dz = 0;
+            if (t < 27) tap_offset(t, dx, dy, dz);
+            v4s bw[NTL];
+#pragma unroll
+            for (int nt = 0; nt < NTL; ++nt) bw[nt] = s_w[(nt * KSTEPS + ks) * 64 + lane];
+#pragma unroll
+            for (int mt = 0; mt < 4; ++mt) {
+                // voxel (x0 + wave, y0 + mt, z0 + m) shifted by the tap, in the brick
+                v4s av = v4s{0, 0, 0, 0};
+                if (t < 27)
+                    av = s_in[(((wave + 1 + dx) * LY + (mt + 1 + dy)) * LZ + (m + 1 + dz)) * G4 +
+                              ci / 4];
+#pragma unroll
+                for (int nt = 0; nt < NTL; ++nt)
+                    acc[mt][nt] =
+                        __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bw[nt], acc[mt][nt], 0, 0, 0);
             }
         }
+        // epilogue: bias, ReLU, optional skip add, stores (D row = z offset)
+        const int xi = x0 + wave;
+#pragma unroll
+        for (int mt = 0; mt < 4; ++mt)
+#pragma unroll
+            for (int nt = 0; nt < NTL; ++nt) {
+                const int c = nt * 16 + m;
+                const int yj = y0 + mt;
+#pragma unroll
+                for (int i = 0; i < 4; ++i) {
+                    const int zk = z0 + g * 4 + i;
+                    if (xi >= n || yj >= n || zk >= n || c >= a.coutp) continue;
+                    const long v = ((long)xi * n + yj) * n + zk;
+                    float val = acc[mt][nt][i];
+                    if (c < a.cout) {
+                        if (a.bias) val += a.bias[c];
+                        if (a.relu) val = val > 0.f ? val : 0.f;
+                    } else {
+                        val = 0.f;
+                    }
+                    const uint16_t rb = f2bf(val);
+                    if (a.yr) a.yr[v * a.coutp + c] = rb;
+                    if (a.yd) {
+                        const float s = a.skip ? bf2f(a.skip[v * a.coutp + c]) : 0.f;
+                        a.yd[v * a.coutp + c] = f2bf(bf2f(rb) + s);
+                    }
+                    if (a.yf && c == 0) a.yf[v] = a.add ? val + a.add[v] : val;
+                }
+            }
+    }
+}
+
+template <int CINP>
+void launch_fwd_cin(const ConvArgs& a, unsigned nb, hipStream_t st) {
+    if (a.cout > 16)
+        hipLaunchKernelGGL((conv_fwd_lds_kernel<CINP, 2>), dim3(nb), dim3(256), 0, st, a);
+    else
+        hipLaunchKernelGGL((conv_fwd_lds_kernel<CINP, 1>), dim3(nb), dim3(256), 0, st, a);
 }
 
 hipError_t conv_forward(const ConvArgs& a, hipStream_t st) {
     if (a.cout < 1 || a.cout > 32 || a.coutp < a.cout || a.n < 1 || (a.up && a.n % 2))
         return hipErrorInvalidValue;  // one wavefront covers at most two 16-channel tiles
     const long nbricks = (long)((a.n + BX - 1) / BX) * ((a.n + BY - 1) / BY) * ((a.n + BZ - 1) / BZ);
-    const unsigned nb = (unsigned)nbricks;
+    const unsigned nb = (unsigned)(nbricks < CONV_MAX_BLOCKS ? nbricks : CONV_MAX_BLOCKS);
     switch (a.cinp) {
-        case 4: hipLaunchKernelGGL(conv_fwd_lds_kernel<4>, dim3(nb), dim3(256), 0, st, a); break;
-        case 8: hipLaunchKernelGGL(conv_fwd_lds_kernel<8>, dim3(nb), dim3(256), 0, st, a); break;
-        case 12: hipLaunchKernelGGL(conv_fwd_lds_kernel<12>, dim3(nb), dim3(256), 0, st, a); break;
-        case 16: hipLaunchKernelGGL(conv_fwd_lds_kernel<16>, dim3(nb), dim3(256), 0, st, a); break;
-        case 24: hipLaunchKernelGGL(conv_fwd_lds_kernel<24>, dim3(nb), dim3(256), 0, st, a); break;
+        case 4: launch_fwd_cin<4>(a, nb, st); break;
+        case 8: launch_fwd_cin<8>(a, nb, st); break;
+        case 12: launch_fwd_cin<12>(a, nb, st); break;
+        case 16: launch_fwd_cin<16>(a, nb, st); break;
+        case 24: launch_fwd_cin<24>(a, nb, st); break;
         default: return hipErrorInvalidValue;
     }
     return hipGetLastError();
