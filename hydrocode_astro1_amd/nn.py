@@ -218,9 +218,10 @@ def build_dataset(n, count, first_seed=0, omega=1.9, tol=1e-8, log=print):
     return ds
 
 
-def train(model, ds, epochs=4, batch=4, lr=2e-3, val=0, seed=0, log=print):
-    """Adam over shuffled mini-batches of fields [0, count - val); returns
-    (per-step losses, validation MSE on the last `val` fields or None)."""
+def train(model, ds, epochs=4, batch=4, lr=2e-3, val=0, seed=0, log=print, decay=0.5):
+    """Adam over shuffled mini-batches of fields [0, count - val), learning
+    rate lr * decay^epoch; returns (per-step losses, validation MSE on the
+    last `val` fields or None)."""
     L = lib()
     rng = np.random.default_rng(seed)
     ntrain = ds.count - val
@@ -230,7 +231,7 @@ def train(model, ds, epochs=4, batch=4, lr=2e-3, val=0, seed=0, log=print):
         order = rng.permutation(ntrain).astype(np.int32)
         steps = ntrain // batch
         order = np.ascontiguousarray(order[:steps * batch])
-        lr_ep = lr * (0.5 ** ep)
+        lr_ep = lr * (decay ** ep)
         ll = np.zeros(max(steps, 1), dtype=np.float32)
         rc = L.nn_train(model.h, ds.h, order.ctypes.data_as(_ip), steps, batch, lr_ep, _f(ll))
         if rc != 0:
@@ -257,13 +258,15 @@ def main(argv=None):
     ap.add_argument("--epochs", type=int, default=4)
     ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--lr", type=float, default=2e-3)
+    ap.add_argument("--decay", type=float, default=0.5, help="learning-rate factor per epoch")
     ap.add_argument("--omega", type=float, default=1.9)
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
     a = ap.parse_args(argv)
     ds = build_dataset(a.n, a.fields, omega=a.omega)
     model = Model(seed=a.seed)
-    losses, vmse = train(model, ds, a.epochs, a.batch, a.lr, val=a.val, seed=a.seed)
+    losses, vmse = train(model, ds, a.epochs, a.batch, a.lr, val=a.val, seed=a.seed,
+                         decay=a.decay)
     os.makedirs(os.path.dirname(os.path.abspath(a.out)), exist_ok=True)
     model.save(a.out)
     print(f"[nn] {model.num_params()} parameters, final loss {losses[-1]:.3e}, "
