@@ -206,10 +206,13 @@ int sor_fp64(GravitySolver3D* s) {
     std::vector<SorLaunch> launches;
 
     auto single = [&](int it) -> int {
-        // a single sweep clears the slot of the iteration after it; only a
-        // fused pair leaves the slot of the next iteration to be cleared
+        // the check before a single iteration clears the slots of the
+        // iteration after it; only a fused pair leaves the slots of the next
+        // iteration to be cleared
         if (!rel && !launches.empty() && launches.back().count == 2)
             HYD_CHECK(hyd::launch_sor_prepare(d->ctl, it, d->stream));
+        // the stop rule of iteration it, evaluated once for both sweeps
+        if (!rel) HYD_CHECK(hyd::launch_sor_check(d->ctl, it, tol, d->stream));
         HYD_CHECK(hyd::launch_sor_sweep(d->g, c, 0, it, stol, buf_r[cur], buf_b[cur], d->rhs_r,
                                         d->rhs_b, d->ctl, d->stream));
         HYD_CHECK(hyd::launch_sor_sweep(d->g, c, 1, it, stol, buf_r[cur], buf_b[cur], d->rhs_r,
@@ -294,7 +297,7 @@ int sor_fp64(GravitySolver3D* s) {
         const SorCtl& h = *d->ctl_host;
         if (h.converged) {
             const int det = h.iters;  // first iteration that did not run
-            const double rm2 = det >= 2 ? bitsd(h.res[(det - 2) & 3]) : 1.0;
+            const double rm2 = det >= 2 ? bitsd(hyd::sor_slot_max(h, (det - 2) & 3)) : 1.0;
             // the launch that ran iteration det - 1
             const SorLaunch* L = nullptr;
             for (auto r = launches.rbegin(); r != launches.rend(); ++r)
@@ -321,13 +324,13 @@ int sor_fp64(GravitySolver3D* s) {
             return 0;
         }
         finish(cur);
-        const double r1 = bitsd(h.res[(it - 1) & 3]);
+        const double r1 = bitsd(hyd::sor_slot_max(h, (it - 1) & 3));
         if (it >= s->max_iterations) {
             s->last_iterations = it;
             s->last_residual = r1;
             return -1;
         }
-        const double r2 = it >= 2 ? bitsd(h.res[(it - 2) & 3]) : 0.0;
+        const double r2 = it >= 2 ? bitsd(hyd::sor_slot_max(h, (it - 2) & 3)) : 0.0;
         batch = next_batch(r1, r2, tol, batch);
         launches.erase(launches.begin(),
                        launches.end() - std::min<size_t>(launches.size(), 2));

@@ -51,17 +51,18 @@ __global__ void split_to_natural_kernel(GridDims g, const double* __restrict__ r
 // k = 2m + ((i+j+colour)&1).  No integer division in the index math.
 constexpr int SOR_ROWS = 4;
 
-// True when iteration `it` must not run; `writer` records the stop.
+// True when iteration `it` must not run; `writer` records the stop.  Called
+// by every lane of one wavefront.
 __device__ __forceinline__ bool sor_stop_check(SorCtl* ctl, int it, double tol, bool writer) {
     if (ctl->converged) return true;
     double prev = 0.0;
     bool stop = false;
     if (it > 0) {
-        prev = bitsd(ctl->res[(it - 1) & 3]);
+        prev = bitsd(sor_wave_slot_max(ctl, (it - 1) & 3));
         stop = prev < tol;
     }
     if (!stop && it > 1) {
-        const double prev2 = bitsd(ctl->res[(it - 2) & 3]);
+        const double prev2 = bitsd(sor_wave_slot_max(ctl, (it - 2) & 3));
         if (prev2 < tol) {
             stop = true;
             prev = prev2;
@@ -80,16 +81,12 @@ __global__ void __launch_bounds__(256) sor_sweep_kernel(GridDims g, SorCoef c, i
                                                         const double* __restrict__ other,
                                                         const double* __restrict__ rhs,
                                                         SorCtl* ctl) {
-    // stop rule of gravity_solver_3d.c:715: iteration it runs only while the
-    // residual of iteration it-1 is still >= tol (and, after a fused pair,
-    // that of it-2)
-    // (tol < 0: no stop rule and no residual, the relative-residual mode)
-    if (tol >= 0.0) {
-        const bool writer = colour == 0 && blockIdx.x == 0 && blockIdx.y == 0 &&
-                            blockIdx.z == 0 && threadIdx.x == 0;
-        if (sor_stop_check(ctl, it, tol, writer)) return;
-        if (writer) ctl->res[(it + 1) & 3] = 0ull;
-    }
+    // stop rule of gravity_solver_3d.c:715: sor_check_kernel, launched before
+    // the red sweep of each iteration, sets converged when the iteration
+    // must not run (tol < 0: no stop rule and no residual, the
+    // relative-residual mode)
+    if (tol >= 0.0 && ctl->converged) return;
+    const unsigned blin = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
 
     const int m = blockIdx.x * 64 + (threadIdx.x & 63);
     const int j = 1 + blockIdx.y * SOR_ROWS + (threadIdx.x >> 6);
@@ -110,28 +107,36 @@ __global__ void __launch_bounds__(256) sor_sweep_kernel(GridDims g, SorCoef c, i
     }
     if (tol < 0.0) return;  // relative-residual mode: the max update is not tracked
     rb = block_max_u64(rb);
-    // most blocks see a running max already above theirs: skip their atomic
-    if (threadIdx.x == 0 && rb != 0ull) {
-        const unsigned long long cur = __atomic_load_n(&ctl->res[it & 3], __ATOMIC_RELAXED);
-        if (rb > cur) atomicMax(&ctl->res[it & 3], rb);
+    if (threadIdx.x == 0 && rb != 0ull) sor_slot_add(ctl, it & 3, blin, rb);
+}
+
+// one wavefront
+__global__ void sor_reset_kernel(SorCtl* ctl) {
+    const int lane = threadIdx.x;
+    for (int q = 0; q < 4; ++q)
+        for (int s = lane; s < SOR_SLOTS; s += 64) ctl->slot[q][s][0] = 0ull;
+    if (lane == 0) {
+        ctl->converged = 0;
+        ctl->iters = 0;
+        ctl->last_res = 0.0;
     }
 }
 
-__global__ void sor_reset_kernel(SorCtl* ctl) {
-    for (int q = 0; q < 4; ++q) ctl->res[q] = 0ull;
-    ctl->converged = 0;
-    ctl->iters = 0;
-    ctl->last_res = 0.0;
-}
-
+// one wavefront
 __global__ void sor_prepare_kernel(SorCtl* ctl, int it) {
     if (ctl->converged) return;
-    ctl->res[it & 3] = 0ull;
-    ctl->res[(it + 1) & 3] = 0ull;
+    for (int s = threadIdx.x; s < SOR_SLOTS; s += 64) {
+        ctl->slot[it & 3][s][0] = 0ull;
+        ctl->slot[(it + 1) & 3][s][0] = 0ull;
+    }
 }
 
+// one wavefront: the stop rule of iteration it (sets converged when it must
+// not run); otherwise clears the slots of iteration it+1
 __global__ void sor_check_kernel(SorCtl* ctl, int it, double tol) {
-    (void)sor_stop_check(ctl, it, tol, true);
+    const bool stop = sor_stop_check(ctl, it, tol, threadIdx.x == 0);
+    if (!stop)
+        for (int s = threadIdx.x; s < SOR_SLOTS; s += 64) ctl->slot[(it + 1) & 3][s][0] = 0ull;
 }
 
 // ---------------------------------------------------- multipole boundary --
@@ -776,12 +781,12 @@ hipError_t launch_sor_sweep(const GridDims& g, const SorCoef& c, int colour, int
 }
 
 hipError_t launch_sor_prepare(SorCtl* ctl, int it, hipStream_t st) {
-    hipLaunchKernelGGL(sor_prepare_kernel, dim3(1), dim3(1), 0, st, ctl, it);
+    hipLaunchKernelGGL(sor_prepare_kernel, dim3(1), dim3(64), 0, st, ctl, it);
     return hipGetLastError();
 }
 
 hipError_t launch_sor_check(SorCtl* ctl, int it, double tol, hipStream_t st) {
-    hipLaunchKernelGGL(sor_check_kernel, dim3(1), dim3(1), 0, st, ctl, it, tol);
+    hipLaunchKernelGGL(sor_check_kernel, dim3(1), dim3(64), 0, st, ctl, it, tol);
     return hipGetLastError();
 }
 
@@ -795,7 +800,7 @@ hipError_t launch_sor_boundary(const GridDims& g, int bc, double G, int lmax, co
 }
 
 hipError_t launch_sor_reset(SorCtl* ctl, hipStream_t st) {
-    hipLaunchKernelGGL(sor_reset_kernel, dim3(1), dim3(1), 0, st, ctl);
+    hipLaunchKernelGGL(sor_reset_kernel, dim3(1), dim3(64), 0, st, ctl);
     return hipGetLastError();
 }
 

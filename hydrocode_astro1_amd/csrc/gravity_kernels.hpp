@@ -18,17 +18,56 @@ struct GridDims {
 };
 
 // Per-solve control of the SOR iteration on the device.  The residual of
-// iteration t is accumulated in res[t & 3]; a launch clears the slots it is
-// about to fill.  A launch of iteration t runs only while the residuals of
-// iterations t-1 and t-2 are both >= tol; the first launch that sees one
-// below it sets converged and iters = t, and the host works out which
-// iteration met the tolerance (t-1, or t-2 when t-2 and t-1 ran as a pair).
+// iteration t (the largest |phi_GS - phi_old|, as ordered bits) is the max
+// over SOR_SLOTS slots of slot[t & 3]; a block adds its maximum to slot
+// (block & (SOR_SLOTS - 1)), each slot on its own 128-byte line, so the
+// atomics of one sweep spread over SOR_SLOTS addresses.  Iteration t runs
+// only while the residuals of iterations t-1 and t-2 are both >= tol: a check
+// kernel before the single sweeps of t (or the fused pass itself) evaluates
+// that rule, clears the slots about to be filled, and on the first residual
+// below tol sets converged and iters = t; the host works out which iteration
+// met the tolerance (t-1, or t-2 when t-2 and t-1 ran as a pair).
+constexpr int SOR_SLOTS = 256;
 struct SorCtl {
-    unsigned long long res[4];  // max |phi_GS - phi_old| per iteration (ordered bits)
     int converged;
     int iters;                  // first iteration that did not run
     double last_res;
+    unsigned long long pad_[14];  // the header keeps a line of its own
+    unsigned long long slot[4][SOR_SLOTS][16];  // value in [..][..][0]
 };
+
+// Host view: the residual of slot group q.
+inline unsigned long long sor_slot_max(const SorCtl& c, int q) {
+    unsigned long long m = 0ull;
+    for (int s = 0; s < SOR_SLOTS; ++s) m = c.slot[q][s][0] > m ? c.slot[q][s][0] : m;
+    return m;
+}
+
+#if defined(__HIP_DEVICE_COMPILE__) || defined(__HIPCC__)
+// Residual of slot group q, read by one whole wavefront (every lane gets it).
+__device__ __forceinline__ unsigned long long sor_wave_slot_max(const SorCtl* c, int q) {
+    const int lane = threadIdx.x & 63;
+    unsigned long long v = 0ull;
+    for (int s = lane; s < SOR_SLOTS; s += 64) {
+        const unsigned long long w = c->slot[q][s][0];
+        v = w > v ? w : v;
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const unsigned long long w = __shfl_xor(v, o, 64);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+// One block's maximum into its slot of group q (skipped when the slot
+// already holds as much).
+__device__ __forceinline__ void sor_slot_add(SorCtl* c, int q, unsigned block,
+                                             unsigned long long v) {
+    unsigned long long* p = &c->slot[q][block & (SOR_SLOTS - 1)][0];
+    if (v > __atomic_load_n(p, __ATOMIC_RELAXED)) atomicMax(p, v);
+}
+#endif
 
 struct SorCoef {
     double dx2i, dy2i, dz2i, factor, omega;

@@ -183,14 +183,15 @@ __global__ void __launch_bounds__(FT) sor2_kernel(SorFusedArgs<T> a) {
     if (TRACK) {
         // the stop rule of the single sweeps (sor_stop_check): nothing runs
         // once iteration it-1 or it-2 met the tolerance
+        // (every wavefront reads the slots itself: all lanes are active here)
         bool stop = a.ctl->converged != 0;
         double prev = 0.0;
         if (!stop && a.it > 0) {
-            prev = bitsd(a.ctl->res[(a.it - 1) & 3]);
+            prev = bitsd(sor_wave_slot_max(a.ctl, (a.it - 1) & 3));
             stop = prev < a.tol;
         }
         if (!stop && a.it > 1) {
-            prev = bitsd(a.ctl->res[(a.it - 2) & 3]);
+            prev = bitsd(sor_wave_slot_max(a.ctl, (a.it - 2) & 3));
             stop = prev < a.tol;
         }
         if (stop) {
@@ -273,10 +274,8 @@ __global__ void __launch_bounds__(FT) sor2_kernel(SorFusedArgs<T> a) {
         __syncthreads();
         r2 = block_max_u64(r2);
         if (threadIdx.x == 0) {
-            unsigned long long* s1 = &a.ctl->res[a.it & 3];
-            unsigned long long* s2 = &a.ctl->res[(a.it + 1) & 3];
-            if (r1 > __atomic_load_n(s1, __ATOMIC_RELAXED)) atomicMax(s1, r1);
-            if (r2 > __atomic_load_n(s2, __ATOMIC_RELAXED)) atomicMax(s2, r2);
+            sor_slot_add(a.ctl, a.it & 3, blockIdx.x, r1);
+            sor_slot_add(a.ctl, (a.it + 1) & 3, blockIdx.x, r2);
         }
     }
 }
