@@ -314,8 +314,9 @@ __device__ __forceinline__ int mslot(int c) { return c + (c >> 4); }
 // The pairwise double-double tree over the block's 256 consecutive cells of
 // each of NQ terms: 16 threads per quantity each sum a 16-leaf subtree, then
 // one thread sums the 16 subtrees; the block's sum of quantity q goes to
-// part[q][blockIdx.x].  The same tree as a lane butterfly followed by the
-// wavefronts pairwise, at one double-double addition per leaf.
+// part[q][blockIdx.x].  Two leaves meet in an exact two-sum (their
+// double-double sum with zero low parts), every later node in a
+// double-double addition.
 template <int NQ>
 __device__ __forceinline__ void block_tree_terms(const double (&term)[NQ], double* __restrict__ part,
                                                  double (*s_term)[MROW], DD (*s_part)[16]) {
@@ -332,7 +333,7 @@ __device__ __forceinline__ void block_tree_terms(const double (&term)[NQ], doubl
                 const double* leaf = &s_term[qq][mslot(st * 16)];
                 DD v[8];
 #pragma unroll
-                for (int e = 0; e < 8; ++e) v[e] = dd_add(DD{leaf[2 * e], 0.0}, DD{leaf[2 * e + 1], 0.0});
+                for (int e = 0; e < 8; ++e) v[e] = two_sum(leaf[2 * e], leaf[2 * e + 1]);
 #pragma unroll
                 for (int e = 0; e < 4; ++e) v[e] = dd_add(v[2 * e], v[2 * e + 1]);
                 v[0] = dd_add(v[0], v[1]);

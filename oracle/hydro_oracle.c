@@ -239,7 +239,8 @@ static void tree_push(DDTree *t, double v) {
     DD cur = {v, 0.0};
     int lev = 0;
     while (t->top > 0 && t->level[t->top - 1] == lev) {
-        cur = dd_add(t->node[t->top - 1], cur);
+        /* two leaves meet in an exact two-sum, later nodes in dd_add */
+        cur = lev == 0 ? two_sum(t->node[t->top - 1].hi, v) : dd_add(t->node[t->top - 1], cur);
         t->top--;
         lev++;
     }
