@@ -28,6 +28,7 @@ GRAV_BC_PERIODIC, GRAV_BC_ISOLATED, GRAV_BC_DIRICHLET = 0, 1, 2
 POISSON_FP64_SOR, POISSON_FP32_DC, POISSON_NN_FP32_DC = 0, 1, 2
 SOR_CRIT_MAX_UPDATE, SOR_CRIT_REL_RESID = 0, 1
 DC_INNER_SOR, DC_INNER_MULTIGRID = 0, 1
+MOMENT_ORDER_TREE, MOMENT_ORDER_SEQUENTIAL = 0, 1
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -53,7 +54,7 @@ class GravitySolver3D(C.Structure):
         ("last_rel_residual", C.c_double), ("last_solve_ms", C.c_double),
         ("nn_model", C.c_void_p), ("last_guess", C.c_int),
         ("last_guess_rel_residual", C.c_double), ("last_warm_rel_residual", C.c_double),
-        ("dc_inner", C.c_int),
+        ("dc_inner", C.c_int), ("moment_order", C.c_int),
     ]
 
 
@@ -119,6 +120,7 @@ EXPORTS = [
     ("gravity_set_multipole_lmax", C.c_int, [_GP, C.c_int]),
     ("gravity_set_poisson_mode", C.c_int, [_GP, C.c_int]),
     ("gravity_set_dc_inner", C.c_int, [_GP, C.c_int]),
+    ("gravity_set_moment_order", C.c_int, [_GP, C.c_int]),
     ("gravity_set_criterion", C.c_int, [_GP, C.c_int]),
     ("gravity_sync_to_device", C.c_int, [_GP]),
     ("gravity_sync_to_host", C.c_int, [_GP]),
@@ -227,6 +229,12 @@ class Gravity:
         """DC_INNER_SOR or DC_INNER_MULTIGRID for the FP32 modes."""
         if self.L.gravity_set_dc_inner(self.h, inner) != 0:
             raise ValueError(f"bad inner solver {inner} for this grid")
+
+    def set_moment_order(self, order):
+        """MOMENT_ORDER_TREE (default) or MOMENT_ORDER_SEQUENTIAL, the
+        reference's cell-after-cell sums."""
+        if self.L.gravity_set_moment_order(self.h, order) != 0:
+            raise ValueError(f"bad moment order {order}")
 
     def set_nn_model(self, model):
         """Attach an nn.Model for POISSON_NN_FP32_DC (None detaches)."""

@@ -41,10 +41,31 @@ int dev_alloc(T** p, long count) {
     return 0;
 }
 
+// The multipole moments of rho (device) into d->mom, in the solver's order.
+int moments_dev(GravitySolver3D* s, const double* d_rho) {
+    GravDevice* d = s->dev;
+    if (s->moment_order != MOMENT_ORDER_SEQUENTIAL) {
+        HYD_CHECK(hyd::launch_moments(d->g, d_rho, s->multipole_lmax, d->mom_scratch, d->mom,
+                                      d->stream));
+        return 0;
+    }
+    const long len = hyd::moments_seq_len(d->n, s->multipole_lmax);
+    if (len > d->mom_seq_len) {
+        if (d->mom_seq) HYD_CHECK(hipFree(d->mom_seq));
+        d->mom_seq = nullptr;
+        d->mom_seq_len = 0;
+        if (dev_alloc(&d->mom_seq, len)) return -2;
+        d->mom_seq_len = len;
+    }
+    HYD_CHECK(hyd::launch_moments_sequential(d->g, d_rho, s->multipole_lmax, d->mom_seq, d->mom,
+                                             d->stream));
+    return 0;
+}
+
 void dev_free(GravDevice* d) {
     if (!d) return;
     void* ptrs[] = {d->phi_r, d->phi_b, d->rhs_r, d->rhs_b, d->gx,  d->gy, d->gz,
-                    d->rho_tmp, d->nat_tmp, d->mom, d->mom_scratch, d->wlm, d->ctl, d->acc, d->res_part,
+                    d->rho_tmp, d->nat_tmp, d->mom, d->mom_scratch, d->mom_seq, d->wlm, d->ctl, d->acc, d->res_part,
                     d->e_r, d->e_b, d->r_r, d->r_b, d->guess_r, d->guess_b,
                     d->phi2_r, d->phi2_b, d->e2_r, d->e2_b};
     for (void* p : ptrs)
@@ -625,9 +646,7 @@ int potential_device(GravitySolver3D* s, const double* d_rho) {
     HYD_CHECK(hipEventRecord(d->ev0, d->stream));
     const double four_pi_G = 4.0 * kPi * s->G;  // gravity_solver_3d.c:527
     HYD_CHECK(hyd::launch_natural_to_split(d->g, d_rho, four_pi_G, d->rhs_r, d->rhs_b, d->stream));
-    if (s->bc_type == GRAV_BC_ISOLATED)
-        HYD_CHECK(hyd::launch_moments(d->g, d_rho, s->multipole_lmax, d->mom_scratch, d->mom,
-                                      d->stream));
+    if (s->bc_type == GRAV_BC_ISOLATED && moments_dev(s, d_rho)) return -2;
     int st;
     if (s->poisson_mode == POISSON_FP64_SOR) {
         st = sor_fp64(s);
@@ -716,6 +735,7 @@ GravitySolver3D* gravity_solver_create(int nx, int ny, int nz, double xmin, doub
     s->poisson_mode = POISSON_FP64_SOR;
     s->criterion = SOR_CRIT_MAX_UPDATE;
     s->dc_inner = DC_INNER_SOR;
+    s->moment_order = MOMENT_ORDER_TREE;
     const long n = (long)nx * ny * nz;
     s->phi = (double*)calloc(n, sizeof(double));
     s->rho = NULL;
@@ -817,6 +837,12 @@ int gravity_set_poisson_mode(GravitySolver3D* s, int mode) {
     return 0;
 }
 
+int gravity_set_moment_order(GravitySolver3D* s, int order) {
+    if (order != MOMENT_ORDER_TREE && order != MOMENT_ORDER_SEQUENTIAL) return -1;
+    s->moment_order = order;
+    return 0;
+}
+
 int gravity_set_dc_inner(GravitySolver3D* s, int inner) {
     if (inner != DC_INNER_SOR && inner != DC_INNER_MULTIGRID) return -1;
     if (inner == DC_INNER_MULTIGRID && !mg_supported(s)) {
@@ -861,8 +887,7 @@ int gravity_sync_to_host(GravitySolver3D* s) {
 int gravity_moments(GravitySolver3D* s, const double* rho, double* out13, double* out_hi) {
     GravDevice* d = s->dev;
     if (upload_rho(s, rho)) return -2;
-    HYD_CHECK(hyd::launch_moments(d->g, d->rho_tmp, s->multipole_lmax, d->mom_scratch, d->mom,
-                                  d->stream));
+    if (moments_dev(s, d->rho_tmp)) return -2;
     const int nhi = hyd::mp_n_hi(s->multipole_lmax);
     double buf[13 + 128];
     HYD_CHECK(hipMemcpyAsync(buf, d->mom, sizeof(double) * (13 + nhi), hipMemcpyDeviceToHost,
@@ -876,9 +901,7 @@ int gravity_moments(GravitySolver3D* s, const double* rho, double* out13, double
 int gravity_boundary_potential(GravitySolver3D* s, const double* rho, double* phi_out) {
     GravDevice* d = s->dev;
     if (upload_rho(s, rho)) return -2;
-    if (s->bc_type == GRAV_BC_ISOLATED)
-        HYD_CHECK(hyd::launch_moments(d->g, d->rho_tmp, s->multipole_lmax, d->mom_scratch, d->mom,
-                                      d->stream));
+    if (s->bc_type == GRAV_BC_ISOLATED && moments_dev(s, d->rho_tmp)) return -2;
     if (apply_boundary(s)) return -2;
     if (!d->nat_tmp && dev_alloc(&d->nat_tmp, d->n)) return -2;
     HYD_CHECK(hyd::launch_split_to_natural(d->g, d->phi_r, d->phi_b, d->nat_tmp, d->stream));

@@ -359,10 +359,27 @@ __device__ __forceinline__ void block_tree_terms(const double (&term)[NQ], doubl
     }
 }
 
+// Where a block's terms go: the double-double tree (part = block partials),
+// or, for the reference's sequential order, stored as they are (part =
+// terms[q][npad], cell index fastest; the grid covers npad cells and the
+// padding cells store zeros).
+template <int NQ>
+__device__ __forceinline__ void emit_terms(bool store, const double (&term)[NQ], long t, long npad,
+                                           double* __restrict__ part, double (*s_term)[MROW],
+                                           DD (*s_part)[16]) {
+    if (store) {
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) part[(long)q * npad + t] = term[q];
+    } else {
+        block_tree_terms<NQ>(term, part, s_term, s_part);
+    }
+}
+
 // pass 1: M and the first moments (gravity_solver_3d.c:182-198)
+template <bool STORE>
 __global__ void __launch_bounds__(256) moments_pass1_kernel(GridDims g,
                                                             const double* __restrict__ rho,
-                                                            double* __restrict__ part) {
+                                                            double* __restrict__ part, long npad) {
     __shared__ double s_term[MQC][MROW];
     __shared__ DD s_part[MQC][16];
     const long n = (long)g.nx * g.ny * g.nz;
@@ -381,7 +398,7 @@ __global__ void __launch_bounds__(256) moments_pass1_kernel(GridDims g,
             term[0] = dm; term[1] = dm * x; term[2] = dm * y; term[3] = dm * z;
         }
     }
-    block_tree_terms<4>(term, part, s_term, s_part);
+    emit_terms<4>(STORE, term, t, npad, part, s_term, s_part);
 }
 
 constexpr int mp_nhi_c(int lmax) {
@@ -391,11 +408,11 @@ constexpr int mp_nhi_c(int lmax) {
 }
 
 // pass 2: Q_ij (gravity_solver_3d.c:226-258) and A_lm, B_lm for l >= 3
-template <int LMAX>
+template <int LMAX, bool STORE>
 __global__ void __launch_bounds__(256) moments_pass2_kernel(GridDims g,
                                                             const double* __restrict__ rho,
                                                             const double* __restrict__ mom,
-                                                            double* __restrict__ part) {
+                                                            double* __restrict__ part, long npad) {
     constexpr int NQ = 9 + mp_nhi_c(LMAX);
     __shared__ double s_term[MQC][MROW];
     __shared__ DD s_part[MQC][16];
@@ -468,7 +485,54 @@ __global__ void __launch_bounds__(256) moments_pass2_kernel(GridDims g,
             }
         }
     }
-    block_tree_terms<NQ>(term, part, s_term, s_part);
+    emit_terms<NQ>(STORE, term, t, npad, part, s_term, s_part);
+}
+
+// The reference's order: s = 0; s = s + x[0]; s = s + x[1]; ... over the
+// npad stored terms of quantity blockIdx.x (gravity_solver_3d.c:182-198,
+// 226-258 add cell by cell, i outermost, k innermost; skipped cells and the
+// padding hold zeros, and adding a zero leaves a sum that started at +0
+// unchanged).  One wavefront adds, the other three stage the next tile in
+// LDS; the add chain is the kernel's whole cost.
+constexpr int SEQ_TILE = 2048;
+
+__global__ void __launch_bounds__(256) seq_sum_kernel(const double* __restrict__ terms, long npad,
+                                                      double* __restrict__ sums) {
+    __shared__ double buf[2][SEQ_TILE];
+    const double* x = terms + (long)blockIdx.x * npad;
+    const long ntiles = npad / SEQ_TILE;
+    const int tid = threadIdx.x;
+    for (int e = tid; e < SEQ_TILE; e += 256) buf[0][e] = x[e];
+    __syncthreads();
+    double s = 0.0;
+    for (long t = 0; t < ntiles; ++t) {
+        if (tid >= 64) {
+            if (t + 1 < ntiles) {
+                const double2* src = reinterpret_cast<const double2*>(x + (t + 1) * SEQ_TILE);
+                double2* dst = reinterpret_cast<double2*>(buf[(t + 1) & 1]);
+                double2 v[6];
+#pragma unroll
+                for (int u = 0; u < 6; ++u) {
+                    const int e = tid - 64 + 192 * u;
+                    if (e < SEQ_TILE / 2) v[u] = src[e];
+                }
+#pragma unroll
+                for (int u = 0; u < 6; ++u) {
+                    const int e = tid - 64 + 192 * u;
+                    if (e < SEQ_TILE / 2) dst[e] = v[u];
+                }
+            }
+        } else {
+            const double* b = buf[t & 1];
+#pragma unroll 32
+            for (int e = 0; e < SEQ_TILE; ++e) s = s + b[e];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        sums[2 * blockIdx.x] = s;
+        sums[2 * blockIdx.x + 1] = 0.0;
+    }
 }
 
 // Fold 256 consecutive partials of quantity q (zero padded) pairwise.
@@ -731,26 +795,60 @@ static double* reduce_partials(double* part, long nb, int nq, hipStream_t st) {
     return cur;
 }
 
-hipError_t launch_moments(const GridDims& g, const double* rho, int lmax, double* scratch,
-                          double* mom_out, hipStream_t st) {
-    const long n = (long)g.nx * g.ny * g.nz;
-    const long nb = (n + 255) / 256;
-    hipLaunchKernelGGL(moments_pass1_kernel, dim3((unsigned)nb), dim3(256), 0, st, g, rho, scratch);
-    double* s1 = reduce_partials(scratch, nb, 4, st);
-    hipLaunchKernelGGL(moments_finish_kernel, dim3(1), dim3(64), 0, st, s1, 4, 1, g, mom_out);
-    const int nq = mp_n_pass2(lmax);
+template <bool STORE>
+static hipError_t launch_pass2(const GridDims& g, const double* rho, int lmax, const double* mom,
+                               double* part, long npad, long nb, hipStream_t st) {
     switch (lmax) {
-#define MP_CASE(L)                                                                               \
-    case L:                                                                                      \
-        hipLaunchKernelGGL(moments_pass2_kernel<L>, dim3((unsigned)nb), dim3(256), 0, st, g, rho, \
-                           mom_out, scratch);                                                    \
+#define MP_CASE(L)                                                                          \
+    case L:                                                                                 \
+        hipLaunchKernelGGL((moments_pass2_kernel<L, STORE>), dim3((unsigned)nb), dim3(256), 0, \
+                           st, g, rho, mom, part, npad);                                   \
         break;
         MP_CASE(2) MP_CASE(3) MP_CASE(4) MP_CASE(5) MP_CASE(6) MP_CASE(7) MP_CASE(8)
 #undef MP_CASE
         default: return hipErrorInvalidValue;
     }
+    return hipSuccess;
+}
+
+hipError_t launch_moments(const GridDims& g, const double* rho, int lmax, double* scratch,
+                          double* mom_out, hipStream_t st) {
+    const long n = (long)g.nx * g.ny * g.nz;
+    const long nb = (n + 255) / 256;
+    hipLaunchKernelGGL(moments_pass1_kernel<false>, dim3((unsigned)nb), dim3(256), 0, st, g, rho,
+                       scratch, 0L);
+    double* s1 = reduce_partials(scratch, nb, 4, st);
+    hipLaunchKernelGGL(moments_finish_kernel, dim3(1), dim3(64), 0, st, s1, 4, 1, g, mom_out);
+    const int nq = mp_n_pass2(lmax);
+    const hipError_t e = launch_pass2<false>(g, rho, lmax, mom_out, scratch, 0L, nb, st);
+    if (e != hipSuccess) return e;
     double* s2 = reduce_partials(scratch, nb, nq, st);
     hipLaunchKernelGGL(moments_finish_kernel, dim3(1), dim3(64), 0, st, s2, nq, 2, g, mom_out);
+    return hipGetLastError();
+}
+
+long moments_seq_npad(long ncells) { return (ncells + SEQ_TILE - 1) / SEQ_TILE * SEQ_TILE; }
+
+long moments_seq_len(long ncells, int lmax) {
+    const long nq = mp_n_pass2(lmax) > 4 ? mp_n_pass2(lmax) : 4;
+    return nq * moments_seq_npad(ncells) + 2 * nq;
+}
+
+hipError_t launch_moments_sequential(const GridDims& g, const double* rho, int lmax,
+                                     double* terms, double* mom_out, hipStream_t st) {
+    const long n = (long)g.nx * g.ny * g.nz;
+    const long npad = moments_seq_npad(n);
+    const long nb = npad / 256;
+    const int nq = mp_n_pass2(lmax);
+    double* sums = terms + (long)(nq > 4 ? nq : 4) * npad;
+    hipLaunchKernelGGL(moments_pass1_kernel<true>, dim3((unsigned)nb), dim3(256), 0, st, g, rho,
+                       terms, npad);
+    hipLaunchKernelGGL(seq_sum_kernel, dim3(4), dim3(256), 0, st, terms, npad, sums);
+    hipLaunchKernelGGL(moments_finish_kernel, dim3(1), dim3(64), 0, st, sums, 4, 1, g, mom_out);
+    const hipError_t e = launch_pass2<true>(g, rho, lmax, mom_out, terms, npad, nb, st);
+    if (e != hipSuccess) return e;
+    hipLaunchKernelGGL(seq_sum_kernel, dim3((unsigned)nq), dim3(256), 0, st, terms, npad, sums);
+    hipLaunchKernelGGL(moments_finish_kernel, dim3(1), dim3(64), 0, st, sums, nq, 2, g, mom_out);
     return hipGetLastError();
 }
 

@@ -43,6 +43,8 @@ struct GravDevice {
     double *mom = nullptr;      // moment results
     double *mom_scratch = nullptr;
     long mom_scratch_len = 0;
+    double *mom_seq = nullptr;  // stored terms and sums of the sequential order
+    long mom_seq_len = 0;
     double *wlm = nullptr;      // multipole weights
     hyd::SorCtl *ctl = nullptr;
     hyd::SorCtl *ctl_host = nullptr;  // pinned readback

@@ -347,6 +347,7 @@ GodunovHLLC3D* solver3d_create_from_deck(const InputDeck* deck) {
             gravity_set_dc_inner(s->gravity_solver, deck->dc_inner) != 0)
             fprintf(stderr, "Warning: dc_inner ignored; the FP32 modes use SOR sweeps.\n");
         gravity_set_criterion(s->gravity_solver, deck->sor_criterion);
+        gravity_set_moment_order(s->gravity_solver, deck->moment_order);
         if (deck->nn_guess[0]) {
             void* m = nn_model_create(0);
             if (m && nn_model_load(m, deck->nn_guess) == 0) {

@@ -149,6 +149,7 @@ InputDeck* input_deck_create(const char* filename) {
     d->multipole_lmax = 2;
     d->poisson_mode = POISSON_FP64_SOR;
     d->dc_inner = DC_INNER_SOR;
+    d->moment_order = MOMENT_ORDER_TREE;
     d->output_enable = true;
 
     if (filename == NULL) {
@@ -193,6 +194,12 @@ InputDeck* input_deck_create(const char* filename) {
             d->dc_inner = (strcmp(v, "multigrid") == 0 || strcmp(v, "1") == 0)
                               ? DC_INNER_MULTIGRID
                               : DC_INNER_SOR;
+            continue;
+        }
+        if (strcmp(k, "moment_order") == 0) {
+            d->moment_order = (strcmp(v, "sequential") == 0 || strcmp(v, "1") == 0)
+                                  ? MOMENT_ORDER_SEQUENTIAL
+                                  : MOMENT_ORDER_TREE;
             continue;
         }
         if (strcmp(k, "sor_criterion") == 0) {
@@ -244,6 +251,8 @@ void input_deck_print(const InputDeck* d) {
                                                    : "fp64");
     if (d->poisson_mode != POISSON_FP64_SOR)
         printf("  dc_inner = %s\n", d->dc_inner == DC_INNER_MULTIGRID ? "multigrid" : "sor");
+    printf("  moment_order = %s\n",
+           d->moment_order == MOMENT_ORDER_SEQUENTIAL ? "sequential" : "tree");
     printf("\nOutput:\n  Directory: %s\n", d->output_dir);
     printf("==============================================\n\n");
 }

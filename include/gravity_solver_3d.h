@@ -70,6 +70,7 @@ typedef struct GravitySolver3D {
     int dc_inner;        /* DcInner from input_deck.h: inner solver of the FP32
                             modes; with DC_INNER_MULTIGRID, last_iterations
                             counts V-cycles                               */
+    int moment_order;    /* MomentOrder from input_deck.h                    */
 } GravitySolver3D;
 
 /* gravity_solver_3d.h:64-68 */

@@ -58,6 +58,14 @@ typedef enum {
                                hierarchy (even nx, ny, nz) */
 } DcInner;
 
+/* Order of the multipole moment sums, added by this build. */
+typedef enum {
+    MOMENT_ORDER_TREE = 0,       /* pairwise double-double tree (fast)      */
+    MOMENT_ORDER_SEQUENTIAL = 1  /* cell after cell as the reference
+                                    (gravity_solver_3d.c:182-198, 226-258):
+                                    the same bits as its loops            */
+} MomentOrder;
+
 /* SOR stopping rules added by this build. */
 typedef enum {
     SOR_CRIT_MAX_UPDATE = 0, /* max |phi_GS - phi_old| < tol (the reference) */
@@ -149,6 +157,7 @@ typedef struct {
     bool output_enable;       /* false skips CSV/VTK writing              */
     char ic_file[256];        /* binary initial state written by this build */
     int dc_inner;             /* DcInner of the FP32 modes, default SOR   */
+    int moment_order;         /* MomentOrder, default TREE                */
 } InputDeck;
 
 /* input_deck.h:127-131 */

@@ -8,9 +8,14 @@ bytes, per-field L1 norms and sums, and the same run with the reference's
 sequential moment sums (final-state L1 norms) for the relative-L1 bar.
 
     python tests/golden/make_c2_fixture.py [steps] [lmax]   (about an hour on one core)
+    python tests/golden/make_c2_fixture.py [steps] [lmax] sequential
 
 lmax 2 (the reference's monopole + quadrupole) writes c2_polytrope_128.npz,
-any other order c2_polytrope_128_l<lmax>.npz.
+any other order c2_polytrope_128_l<lmax>.npz.  The second form reruns only
+the sequential-sum run, checks it against the dt, iteration counts and L1
+norms the fixture already holds, and adds its final-state SHA-256 and
+mid-plane density (seq_sha256, seq_rho_slice), the bitwise target of the
+device's sequential moment order.
 """
 import hashlib
 import os
@@ -48,22 +53,38 @@ def run(mode, steps, log, lmax):
     return o.state(), np.array(dts), np.array(its)
 
 
+def sha(a):
+    return np.frombuffer(hashlib.sha256(np.ascontiguousarray(a).tobytes()).digest(),
+                         dtype=np.uint8)
+
+
 def main():
     steps = int(sys.argv[1]) if len(sys.argv) > 1 else 100
     lmax = int(sys.argv[2]) if len(sys.argv) > 2 else 2
+    only_seq = len(sys.argv) > 3 and sys.argv[3] == "sequential"
     log = lambda m: print(m, flush=True)
+    name = f"c2_polytrope_{N}.npz" if lmax == 2 else f"c2_polytrope_{N}_l{lmax}.npz"
+    if only_seq:
+        out = dict(np.load(os.path.join(HERE, name)))
+        assert int(out["steps"]) == steps
+        Us, dts_s, its_s = run(MOM_SEQUENTIAL, steps, log, lmax)
+        assert np.array_equal(dts_s, out["seq_dt"]) and np.array_equal(its_s, out["seq_iters"])
+        assert np.array_equal(np.abs(Us).reshape(-1, 5).sum(axis=0), out["seq_l1"])
+        out.update(seq_sha256=sha(Us), seq_rho_slice=np.ascontiguousarray(Us[N // 2, :, :, 0]))
+        np.savez_compressed(os.path.join(HERE, name), **out)
+        log(f"added the sequential run's digest to {name}")
+        return
     U, dts, its = run(MOM_DDTREE, steps, log, lmax)
     Us, dts_s, its_s = run(MOM_SEQUENTIAL, steps, log, lmax)
     out = dict(
         steps=steps, n=N, tol=TOL, omega=OMEGA, lmax=lmax, dt=dts, iters=its,
-        sha256=np.frombuffer(hashlib.sha256(np.ascontiguousarray(U).tobytes()).digest(),
-                             dtype=np.uint8),
+        sha256=sha(U),
         l1=np.abs(U).reshape(-1, 5).sum(axis=0), sums=U.reshape(-1, 5).sum(axis=0),
         seq_dt=dts_s, seq_iters=its_s, seq_l1=np.abs(Us).reshape(-1, 5).sum(axis=0),
         seq_diff_l1=np.abs(U - Us).reshape(-1, 5).sum(axis=0),
         rho_slice=np.ascontiguousarray(U[N // 2, :, :, 0]),
+        seq_sha256=sha(Us), seq_rho_slice=np.ascontiguousarray(Us[N // 2, :, :, 0]),
     )
-    name = f"c2_polytrope_{N}.npz" if lmax == 2 else f"c2_polytrope_{N}_l{lmax}.npz"
     np.savez_compressed(os.path.join(HERE, name), **out)
     log(f"wrote {name}; ddtree vs sequential relative L1 per field: "
         f"{out['seq_diff_l1'] / out['seq_l1']}")

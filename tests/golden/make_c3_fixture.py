@@ -8,7 +8,7 @@ the same run with sequential moment sums for the relative-L1 bar.
 Stored: SHA-256 of the initial state, per-step dt and SOR iteration
 counts, SHA-256 of the final state, per-field L1 norms and sums, a 32 x 32
 subsample of the final mid-plane density, and the sequential run's dt,
-iteration counts and final L1 distance.
+iteration counts, final-state SHA-256 and final L1 distance.
 
     python tests/golden/make_c3_fixture.py ddtree [steps]      (one run, hours on one core)
     python tests/golden/make_c3_fixture.py sequential [steps]
@@ -82,7 +82,8 @@ def main():
         ic_sha256=a["ic"], dt=a["dt"], iters=a["iters"], sha256=sha(U), l1=l1,
         sums=np.asarray(U).reshape(-1, 5).sum(axis=0),
         rho_sub=np.ascontiguousarray(U[N // 2, ::8, ::8, 0]),
-        seq_dt=b["dt"], seq_iters=b["iters"], seq_l1=np.abs(Us).reshape(-1, 5).sum(axis=0),
+        seq_dt=b["dt"], seq_iters=b["iters"], seq_sha256=sha(Us),
+        seq_l1=np.abs(Us).reshape(-1, 5).sum(axis=0),
         seq_diff_l1=np.abs(np.asarray(U) - np.asarray(Us)).reshape(-1, 5).sum(axis=0),
     )
     np.savez_compressed(os.path.join(HERE, f"c3_explosion_{N}.npz"), **out)

@@ -2,14 +2,16 @@
 1e-8 under the reference's max-update criterion) on the device, step by
 step against the oracle fixtures tests/golden/c2_polytrope_128.npz
 (multipole boundary to l = 2, the reference's) and c2_polytrope_128_l4.npz
-(to l = 4): dt and
-SOR iteration count of every step exactly, the final state bit for bit
-(SHA-256), and the distance to the run with the reference's sequential
-moment sums: relative L1 <= 1e-12 for density and energy, and for the
-momenta relative to the density's L1 (a star near hydrostatic balance has
-momenta that are the small remainder of pressure against gravity, L1 about
-0.2% of the density's, so their own relative L1 reads 6e-12 for the same
-last-bit moment difference)."""
+(to l = 4): dt and SOR iteration count of every step exactly and the final
+state bit for bit (SHA-256), in both moment orders.  With the default
+pairwise-tree moment sums the target is the oracle's tree-summed run; with
+MOMENT_ORDER_SEQUENTIAL (the reference's cell-after-cell sums) it is the
+oracle's sequential run.  The distance between the two runs is checked
+too: relative L1 <= 1e-12 for density and energy, and for the momenta
+relative to the density's L1 (a star near hydrostatic balance has momenta
+that are the small remainder of pressure against gravity, L1 about 0.2% of
+the density's, so their own relative L1 reads 6e-12 for the same last-bit
+moment difference)."""
 import hashlib
 import os
 
@@ -24,32 +26,36 @@ pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
+@pytest.mark.parametrize("order", ["tree", "sequential"])
 @pytest.mark.parametrize("name", ["c2_polytrope_128.npz", "c2_polytrope_128_l4.npz"])
-def test_polytrope_128_steps_bitwise(name):
+def test_polytrope_128_steps_bitwise(name, order):
     path = os.path.join(GOLDEN, name)
     if not os.path.exists(path):
         pytest.skip(f"fixture {name} not generated")
     f = np.load(path)
+    pre = "seq_" if order == "sequential" else ""
+    if pre + "sha256" not in f:
+        pytest.skip(f"fixture {name} holds no sequential-order digest")
     n, steps = int(f["n"]), int(f["steps"])
     lmax = int(f["lmax"]) if "lmax" in f else 2
-    print(f"polytrope 128^3, l = {lmax}, {steps} steps", flush=True)
+    print(f"polytrope 128^3, l = {lmax}, {steps} steps, {order} moment order", flush=True)
     U0 = ics.polytrope_n1(n)
     s = H.Solver(n, n, n, G=1.0, gamma=2.0, cfl=0.4, bc=(H.BC_OUTFLOW,) * 6)
     s.gravity.set_lmax(lmax)
     s.gravity.set_params(float(f["omega"]), 200000, float(f["tol"]))
+    if order == "sequential":
+        s.gravity.set_moment_order(H.MOMENT_ORDER_SEQUENTIAL)
     s.set_state(U0)
     for k in range(steps):
         assert s.step() == 0
         it, _ = s.gravity.diagnostics()
-        assert it == f["iters"][k], f"step {k + 1}"
-        assert s.dt == f["dt"][k], f"step {k + 1}"
+        assert it == f[pre + "iters"][k], f"step {k + 1}"
+        assert s.dt == f[pre + "dt"][k], f"step {k + 1}"
     U = s.state()
     digest = np.frombuffer(hashlib.sha256(np.ascontiguousarray(U).tobytes()).digest(),
                            dtype=np.uint8)
-    assert np.array_equal(digest, f["sha256"])
-    assert np.array_equal(U[n // 2, :, :, 0], f["rho_slice"])
-    # the device equals the tree-summed run bit for bit, so its distance to
-    # the run with the reference's sequential moment sums is the fixture's
+    assert np.array_equal(digest, f[pre + "sha256"])
+    assert np.array_equal(U[n // 2, :, :, 0], f[pre + "rho_slice"])
     d, l1 = f["seq_diff_l1"], f["seq_l1"]
     assert d[0] <= 1e-12 * l1[0] and d[4] <= 1e-12 * l1[4]
     assert np.all(d[1:4] <= 1e-12 * l1[0])
