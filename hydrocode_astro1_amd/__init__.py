@@ -117,6 +117,7 @@ EXPORTS = [
     ("solver3d_last_timing", C.c_int, [_SP, _dp, _dp]),
     ("solver3d_benchmark_steps", C.c_int, [_SP, C.c_int, _dp, _dp, _dp]),
     ("hydro_check_division", C.c_long, [_dp, _dp, C.c_long]),
+    ("hydro_sequential_sum", C.c_double, [_dp, C.c_long, C.c_int, C.POINTER(C.c_longlong), _ip]),
     ("gravity_set_multipole_lmax", C.c_int, [_GP, C.c_int]),
     ("gravity_set_poisson_mode", C.c_int, [_GP, C.c_int]),
     ("gravity_set_dc_inner", C.c_int, [_GP, C.c_int]),
@@ -174,6 +175,24 @@ def lib():
 def _d(a):
     assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
     return a.ctypes.data_as(_dp)
+
+
+SEQSUM_STATS = ("blocks16384", "blocks1024", "segments", "chunks", "chunks_added",
+                "blocks1024_opened", "cycles", "cycles_adding", "cycles_opened")
+
+
+def sequential_sum(x, method=1, stats=False):
+    """((0 + x[0]) + x[1]) + ... rounded after every addition, computed on
+    the device: method 0 adds the terms in one chain, method 1 moves by
+    block increments checked exactly (the moment sums of
+    MOMENT_ORDER_SEQUENTIAL use method 1).  stats=True also returns method
+    1's walk counters as a dict."""
+    x = np.ascontiguousarray(x, dtype=np.float64)
+    st = C.c_int(-1)
+    cnt = (C.c_longlong * 9)()
+    r = lib().hydro_sequential_sum(_d(x), x.size, method, cnt if stats else None, C.byref(st))
+    _check(st.value, "hydro_sequential_sum")
+    return (r, dict(zip(SEQSUM_STATS, list(cnt)))) if stats else r
 
 
 def _check(rc, what):

@@ -57,8 +57,8 @@ int moments_dev(GravitySolver3D* s, const double* d_rho) {
         if (dev_alloc(&d->mom_seq, len)) return -2;
         d->mom_seq_len = len;
     }
-    HYD_CHECK(hyd::launch_moments_sequential(d->g, d_rho, s->multipole_lmax, d->mom_seq, d->mom,
-                                             d->stream));
+    HYD_CHECK(hyd::launch_moments_sequential(d->g, d_rho, s->multipole_lmax, d->seq_method,
+                                             d->mom_seq, d->mom, d->stream));
     return 0;
 }
 
@@ -97,6 +97,8 @@ int dev_create(GravitySolver3D* s) {
     if (!d->stream) return -2;
     if (const char* e = getenv("HYDRO_MI355X_SOR_FUSED")) d->fused = atoi(e) != 0;
     if (const char* e = getenv("HYDRO_MI355X_MG_TAIL")) d->mg_use_tail = atoi(e) != 0;
+    if (const char* e = getenv("HYDRO_MI355X_SEQSUM"))
+        d->seq_method = strcmp(e, "chain") == 0 ? hyd::SEQSUM_CHAIN : hyd::SEQSUM_COMPOSITE;
     if (const char* e = getenv("HYDRO_MI355X_MG_SMOOTH")) {
         int a = 0, b = 0;
         float w = 0.0f;

@@ -7,6 +7,7 @@
 // the reference's sequential sweep (gravity_solver_3d.c:642-708) bit for bit.
 #include "device_common.hpp"
 #include "gravity_kernels.hpp"
+#include "seqsum.hpp"
 
 namespace hyd {
 
@@ -488,53 +489,6 @@ __global__ void __launch_bounds__(256) moments_pass2_kernel(GridDims g,
     emit_terms<NQ>(STORE, term, t, npad, part, s_term, s_part);
 }
 
-// The reference's order: s = 0; s = s + x[0]; s = s + x[1]; ... over the
-// npad stored terms of quantity blockIdx.x (gravity_solver_3d.c:182-198,
-// 226-258 add cell by cell, i outermost, k innermost; skipped cells and the
-// padding hold zeros, and adding a zero leaves a sum that started at +0
-// unchanged).  One wavefront adds, the other three stage the next tile in
-// LDS; the add chain is the kernel's whole cost.
-constexpr int SEQ_TILE = 2048;
-
-__global__ void __launch_bounds__(256) seq_sum_kernel(const double* __restrict__ terms, long npad,
-                                                      double* __restrict__ sums) {
-    __shared__ double buf[2][SEQ_TILE];
-    const double* x = terms + (long)blockIdx.x * npad;
-    const long ntiles = npad / SEQ_TILE;
-    const int tid = threadIdx.x;
-    for (int e = tid; e < SEQ_TILE; e += 256) buf[0][e] = x[e];
-    __syncthreads();
-    double s = 0.0;
-    for (long t = 0; t < ntiles; ++t) {
-        if (tid >= 64) {
-            if (t + 1 < ntiles) {
-                const double2* src = reinterpret_cast<const double2*>(x + (t + 1) * SEQ_TILE);
-                double2* dst = reinterpret_cast<double2*>(buf[(t + 1) & 1]);
-                double2 v[6];
-#pragma unroll
-                for (int u = 0; u < 6; ++u) {
-                    const int e = tid - 64 + 192 * u;
-                    if (e < SEQ_TILE / 2) v[u] = src[e];
-                }
-#pragma unroll
-                for (int u = 0; u < 6; ++u) {
-                    const int e = tid - 64 + 192 * u;
-                    if (e < SEQ_TILE / 2) dst[e] = v[u];
-                }
-            }
-        } else {
-            const double* b = buf[t & 1];
-#pragma unroll 32
-            for (int e = 0; e < SEQ_TILE; ++e) s = s + b[e];
-        }
-        __syncthreads();
-    }
-    if (tid == 0) {
-        sums[2 * blockIdx.x] = s;
-        sums[2 * blockIdx.x + 1] = 0.0;
-    }
-}
-
 // Fold 256 consecutive partials of quantity q (zero padded) pairwise.
 __global__ void __launch_bounds__(256) moments_reduce_kernel(const double* __restrict__ in,
                                                              long nin, double* __restrict__ out,
@@ -827,27 +781,34 @@ hipError_t launch_moments(const GridDims& g, const double* rho, int lmax, double
     return hipGetLastError();
 }
 
-long moments_seq_npad(long ncells) { return (ncells + SEQ_TILE - 1) / SEQ_TILE * SEQ_TILE; }
+long moments_seq_npad(long ncells) { return seqsum_npad(ncells); }
+
+static int seq_nq(int lmax) { return mp_n_pass2(lmax) > 4 ? mp_n_pass2(lmax) : 4; }
 
 long moments_seq_len(long ncells, int lmax) {
-    const long nq = mp_n_pass2(lmax) > 4 ? mp_n_pass2(lmax) : 4;
-    return nq * moments_seq_npad(ncells) + 2 * nq;
+    const long npad = seqsum_npad(ncells);
+    const int nq = seq_nq(lmax);
+    return nq * npad + seqsum_workspace_len(npad, nq) + 2L * nq;
 }
 
-hipError_t launch_moments_sequential(const GridDims& g, const double* rho, int lmax,
-                                     double* terms, double* mom_out, hipStream_t st) {
+hipError_t launch_moments_sequential(const GridDims& g, const double* rho, int lmax, int method,
+                                     double* buf, double* mom_out, hipStream_t st) {
     const long n = (long)g.nx * g.ny * g.nz;
-    const long npad = moments_seq_npad(n);
+    const long npad = seqsum_npad(n);
     const long nb = npad / 256;
     const int nq = mp_n_pass2(lmax);
-    double* sums = terms + (long)(nq > 4 ? nq : 4) * npad;
+    double* terms = buf;
+    double* ws = terms + (long)seq_nq(lmax) * npad;
+    double* sums = ws + seqsum_workspace_len(npad, seq_nq(lmax));
     hipLaunchKernelGGL(moments_pass1_kernel<true>, dim3((unsigned)nb), dim3(256), 0, st, g, rho,
                        terms, npad);
-    hipLaunchKernelGGL(seq_sum_kernel, dim3(4), dim3(256), 0, st, terms, npad, sums);
-    hipLaunchKernelGGL(moments_finish_kernel, dim3(1), dim3(64), 0, st, sums, 4, 1, g, mom_out);
-    const hipError_t e = launch_pass2<true>(g, rho, lmax, mom_out, terms, npad, nb, st);
+    hipError_t e = launch_seqsum(method, terms, npad, 4, ws, sums, st);
     if (e != hipSuccess) return e;
-    hipLaunchKernelGGL(seq_sum_kernel, dim3((unsigned)nq), dim3(256), 0, st, terms, npad, sums);
+    hipLaunchKernelGGL(moments_finish_kernel, dim3(1), dim3(64), 0, st, sums, 4, 1, g, mom_out);
+    e = launch_pass2<true>(g, rho, lmax, mom_out, terms, npad, nb, st);
+    if (e != hipSuccess) return e;
+    e = launch_seqsum(method, terms, npad, nq, ws, sums, st);
+    if (e != hipSuccess) return e;
     hipLaunchKernelGGL(moments_finish_kernel, dim3(1), dim3(64), 0, st, sums, nq, 2, g, mom_out);
     return hipGetLastError();
 }

@@ -89,12 +89,13 @@ long moments_scratch_len(long ncells, int lmax);
 hipError_t launch_moments(const GridDims& g, const double* rho, int lmax, double* scratch,
                           double* mom_out, hipStream_t st);
 // The same sums in the reference's order (one cell after another): every
-// term is stored (max(4, nq) x npad doubles, then 2 nq sums) and added in
-// sequence.  moments_seq_len gives the buffer length in doubles.
+// term is stored (max(4, nq) x npad doubles), then summed by launch_seqsum
+// with `method` (SeqSumMethod).  moments_seq_len gives the buffer length in
+// doubles.
 long moments_seq_npad(long ncells);
 long moments_seq_len(long ncells, int lmax);
-hipError_t launch_moments_sequential(const GridDims& g, const double* rho, int lmax,
-                                     double* terms, double* mom_out, hipStream_t st);
+hipError_t launch_moments_sequential(const GridDims& g, const double* rho, int lmax, int method,
+                                     double* buf, double* mom_out, hipStream_t st);
 
 // ---- potential layout ----------------------------------------------------
 hipError_t launch_natural_to_split(const GridDims& g, const double* nat, double scale,

@@ -647,6 +647,34 @@ long hydro_check_division(const double* a, const double* b, long n) {
     return rc;
 }
 
+double hydro_sequential_sum(const double* x, long n, int method, long long* stats, int* status) {
+    hipStream_t st = hyd_stream();
+    *status = -2;
+    if (!st || n < 0) return 0.0;
+    const long npad = hyd::seqsum_npad(n > 0 ? n : 1);
+    const long len = npad + hyd::seqsum_workspace_len(npad, 1) + 2 + 9;
+    double* buf = nullptr;
+    double out = 0.0;
+    long long* dstats = nullptr;
+    if (hipMalloc((void**)&buf, sizeof(double) * len) == hipSuccess) {
+        dstats = stats ? reinterpret_cast<long long*>(buf + len - 9) : nullptr;
+        if (hipMemsetAsync(buf, 0, sizeof(double) * npad, st) == hipSuccess &&
+            (n == 0 || hipMemcpyAsync(buf, x, sizeof(double) * n, hipMemcpyHostToDevice, st) ==
+                           hipSuccess) &&
+            hyd::launch_seqsum(method, buf, npad, 1, buf + npad, buf + len - 11, st, dstats) ==
+                hipSuccess &&
+            hipMemcpyAsync(&out, buf + len - 11, sizeof out, hipMemcpyDeviceToHost, st) ==
+                hipSuccess &&
+            (!stats || method != hyd::SEQSUM_COMPOSITE ||
+             hipMemcpyAsync(stats, dstats, 9 * sizeof(long long), hipMemcpyDeviceToHost, st) ==
+                 hipSuccess) &&
+            hipStreamSynchronize(st) == hipSuccess)
+            *status = 0;
+    }
+    if (buf) (void)hipFree(buf);
+    return out;
+}
+
 int solver3d_last_timing(const GodunovHLLC3D* s, double* hydro_ms, double* gravity_ms) {
     HydroDevice* d = s->dev;
     float ms = 0.0f;

@@ -65,6 +65,17 @@ int solver3d_last_timing(const GodunovHLLC3D *solver, double *hydro_ms, double *
  * bit from the IEEE division on the device (0 expected), < 0 on error. */
 long hydro_check_division(const double *a, const double *b, long n);
 
+/* Diagnostic: the sum ((0 + x[0]) + x[1]) + ... of n doubles (host array)
+ * with every addition rounded in turn, as the reference's moment loops add
+ * (gravity_solver_3d.c:182-198), computed on the device by method 0 (one
+ * chain of additions) or 1 (block increments with exact checks, falling back
+ * to the chain where a block leaves its binade).  stats (9 values, or NULL)
+ * receives method 1's walk counters: blocks of 16384 and 1024, segments and
+ * chunks applied, chunks added term by term, 1024-blocks opened, and clock
+ * cycles in all, in term-by-term adds and in opened 1024-blocks.
+ * *status = 0 on success. */
+double hydro_sequential_sum(const double *x, long n, int method, long long *stats, int *status);
+
 /* ---- gravity settings (no reference counterpart) ----------------------- */
 
 /* Multipole order of the isolated boundary: 2 = monopole + quadrupole as

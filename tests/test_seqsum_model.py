@@ -1,0 +1,59 @@
+"""The numpy statement of the block-composite sequential sum
+(tests/seqsum_model.py) equals a plain in-order loop bit for bit on
+sequences built to break it, and on the moment terms of a star-like
+density, where it must also skip most of the terms (a method that fell back
+to the chain everywhere would be correct but pointless)."""
+import numpy as np
+import pytest
+
+import seqsum_model as M
+from hydrocode_astro1_amd import ics
+
+
+def same(a, b):
+    return np.float64(a).view(np.int64) == np.float64(b).view(np.int64)
+
+
+@pytest.mark.parametrize("name", list(M.hard_cases().keys()))
+def test_hard_cases_bitwise(name):
+    x = M.hard_cases()[name]
+    st = {}
+    got = M.sequential_sum(x, st)
+    assert same(got, M.plain_sum(x)), (got, M.plain_sum(x), st)
+
+
+def test_moment_terms_of_a_star():
+    n = 64
+    U = ics.explosion_n3(n)
+    rho = U[..., 0].astype(np.float64)
+    X = (np.arange(n) + 0.5) / n
+    x, y, z = (a.ravel() for a in np.meshgrid(X, X, X, indexing="ij"))
+    dm = rho.ravel() / n ** 3
+    M0 = M.plain_sum(dm)
+    xc, yc, zc = (M.plain_sum(dm * c) / M0 for c in (x, y, z))
+    X_, Y_, Z_ = x - xc, y - yc, z - zc
+    r2 = X_ * X_ + Y_ * Y_ + Z_ * Z_
+    for name, t in {"M": dm, "Mx": dm * x, "Qxx": dm * (3.0 * X_ * X_ - r2),
+                    "Qxz": dm * (3.0 * X_ * Z_)}.items():
+        st = {}
+        assert same(M.sequential_sum(t, st), M.plain_sum(t)), name
+        if name in ("M", "Mx"):  # monotone sums: nearly everything by blocks
+            assert st["chain"] * M.CHUNK < 0.1 * t.size, (name, st)
+
+
+@pytest.mark.parametrize("name", ["positive uniform", "mixed normal", "halves (ties)",
+                                  "ragged length"])
+def test_a_wrong_guess_costs_time_not_bits(name):
+    """Binade guesses made wrong on purpose (each approximate prefix scaled
+    by a random factor in [1/4, 4], a third of them negated) never apply a
+    block wrongly: the walk falls back and still lands on the plain sum."""
+    rng = np.random.default_rng(11)
+    x = M.hard_cases()[name]
+
+    def noise(m):
+        f = np.exp2(rng.uniform(-2.0, 2.0, m))
+        return np.where(rng.random(m) < 1 / 3, -f, f)
+
+    st = {}
+    assert same(M.sequential_sum(x, st, guess_noise=noise), M.plain_sum(x)), st
+    assert st["chain"] > 0
