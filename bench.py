@@ -205,19 +205,23 @@ def poisson_block(H, ics, n):
     return out
 
 
-def selfgravity_block(H, ics, n, steps=10):
+def selfgravity_block(H, ics, n, steps=10, arms=None):
     """The full step with self-gravity on the same 256^3 state, l = 6
     multipole boundary.  "reference_sor": the reference's Poisson settings
     (red-black SOR, omega 1.5, max |update| < 1e-6, at most 10000
-    iterations).  "nn_fp32_dc_multigrid": relative residual 1e-8, the learned
-    guess (kept only when its residual beats the warm start) and FP32 defect
-    correction with multigrid inner solves.  The first step solves cold, the
-    next `steps` warm-start from the previous potential."""
+    iterations), moments summed by the pairwise tree.
+    "reference_sor_sequential_moments": the same with the moment terms added
+    one after another as the reference adds them (its bits; fewer steps,
+    each costs tens of milliseconds).  "nn_fp32_dc_multigrid": relative
+    residual 1e-8, the learned guess (kept only when its residual beats the
+    warm start) and FP32 defect correction with multigrid inner solves.  The
+    first step solves cold, the next `steps` warm-start from the previous
+    potential.  `arms` picks a subset by name."""
     from hydrocode_astro1_amd import nn
 
     U = ics.explosion_n3(n)
 
-    def run(setup):
+    def run(setup, steps=steps):
         s = H.Solver(n, n, n, G=1.0, gamma=5.0 / 3.0, cfl=0.4, bc=(H.BC_OUTFLOW,) * 6)
         gr = s.gravity
         gr.set_lmax(6)
@@ -243,6 +247,12 @@ def selfgravity_block(H, ics, n, steps=10):
         gr.set_params(1.5, 10000, 1e-6)
         return {"sor": {"omega": 1.5, "tol_max_update": 1e-6, "max_iter": 10000}}
 
+    def reference_sequential(gr):
+        info = reference(gr)
+        gr.set_moment_order(H.MOMENT_ORDER_SEQUENTIAL)
+        info["moment_order"] = "sequential"
+        return info
+
     def learned_multigrid(gr):
         gr.set_params(1.5, 1000, 1e-8)
         gr.set_criterion(H.SOR_CRIT_REL_RESID)
@@ -252,8 +262,14 @@ def selfgravity_block(H, ics, n, steps=10):
             gr.set_nn_model(nn.Model(path=nn.DEFAULT_WEIGHTS))
         return {"tol_rel_residual": 1e-8, "iterations_are": "V-cycles"}
 
-    return {"grid": n, "multipole_lmax": 6, "warm_steps": steps,
-            "reference_sor": run(reference), "nn_fp32_dc_multigrid": run(learned_multigrid)}
+    table = {"reference_sor": (reference, steps),
+             "reference_sor_sequential_moments": (reference_sequential, min(steps, 3)),
+             "nn_fp32_dc_multigrid": (learned_multigrid, steps)}
+    out = {"grid": n, "multipole_lmax": 6, "warm_steps": steps}
+    for name, (setup, k) in table.items():
+        if arms is None or name in arms:
+            out[name] = run(setup, k)
+    return out
 
 
 def snapshot_study(H, ics, n, steps=200, every=20, big=(0, 100, 200)):
@@ -362,6 +378,9 @@ def main():
                          "prolongations at steps 0, STEPS/2 and STEPS), then exit")
     ap.add_argument("--selfgravity-probe", type=int, default=0, metavar="STEPS",
                     help="only run the self-gravity step block with STEPS warm steps, then exit")
+    ap.add_argument("--selfgravity-arms", default="", metavar="NAMES",
+                    help="comma-separated subset of the self-gravity arms (reference_sor, "
+                         "reference_sor_sequential_moments, nn_fp32_dc_multigrid)")
     ap.add_argument("--calibrate", action="store_true",
                     help="only run the device layout-conversion copy (40 B read + 40 B written "
                          "per cell) used to scale HBM byte counters, then exit")
@@ -397,7 +416,8 @@ def run(args):
         sys.exit(1)
     n = args.n
     if args.selfgravity_probe > 0:
-        return {"selfgravity_step": selfgravity_block(H, ics, n, args.selfgravity_probe)}
+        arms = args.selfgravity_arms.split(",") if args.selfgravity_arms else None
+        return {"selfgravity_step": selfgravity_block(H, ics, n, args.selfgravity_probe, arms)}
     if args.snapshot_study > 0:
         k = args.snapshot_study
         return {"snapshot_study": snapshot_study(H, ics, n, steps=k, every=max(1, k // 10),

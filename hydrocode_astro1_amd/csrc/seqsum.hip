@@ -161,9 +161,7 @@ __device__ __forceinline__ void load_chunk(const double* __restrict__ p, double 
 
 // Approximate sum of each 1024-term block (wavefront per block).
 __global__ void __launch_bounds__(256) seq_blocksum_kernel(const double* __restrict__ terms,
-                                                           long npad, double* __restrict__ bs,
-                                                           int* __restrict__ nbad) {
-    if (blockIdx.x == 0 && threadIdx.x == 0) nbad[blockIdx.y] = 0;
+                                                           long npad, double* __restrict__ bs) {
     const long n1 = npad / SEQ_BLOCK1;
     const long b1 = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
@@ -209,8 +207,7 @@ __global__ void __launch_bounds__(1024) seq_scan_kernel(const double* __restrict
 __global__ void __launch_bounds__(256) seq_prep_kernel(const double* __restrict__ terms, long npad,
                                                        const double* __restrict__ pre,
                                                        SeqDesc* __restrict__ dseg,
-                                                       SeqDesc* __restrict__ d1,
-                                                       int* __restrict__ nbad) {
+                                                       SeqDesc* __restrict__ d1) {
     const long n1 = npad / SEQ_BLOCK1;
     const long b1 = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
@@ -339,7 +336,6 @@ __global__ void __launch_bounds__(256) seq_prep_kernel(const double* __restrict_
         }
         b.flags &= SEQ_VALID | SEQ_NEG | SEQ_ZERO;
         d1[(long)q * n1 + b1] = b;
-        if (!(b.flags & (SEQ_VALID | SEQ_ZERO))) atomicAdd(&nbad[q], 1);
     }
 }
 
@@ -502,14 +498,25 @@ __global__ void __launch_bounds__(256) seq_walk_kernel(const double* __restrict_
                                                        const SeqDesc* __restrict__ dseg,
                                                        const SeqDesc* __restrict__ d1,
                                                        const SeqDesc* __restrict__ d2,
-                                                       const int* __restrict__ nbad,
                                                        double* __restrict__ sums,
                                                        long long* __restrict__ stats) {
     __shared__ double buf[2][SEQ_CHAIN_TILE];
+    __shared__ int s_bad[4];
     double* blk = buf[0];
     const int q = blockIdx.x;
+    // how many of the sequence's 1024-blocks are not of one binade
+    {
+        const long nb1 = npad / SEQ_BLOCK1;
+        int bad = 0;
+        for (long b = threadIdx.x; b < nb1; b += 256)
+            bad += (d1[(long)q * nb1 + b].flags & (SEQ_VALID | SEQ_ZERO)) == 0;
+#pragma unroll
+        for (int o = 32; o > 0; o >>= 1) bad += __shfl_xor(bad, o, 64);
+        if ((threadIdx.x & 63) == 0) s_bad[threadIdx.x >> 6] = bad;
+        __syncthreads();
+    }
     const long n1_ = npad / SEQ_BLOCK1;
-    if (2L * nbad[q] >= n1_) {
+    if (2L * (s_bad[0] + s_bad[1] + s_bad[2] + s_bad[3]) >= n1_) {
         // at least half the 1024-blocks leave their binade: the walk would
         // add most terms one by one at a slower pace than the plain chain
         const long long t0 = stats ? clock64() : 0;
@@ -599,7 +606,7 @@ long seqsum_npad(long n) { return (n + SEQ_BLOCK2 - 1) / SEQ_BLOCK2 * SEQ_BLOCK2
 long seqsum_workspace_len(long npad, int nq) {
     const long n0 = npad / SEQ_CHUNK, n1 = npad / SEQ_BLOCK1, n2 = npad / SEQ_BLOCK2;
     const long dsz = (long)(sizeof(SeqDesc) / sizeof(double));
-    return (long)nq * (2 * n1 + dsz * (n0 + n1 + n2) + 1);
+    return (long)nq * (2 * n1 + dsz * (n0 + n1 + n2));
 }
 
 hipError_t launch_seqsum(int method, const double* terms, long npad, int nq, double* ws,
@@ -617,15 +624,14 @@ hipError_t launch_seqsum(int method, const double* terms, long npad, int nq, dou
     SeqDesc* dseg = reinterpret_cast<SeqDesc*>(pre + (long)nq * n1);
     SeqDesc* d1 = dseg + (long)nq * n0;
     SeqDesc* d2 = d1 + (long)nq * n1;
-    int* nbad = reinterpret_cast<int*>(d2 + (long)nq * n2);
     const dim3 g1((unsigned)((n1 + 3) / 4), (unsigned)nq);
-    hipLaunchKernelGGL(seq_blocksum_kernel, g1, dim3(256), 0, st, terms, npad, bs, nbad);
+    hipLaunchKernelGGL(seq_blocksum_kernel, g1, dim3(256), 0, st, terms, npad, bs);
     hipLaunchKernelGGL(seq_scan_kernel, dim3((unsigned)nq), dim3(1024), 0, st, bs, n1, pre);
-    hipLaunchKernelGGL(seq_prep_kernel, g1, dim3(256), 0, st, terms, npad, pre, dseg, d1, nbad);
+    hipLaunchKernelGGL(seq_prep_kernel, g1, dim3(256), 0, st, terms, npad, pre, dseg, d1);
     hipLaunchKernelGGL(seq_level2_kernel, dim3((unsigned)((n2 * nq + 255) / 256)), dim3(256), 0,
                        st, d1, n2, nq, d2);
     hipLaunchKernelGGL(seq_walk_kernel, dim3((unsigned)nq), dim3(256), 0, st, terms, npad, dseg, d1,
-                       d2, nbad, sums, stats);
+                       d2, sums, stats);
     return hipGetLastError();
 }
 
