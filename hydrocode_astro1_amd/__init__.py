@@ -29,6 +29,9 @@ POISSON_FP64_SOR, POISSON_FP32_DC, POISSON_NN_FP32_DC = 0, 1, 2
 SOR_CRIT_MAX_UPDATE, SOR_CRIT_REL_RESID = 0, 1
 DC_INNER_SOR, DC_INNER_MULTIGRID = 0, 1
 MOMENT_ORDER_TREE, MOMENT_ORDER_SEQUENTIAL = 0, 1
+# ProblemType (input_deck.h): the values the reference has generators for
+(KELVIN_HELMHOLTZ_3D, BLAST_WAVE_3D, STELLAR_PROFILE_3D, BINARY_STAR_3D,
+ STELLAR_OCTANT_EXPLOSION_3D) = 11, 12, 15, 16, 17
 
 _dp = C.POINTER(C.c_double)
 _ip = C.POINTER(C.c_int)
@@ -118,6 +121,8 @@ EXPORTS = [
     ("solver3d_benchmark_steps", C.c_int, [_SP, C.c_int, _dp, _dp, _dp]),
     ("hydro_check_division", C.c_long, [_dp, _dp, C.c_long]),
     ("hydro_sequential_sum", C.c_double, [_dp, C.c_long, C.c_int, C.POINTER(C.c_longlong), _ip]),
+    ("hydro_fill_initial_state", C.c_int, [C.c_int, C.c_int, C.c_int] + [C.c_double] * 8
+     + [C.c_int, C.c_void_p, _dp]),
     ("gravity_set_multipole_lmax", C.c_int, [_GP, C.c_int]),
     ("gravity_set_poisson_mode", C.c_int, [_GP, C.c_int]),
     ("gravity_set_dc_inner", C.c_int, [_GP, C.c_int]),
@@ -175,6 +180,28 @@ def lib():
 def _d(a):
     assert a.dtype == np.float64 and a.flags["C_CONTIGUOUS"]
     return a.ctypes.data_as(_dp)
+
+
+def initial_state(problem, shape, box=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0), gamma=5.0 / 3.0, G=1.0,
+                  deck=None):
+    """The reference's initial-condition problem as an (nx, ny, nz, 5) array
+    of conserved variables, made on the host by hydro_fill_initial_state:
+    `problem` alone as set_initial_conditions_3d, or `deck` (a deck file
+    path, whose problem and parameters are used) as
+    set_initial_conditions_3d_from_deck.  None when the problem has no
+    generator."""
+    nx, ny, nz = shape
+    U = np.zeros((nx, ny, nz, 5))
+    L = lib()
+    dp = L.input_deck_create(deck.encode()) if deck else None
+    try:
+        rc = L.hydro_fill_initial_state(nx, ny, nz, *box, gamma, G, int(problem), dp, _d(U))
+    finally:
+        if dp:
+            L.input_deck_destroy(dp)
+    if rc < 0:
+        raise ValueError("hydro_fill_initial_state: bad arguments")
+    return U if rc == 0 else None
 
 
 SEQSUM_STATS = ("blocks16384", "blocks1024", "segments", "chunks", "chunks_added",

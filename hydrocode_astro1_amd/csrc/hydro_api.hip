@@ -365,24 +365,32 @@ GodunovHLLC3D* solver3d_create_from_deck(const InputDeck* deck) {
     return s;
 }
 
+// The reference's problem generators run on the host (problems.cpp); the
+// state reaches the device at the next upload.
 void set_initial_conditions_3d(GodunovHLLC3D* s, ProblemType problem) {
-    // The reference's problem generators (hydro_solver_3d.c:1383-1515) are
-    // host-side set-up outside this build; the state is left zero, as the
-    // reference leaves it for problem types it has no generator for.
-    fprintf(stderr,
-            "set_initial_conditions_3d: problem type %d has no generator in this build; "
-            "use ic_file or solver3d_set_state.\n",
-            (int)problem);
-    memset(s->U, 0, sizeof(State3D) * (size_t)s->nx * s->ny * s->nz);
     s->problem = problem;
     s->dev->uploaded = false;
+    if (hydro_fill_initial_state(s->nx, s->ny, s->nz, s->xmin, s->xmax, s->ymin, s->ymax, s->zmin,
+                                 s->zmax, s->gamma, s->G_const, (int)problem, NULL, s->U) != 0)
+        fprintf(stderr,
+                "set_initial_conditions_3d: problem type %d has no generator (as in the "
+                "reference); the state is left as it was.\n",
+                (int)problem);
 }
 
 void set_initial_conditions_3d_from_deck(GodunovHLLC3D* s, const InputDeck* deck) {
     if (deck->ic_file[0]) {
         if (load_ic_file(s, deck->ic_file) == 0) return;
     }
-    set_initial_conditions_3d(s, deck->problem);
+    s->problem = deck->problem;
+    s->dev->uploaded = false;
+    if (hydro_fill_initial_state(s->nx, s->ny, s->nz, s->xmin, s->xmax, s->ymin, s->ymax, s->zmin,
+                                 s->zmax, s->gamma, s->G_const, (int)deck->problem, deck,
+                                 s->U) != 0)
+        fprintf(stderr,
+                "set_initial_conditions_3d_from_deck: problem type %d has no generator (as in "
+                "the reference); the state is left as it was.\n",
+                (int)deck->problem);
 }
 
 void run_simulation_3d(GodunovHLLC3D* s, double t_end, double output_dt) {

@@ -170,6 +170,27 @@ def blobs(n=128, seed=0, turb_sigma=0.3, floor=1e-3):
     return rho
 
 
+def write_profile_file(path, nidx=1.5, R=1.0, rho_c=1.0, G=1.0, gamma=5.0 / 3.0, points=1000):
+    """A polytropic star as the radial profile file the reference's stellar
+    set-ups read (hydro_solver_3d.c:825-878): '#' header lines, then
+    "r rho u" per line, u the specific internal energy p / ((gamma-1) rho),
+    from the centre to the surface (Lane-Emden of index nidx, radius R)."""
+    xs, th, xi1, _ = lane_emden(nidx)
+    alpha = R / xi1
+    K = 4.0 * np.pi * G * alpha * alpha * rho_c ** (1.0 - 1.0 / nidx) / (nidx + 1.0)
+    pick = np.unique(np.linspace(0, len(xs) - 1, points).astype(int))
+    r = xs[pick] * alpha
+    rho = rho_c * np.maximum(th[pick], 0.0) ** nidx
+    p = K * rho ** (1.0 + 1.0 / nidx)
+    u = np.where(rho > 0.0, p / ((gamma - 1.0) * np.where(rho > 0.0, rho, 1.0)), 0.0)
+    with open(path, "w") as f:
+        f.write(f"# polytrope n = {nidx}, R = {R}, rho_c = {rho_c}, G = {G}, gamma = {gamma}\n")
+        f.write("# Columns: radius, density, internal_energy\n")
+        for a, b, c in zip(r, rho, u):
+            f.write(f"{a:.10e}  {b:.10e}  {c:.10e}\n")
+    return path
+
+
 def write_ic_file(path, U):
     """Binary initial state read by set_initial_conditions_3d_from_deck when
     the deck names it as ic_file: 8-byte magic, int32 nx ny nz, then the

@@ -65,6 +65,17 @@ int solver3d_last_timing(const GodunovHLLC3D *solver, double *hydro_ms, double *
  * bit from the IEEE division on the device (0 expected), < 0 on error. */
 long hydro_check_division(const double *a, const double *b, long n);
 
+/* The reference's initial-condition problems into a host state array U
+ * (nx*ny*nz cells, the reference's layout) without a solver or a device:
+ * with deck == NULL as set_initial_conditions_3d(problem)
+ * (hydro_solver_3d.c:1383-1515), otherwise as
+ * set_initial_conditions_3d_from_deck(deck) (hydro_solver_3d.c:2092-2133).
+ * Profile files are read from the paths the reference reads.  Returns 0, 1
+ * for a problem type with no generator (U untouched), -1 on bad arguments. */
+int hydro_fill_initial_state(int nx, int ny, int nz, double xmin, double xmax, double ymin,
+                             double ymax, double zmin, double zmax, double gamma, double G,
+                             int problem, const InputDeck *deck, State3D *U);
+
 /* Diagnostic: the sum ((0 + x[0]) + x[1]) + ... of n doubles (host array)
  * with every addition rounded in turn, as the reference's moment loops add
  * (gravity_solver_3d.c:182-198), computed on the device by method 0 (one

@@ -108,3 +108,28 @@ double ref_compute_dt(const double *U, int nx, int ny, int nz, double dx, double
     compute_dt_3d(&s);
     return s.dt;
 }
+
+/* The reference's initial-condition problems on its own solver (test
+ * infrastructure: compared with hydro_fill_initial_state). */
+int ref_ic_problem(void *p, int problem) {
+    set_initial_conditions_3d((GodunovHLLC3D *)p, (ProblemType)problem);
+    return 0;
+}
+int ref_ic_deck(void *p, const char *deck_path) {
+    InputDeck *d = input_deck_create(deck_path);
+    if (!d) return -1;
+    set_initial_conditions_3d_from_deck((GodunovHLLC3D *)p, d);
+    input_deck_destroy(d);
+    return 0;
+}
+void ref_set_gamma(void *p, double gamma) { ((GodunovHLLC3D *)p)->gamma = gamma; }
+/* main_3d.c's set-up without output: the solver and its initial state from a
+ * deck file (hydro_solver_3d.c:130-156, 2092-2133). */
+void *ref_from_deck(const char *deck_path) {
+    InputDeck *d = input_deck_create(deck_path);
+    if (!d) return NULL;
+    GodunovHLLC3D *s = solver3d_create_from_deck(d);
+    if (s) set_initial_conditions_3d_from_deck(s, d);
+    input_deck_destroy(d);
+    return s;
+}

@@ -98,6 +98,11 @@ def ref_lib():
         lib.ref_dt.argtypes = [C.c_void_p]
         lib.ref_step.argtypes = [C.c_void_p]
         lib.ref_run.argtypes = [C.c_void_p, C.c_double, C.c_int, _dp, _dp, _ip]
+        lib.ref_ic_problem.argtypes = [C.c_void_p, C.c_int]
+        lib.ref_ic_deck.argtypes = [C.c_void_p, C.c_char_p]
+        lib.ref_set_gamma.argtypes = [C.c_void_p, C.c_double]
+        lib.ref_from_deck.restype = C.c_void_p
+        lib.ref_from_deck.argtypes = [C.c_char_p]
         lib.ref_c2p_batch.argtypes = [C.c_int, _dp, C.c_double, _dp]
         lib.ref_p2c_batch.argtypes = [C.c_int, _dp, C.c_double, _dp]
         lib.ref_hllc_batch.argtypes = [C.c_int, C.c_int, _dp, _dp, C.c_double, _dp]
