@@ -27,6 +27,7 @@ constexpr long long SEQ_LIM = 1LL << 55;  // any |D|, lo, hi beyond this never a
 constexpr long long SEQ_2P52 = 1LL << 52;
 constexpr long long SEQ_2P53 = 1LL << 53;
 constexpr int SEQ_CHAIN_TILE = 2048;
+constexpr int SEQ_CHAIN_ROW = SEQ_CHAIN_TILE + 32;  // a tile and the overread of its last reads
 
 __device__ __forceinline__ int rl(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
 
@@ -106,7 +107,7 @@ __device__ __forceinline__ long long wave_max64(long long v) {
 // 256-thread workgroup stage the next tile in LDS.  Returns the sum (valid in
 // every thread of the first wavefront).
 __device__ __forceinline__ double seq_chain_group(const double* __restrict__ x, long npad,
-                                                  double (*buf)[SEQ_CHAIN_TILE]) {
+                                                  double (*buf)[SEQ_CHAIN_ROW]) {
     const long ntiles = npad / SEQ_CHAIN_TILE;
     const int tid = threadIdx.x;
     for (int e = tid; e < SEQ_CHAIN_TILE; e += 256) buf[0][e] = x[e];
@@ -130,9 +131,27 @@ __device__ __forceinline__ double seq_chain_group(const double* __restrict__ x, 
                 }
             }
         } else {
-            const double* b = buf[t & 1];
-#pragma unroll 32
-            for (int e = 0; e < SEQ_CHAIN_TILE; ++e) s = s + b[e];
+            // the next 16 terms are read while the current 16 are added
+            const double2* b = reinterpret_cast<const double2*>(buf[t & 1]);
+            double2 a[8], c[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) a[u] = b[u];
+            for (int e = 0; e < SEQ_CHAIN_TILE / 2; e += 16) {
+#pragma unroll
+                for (int u = 0; u < 8; ++u) c[u] = b[e + 8 + u];
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    s = s + a[u].x;
+                    s = s + a[u].y;
+                }
+#pragma unroll
+                for (int u = 0; u < 8; ++u) a[u] = b[e + 16 + u];  // past the tile: padding, unused
+#pragma unroll
+                for (int u = 0; u < 8; ++u) {
+                    s = s + c[u].x;
+                    s = s + c[u].y;
+                }
+            }
         }
         __syncthreads();
     }
@@ -141,7 +160,7 @@ __device__ __forceinline__ double seq_chain_group(const double* __restrict__ x, 
 
 __global__ void __launch_bounds__(256) seq_chain_kernel(const double* __restrict__ terms,
                                                         long npad, double* __restrict__ sums) {
-    __shared__ double buf[2][SEQ_CHAIN_TILE];
+    __shared__ double buf[2][SEQ_CHAIN_ROW];
     const double s = seq_chain_group(terms + (long)blockIdx.x * npad, npad, buf);
     if (threadIdx.x == 0) {
         sums[2 * blockIdx.x] = s;
@@ -500,7 +519,7 @@ __global__ void __launch_bounds__(256) seq_walk_kernel(const double* __restrict_
                                                        const SeqDesc* __restrict__ d2,
                                                        double* __restrict__ sums,
                                                        long long* __restrict__ stats) {
-    __shared__ double buf[2][SEQ_CHAIN_TILE];
+    __shared__ double buf[2][SEQ_CHAIN_ROW];
     __shared__ int s_bad[4];
     double* blk = buf[0];
     const int q = blockIdx.x;
