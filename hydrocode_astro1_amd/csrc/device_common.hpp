@@ -56,6 +56,25 @@ __device__ __forceinline__ double div_rcp(double a, double b, double yb) {
 // executes the full divisions.
 #define HYD_COLD_PATH() asm volatile("")
 
+// x / d for a divisor d that is a compile-time constant after unrolling (a
+// small integer here), correctly rounded: x (1/d) for a power of two (exact),
+// else the Markstein correction of x RN(1/d) as in div_rcp_q, whose quotient
+// theorem holds while x stays clear of underflow and overflow; outside that
+// range the IEEE division runs.
+__device__ __forceinline__ double div_by_const(double x, double d) {
+    if (d == 2.0 || d == 4.0 || d == 8.0) return x * (1.0 / d);
+    const double y = 1.0 / d;
+    const double q0 = x * y;
+    const double r = fma(-q0, d, x);
+    double q = fma(r, y, q0);
+    const double ax = fabs(x);
+    if (!(ax >= 0x1p-900 && ax <= 0x1p+900)) {
+        HYD_COLD_PATH();
+        q = x / d;
+    }
+    return q;
+}
+
 // Conserved state of one cell plus its primitives (P.rho itself is
 // recomputed from d by the floor).
 struct Cell {

@@ -314,49 +314,46 @@ __device__ __forceinline__ int mslot(int c) { return c + (c >> 4); }
 
 // The pairwise double-double tree over the block's 256 consecutive cells of
 // each of NQ terms: 16 threads per quantity each sum a 16-leaf subtree, then
-// one thread sums the 16 subtrees; the block's sum of quantity q goes to
-// part[q][blockIdx.x].  Two leaves meet in an exact two-sum (their
-// double-double sum with zero low parts), every later node in a
-// double-double addition.
+// the 16 subtree sums meet pairwise across those 16 lanes (partners 1, 2, 4,
+// 8 apart); the block's sum of quantity q goes to part[q][blockIdx.x].  Two
+// leaves meet in an exact two-sum (their double-double sum with zero low
+// parts), every later node in a double-double addition, the left operand
+// always the lower index.
 template <int NQ>
 __device__ __forceinline__ void block_tree_terms(const double (&term)[NQ], double* __restrict__ part,
-                                                 double (*s_term)[MROW], DD (*s_part)[16]) {
+                                                 double (*s_term)[MROW]) {
     const int tid = threadIdx.x;
+    const int qq = tid >> 4, st = tid & 15;
 #pragma unroll
     for (int c0 = 0; c0 < NQ; c0 += MQC) {
+        if (c0 > 0) __syncthreads();  // the previous round's reads of s_term are done
 #pragma unroll
-        for (int qq = 0; qq < MQC; ++qq)
-            if (c0 + qq < NQ) s_term[qq][mslot(tid)] = term[c0 + qq];
+        for (int k = 0; k < MQC; ++k)
+            if (c0 + k < NQ) s_term[k][mslot(tid)] = term[c0 + k];
         __syncthreads();
-        {
-            const int qq = tid >> 4, st = tid & 15;
-            if (c0 + qq < NQ) {
-                const double* leaf = &s_term[qq][mslot(st * 16)];
-                DD v[8];
-#pragma unroll
-                for (int e = 0; e < 8; ++e) v[e] = two_sum(leaf[2 * e], leaf[2 * e + 1]);
-#pragma unroll
-                for (int e = 0; e < 4; ++e) v[e] = dd_add(v[2 * e], v[2 * e + 1]);
-                v[0] = dd_add(v[0], v[1]);
-                v[1] = dd_add(v[2], v[3]);
-                s_part[qq][st] = dd_add(v[0], v[1]);
-            }
-        }
-        __syncthreads();
-        if (tid < MQC && c0 + tid < NQ) {
+        if (c0 + qq < NQ) {  // whole groups of 16 lanes
+            const double* leaf = &s_term[qq][mslot(st * 16)];
             DD v[8];
 #pragma unroll
-            for (int e = 0; e < 8; ++e) v[e] = dd_add(s_part[tid][2 * e], s_part[tid][2 * e + 1]);
+            for (int e = 0; e < 8; ++e) v[e] = two_sum(leaf[2 * e], leaf[2 * e + 1]);
 #pragma unroll
             for (int e = 0; e < 4; ++e) v[e] = dd_add(v[2 * e], v[2 * e + 1]);
             v[0] = dd_add(v[0], v[1]);
             v[1] = dd_add(v[2], v[3]);
-            const DD r = dd_add(v[0], v[1]);
-            const long o = 2 * ((long)(c0 + tid) * gridDim.x + blockIdx.x);
-            part[o] = r.hi;
-            part[o + 1] = r.lo;
+            DD sub = dd_add(v[0], v[1]);
+#pragma unroll
+            for (int o = 1; o < 16; o <<= 1) {
+                DD w;
+                w.hi = __shfl_xor(sub.hi, o, 64);
+                w.lo = __shfl_xor(sub.lo, o, 64);
+                if ((st & (2 * o - 1)) == 0) sub = dd_add(sub, w);
+            }
+            if (st == 0) {
+                const long o = 2 * ((long)(c0 + qq) * gridDim.x + blockIdx.x);
+                part[o] = sub.hi;
+                part[o + 1] = sub.lo;
+            }
         }
-        __syncthreads();  // the next round reuses s_term and s_part
     }
 }
 
@@ -366,13 +363,12 @@ __device__ __forceinline__ void block_tree_terms(const double (&term)[NQ], doubl
 // padding cells store zeros).
 template <int NQ>
 __device__ __forceinline__ void emit_terms(bool store, const double (&term)[NQ], long t, long npad,
-                                           double* __restrict__ part, double (*s_term)[MROW],
-                                           DD (*s_part)[16]) {
+                                           double* __restrict__ part, double (*s_term)[MROW]) {
     if (store) {
 #pragma unroll
         for (int q = 0; q < NQ; ++q) part[(long)q * npad + t] = term[q];
     } else {
-        block_tree_terms<NQ>(term, part, s_term, s_part);
+        block_tree_terms<NQ>(term, part, s_term);
     }
 }
 
@@ -382,7 +378,6 @@ __global__ void __launch_bounds__(256) moments_pass1_kernel(GridDims g,
                                                             const double* __restrict__ rho,
                                                             double* __restrict__ part, long npad) {
     __shared__ double s_term[MQC][MROW];
-    __shared__ DD s_part[MQC][16];
     const long n = (long)g.nx * g.ny * g.nz;
     const long t = (long)blockIdx.x * 256 + threadIdx.x;
     double term[4] = {0.0, 0.0, 0.0, 0.0};
@@ -399,7 +394,7 @@ __global__ void __launch_bounds__(256) moments_pass1_kernel(GridDims g,
             term[0] = dm; term[1] = dm * x; term[2] = dm * y; term[3] = dm * z;
         }
     }
-    emit_terms<4>(STORE, term, t, npad, part, s_term, s_part);
+    emit_terms<4>(STORE, term, t, npad, part, s_term);
 }
 
 constexpr int mp_nhi_c(int lmax) {
@@ -416,7 +411,6 @@ __global__ void __launch_bounds__(256) moments_pass2_kernel(GridDims g,
                                                             double* __restrict__ part, long npad) {
     constexpr int NQ = 9 + mp_nhi_c(LMAX);
     __shared__ double s_term[MQC][MROW];
-    __shared__ DD s_part[MQC][16];
     const long n = (long)g.nx * g.ny * g.nz;
     const long t = (long)blockIdx.x * 256 + threadIdx.x;
     const double r = (t < n) ? rho[t] : 0.0;
@@ -470,8 +464,8 @@ __global__ void __launch_bounds__(256) moments_pass2_kernel(GridDims g,
                     cl = f * z * cr; sl = f * z * ci;
                 } else {
                     const double a = (double)(2 * l - 1), b = (double)(l + m - 1), d = (double)(l - m);
-                    cl = (a * z * c1 - b * r2 * c2) / d;
-                    sl = (a * z * s1 - b * r2 * s2) / d;
+                    cl = div_by_const(a * z * c1 - b * r2 * c2, d);
+                    sl = div_by_const(a * z * s1 - b * r2 * s2, d);
                 }
                 if (l == m) {
                     c1 = cl; s1 = sl;
@@ -486,7 +480,7 @@ __global__ void __launch_bounds__(256) moments_pass2_kernel(GridDims g,
             }
         }
     }
-    emit_terms<NQ>(STORE, term, t, npad, part, s_term, s_part);
+    emit_terms<NQ>(STORE, term, t, npad, part, s_term);
 }
 
 // Fold 256 consecutive partials of quantity q (zero padded) pairwise.
