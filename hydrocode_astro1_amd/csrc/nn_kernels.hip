@@ -32,7 +32,9 @@ __device__ __forceinline__ void tap_offset(int t, int& dx, int& dy, int& dz) {
 // ------------------------------------------------------------- forward --
 // The convolution with the input tile staged in LDS.  A block owns a
 // 4 (x) x 4 (y) x 16 (z) brick of output voxels at a time; wavefront w takes
-// x slice w and its four y rows as four 16-voxel M tiles along z.  The
+// x slice w and its four y rows as four 16-voxel tiles along z.  The product
+// is formed as D = W^T X^T (output channels down, voxels across), so that a
+// lane ends with four consecutive channels of one voxel: one 8-byte store.  The
 // brick's input, widened by one voxel on every side (upsampled on the fly
 // for the decoder layers, zero outside the grid), is read from HBM once into
 // LDS, so every A fragment comes from LDS instead of a 27-fold global gather.
@@ -100,38 +102,48 @@ __global__ void __launch_bounds__(256) conv_fwd_lds_kernel(ConvArgs a) {
 #pragma unroll
                 for (int nt = 0; nt < NTL; ++nt)
                     acc[mt][nt] =
-                        __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bw[nt], acc[mt][nt], 0, 0, 0);
+                        __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(bw[nt], av, acc[mt][nt], 0, 0, 0);
             }
         }
-        // epilogue: bias, ReLU, optional skip add, stores (D row = z offset)
-        const int xi = x0 + wave;
+        // epilogue: bias, ReLU, optional skip add.  D = W^T X^T: lane (g, m)
+        // holds channels nt*16 + 4g .. +3 of voxel z0 + m, stored as 8 bytes.
+        const int xi = x0 + wave, zk = z0 + m;
 #pragma unroll
-        for (int mt = 0; mt < 4; ++mt)
+        for (int mt = 0; mt < 4; ++mt) {
+            const int yj = y0 + mt;
+            if (xi >= n || yj >= n || zk >= n) continue;
+            const long v = ((long)xi * n + yj) * n + zk;
 #pragma unroll
             for (int nt = 0; nt < NTL; ++nt) {
-                const int c = nt * 16 + m;
-                const int yj = y0 + mt;
+                const int c0 = nt * 16 + 4 * g;
+                if (c0 >= a.coutp) continue;
+                float val[4];
+                v4s rb;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
-                    const int zk = z0 + g * 4 + i;
-                    if (xi >= n || yj >= n || zk >= n || c >= a.coutp) continue;
-                    const long v = ((long)xi * n + yj) * n + zk;
-                    float val = acc[mt][nt][i];
-                    if (c < a.cout) {
-                        if (a.bias) val += a.bias[c];
-                        if (a.relu) val = val > 0.f ? val : 0.f;
+                    float t = acc[mt][nt][i];
+                    if (c0 + i < a.cout) {
+                        if (a.bias) t += a.bias[c0 + i];
+                        if (a.relu) t = t > 0.f ? t : 0.f;
                     } else {
-                        val = 0.f;
+                        t = 0.f;
                     }
-                    const uint16_t rb = f2bf(val);
-                    if (a.yr) a.yr[v * a.coutp + c] = rb;
-                    if (a.yd) {
-                        const float s = a.skip ? bf2f(a.skip[v * a.coutp + c]) : 0.f;
-                        a.yd[v * a.coutp + c] = f2bf(bf2f(rb) + s);
-                    }
-                    if (a.yf && c == 0) a.yf[v] = a.add ? val + a.add[v] : val;
+                    val[i] = t;
+                    rb[i] = (short)f2bf(t);
                 }
+                if (a.yr) *reinterpret_cast<v4s*>(a.yr + v * a.coutp + c0) = rb;
+                if (a.yd) {
+                    v4s sk = v4s{0, 0, 0, 0};
+                    if (a.skip) sk = *reinterpret_cast<const v4s*>(a.skip + v * a.coutp + c0);
+                    v4s o;
+#pragma unroll
+                    for (int i = 0; i < 4; ++i)
+                        o[i] = (short)f2bf(bf2f((uint16_t)rb[i]) + bf2f((uint16_t)sk[i]));
+                    *reinterpret_cast<v4s*>(a.yd + v * a.coutp + c0) = o;
+                }
+                if (a.yf && c0 == 0) a.yf[v] = a.add ? val[0] + a.add[v] : val[0];
             }
+        }
     }
 }
 
@@ -144,7 +156,8 @@ void launch_fwd_cin(const ConvArgs& a, unsigned nb, hipStream_t st) {
 }
 
 hipError_t conv_forward(const ConvArgs& a, hipStream_t st) {
-    if (a.cout < 1 || a.cout > 32 || a.coutp < a.cout || a.n < 1 || (a.up && a.n % 2))
+    if (a.cout < 1 || a.cout > 32 || a.coutp < a.cout || a.coutp % 4 || a.n < 1 ||
+        (a.up && a.n % 2))
         return hipErrorInvalidValue;  // one wavefront covers at most two 16-channel tiles
     const long nbricks = (long)((a.n + BX - 1) / BX) * ((a.n + BY - 1) / BY) * ((a.n + BZ - 1) / BZ);
     const unsigned nb = (unsigned)(nbricks < CONV_MAX_BLOCKS ? nbricks : CONV_MAX_BLOCKS);
@@ -165,7 +178,9 @@ hipError_t conv_forward(const ConvArgs& a, hipStream_t st) {
 // walks bricks of 4 x 4 x 16 voxels (the forward kernel's bricks).  Per brick
 // it stages the gradient transposed, gs[co][voxel], and the input with a
 // one-voxel halo as three copies shifted by dz = -1, 0, 1 along z, so that
-// every fragment is one aligned 8-byte LDS read.  Rows past 27 * cinp are
+// every fragment is one aligned 8-byte LDS read.  Both LDS tiles pad their
+// rows by 16 elements (8 banks): unpadded, the 16 rows of a fragment fell on
+// the same 8 of the 64 banks.  Rows past 27 * cinp are
 // zero except the first, which is all ones and yields the bias gradient in
 // the same product.  Each block writes its sums to part[block]; a second
 // kernel adds the blocks in a fixed order (no atomics: the gradient does
@@ -177,8 +192,10 @@ __global__ void __launch_bounds__(256) conv_wgrad_lds_kernel(WgradArgs a) {
     constexpr int PAIRS = NT * MT;
     constexpr int PW = (PAIRS + 3) / 4;  // tile pairs per wavefront
     constexpr int G4 = CINP / 4;
-    __shared__ __attribute__((aligned(16))) uint16_t xs[3][CINP][LX][LY][BZ];
-    __shared__ __attribute__((aligned(16))) uint16_t gs[16 * MT][256];
+    constexpr int XCH = LX * LY * BZ + 16;  // one (copy, channel) plane, padded
+    constexpr int GROW = 256 + 16;          // one gradient channel, padded
+    __shared__ __attribute__((aligned(16))) uint16_t xs[3 * CINP * XCH];
+    __shared__ __attribute__((aligned(16))) uint16_t gs[16 * MT * GROW];
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int g4 = lane >> 4, m = lane & 15;
     const int n = a.n, nin = a.up ? n / 2 : n;
@@ -199,7 +216,7 @@ __global__ void __launch_bounds__(256) conv_wgrad_lds_kernel(WgradArgs a) {
         if (r < NROWS) {
             int dx, dy, dz;
             tap_offset(r / CINP, dx, dy, dz);
-            boff[q] = (((dz + 1) * CINP + r % CINP) * LX + (1 + dx)) * LY * BZ + (1 + dy) * BZ;
+            boff[q] = ((dz + 1) * CINP + r % CINP) * XCH + ((1 + dx) * LY + (1 + dy)) * BZ;
         }
     }
     v4f acc[PW];
@@ -228,7 +245,9 @@ __global__ void __launch_bounds__(256) conv_wgrad_lds_kernel(WgradArgs a) {
                 const int z = vz - sft;  // input z = z0 + z + sft - 1
                 if (z >= 0 && z < BZ) {
 #pragma unroll
-                    for (int c = 0; c < 4; ++c) xs[sft][4 * q + c][vx][vy][z] = (uint16_t)val[c];
+                    for (int c = 0; c < 4; ++c)
+                        xs[(sft * CINP + 4 * q + c) * XCH + (vx * LY + vy) * BZ + z] =
+                            (uint16_t)val[c];
                 }
             }
         }
@@ -244,7 +263,7 @@ __global__ void __launch_bounds__(256) conv_wgrad_lds_kernel(WgradArgs a) {
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
                 const int co = 4 * q + c;
-                gs[co][v] = (co < a.cout) ? (uint16_t)val[c] : (uint16_t)0;
+                gs[co * GROW + v] = (co < a.cout) ? (uint16_t)val[c] : (uint16_t)0;
             }
         }
         __syncthreads();
@@ -255,14 +274,14 @@ __global__ void __launch_bounds__(256) conv_wgrad_lds_kernel(WgradArgs a) {
             v4s av[MT];
 #pragma unroll
             for (int mt = 0; mt < MT; ++mt)
-                av[mt] = *reinterpret_cast<const v4s*>(&gs[mt * 16 + m][vbase]);
+                av[mt] = *reinterpret_cast<const v4s*>(&gs[(mt * 16 + m) * GROW + vbase]);
 #pragma unroll
             for (int q = 0; q < PW; ++q) {
                 if (wave + 4 * q >= PAIRS) break;
                 const int mt = (wave + 4 * q) % MT;
                 v4s bv = v4s{0, 0, 0, 0};
                 if (bkind[q] == 0)
-                    bv = *reinterpret_cast<const v4s*>(&xs[0][0][0][0][0] + boff[q] + xoff);
+                    bv = *reinterpret_cast<const v4s*>(&xs[boff[q] + xoff]);
                 else if (bkind[q] == 1)
                     bv = ones;
                 acc[q] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[mt], bv, acc[q], 0, 0, 0);
