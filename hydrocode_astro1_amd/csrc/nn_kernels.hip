@@ -194,26 +194,31 @@ __global__ void __launch_bounds__(256) conv_wgrad_lds_kernel(WgradArgs a) {
     constexpr int G4 = CINP / 4;
     constexpr int XCH = LX * LY * BZ + 16;  // one (copy, channel) plane, padded
     constexpr int GROW = 256 + 16;          // one gradient channel, padded
-    __shared__ __attribute__((aligned(16))) uint16_t xs[3 * CINP * XCH];
+    constexpr int XONES = 3 * CINP * XCH;   // a plane of ones: the bias row
+    constexpr int XZERO = XONES + XCH;      // a plane of zeros: padding rows
+    __shared__ __attribute__((aligned(16))) uint16_t xs[3 * CINP * XCH + 2 * XCH];
     __shared__ __attribute__((aligned(16))) uint16_t gs[16 * MT * GROW];
-    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int lane = threadIdx.x & 63;
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int mtw = wave % MT;  // pairs p = wave + 4 q all take channel tile p % MT
     const int g4 = lane >> 4, m = lane & 15;
     const int n = a.n, nin = a.up ? n / 2 : n;
     const int nbx = (n + BX - 1) / BX, nby = (n + BY - 1) / BY, nbz = (n + BZ - 1) / BZ;
     const int nbricks = nbx * nby * nbz;
     const v4s* xin = reinterpret_cast<const v4s*>(a.x);
 
-    // per tile pair of this wavefront: LDS element offset of its B row
-    // (tap and channel) and whether the row is real, the ones row or padding
-    int boff[PW], bkind[PW];
+    // per tile pair of this wavefront: LDS element offset of its B row (tap
+    // and channel), of the ones plane for the bias row, of the zeros plane
+    // for padding rows and for a wavefront's pairs past the last; every
+    // fragment is then one unconditional LDS read
+    int boff[PW];
 #pragma unroll
     for (int q = 0; q < PW; ++q) {
         const int p = wave + 4 * q;
-        const int nt = p / MT;
-        const int r = nt * 16 + m;
-        boff[q] = 0;
-        bkind[q] = (p < PAIRS) ? (r < NROWS ? 0 : (r == NROWS ? 1 : 2)) : 2;
-        if (r < NROWS) {
+        const int r = (p / MT) * 16 + m;
+        boff[q] = XZERO;
+        if (p < PAIRS && r == NROWS) boff[q] = XONES;
+        if (p < PAIRS && r < NROWS) {
             int dx, dy, dz;
             tap_offset(r / CINP, dx, dy, dz);
             boff[q] = ((dz + 1) * CINP + r % CINP) * XCH + ((1 + dx) * LY + (1 + dy)) * BZ;
@@ -222,69 +227,76 @@ __global__ void __launch_bounds__(256) conv_wgrad_lds_kernel(WgradArgs a) {
     v4f acc[PW];
 #pragma unroll
     for (int q = 0; q < PW; ++q) acc[q] = v4f{0.f, 0.f, 0.f, 0.f};
-    const v4s ones = v4s{(short)0x3F80, (short)0x3F80, (short)0x3F80, (short)0x3F80};
+    for (int e = threadIdx.x; e < XCH; e += 256) {  // ordered by the first brick's barrier
+        xs[XONES + e] = 0x3F80;                       // 1.0 in bfloat16
+        xs[XZERO + e] = 0;
+    }
 
     for (int b = blockIdx.x; b < nbricks; b += gridDim.x) {
         const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
         const int x0 = bx * BX, y0 = by * BY, z0 = bz * BZ;
         __syncthreads();  // the previous brick's reads are done
         // input: positions (x0-1 .. x0+4) x (y0-1 .. y0+4) x (z0-1 .. z0+16),
-        // 4-channel groups; position z_in goes to copy s at z = z_in - z0 - s + 1
-        for (int e = threadIdx.x; e < LX * LY * (BZ + 2) * G4; e += 256) {
-            const int q = e % G4, vz = (e / G4) % (BZ + 2), vy = (e / (G4 * (BZ + 2))) % LY,
-                      vx = e / (G4 * (BZ + 2) * LY);
-            const int ii = x0 - 1 + vx, jj = y0 - 1 + vy, kz = z0 - 1 + vz;
-            v4s val = v4s{0, 0, 0, 0};
-            if (ii >= 0 && ii < n && jj >= 0 && jj < n && kz >= 0 && kz < n) {
-                const int si = a.up ? ii >> 1 : ii, sj = a.up ? jj >> 1 : jj,
-                          sk = a.up ? kz >> 1 : kz;
-                val = xin[(((long)si * nin + sj) * nin + sk) * G4 + q];
-            }
+        // 4-channel groups; position z_in goes to copy s at z = z_in - z0 - s + 1.
+        // A thread takes one channel group of one (x, y) row and four output
+        // z of every copy: it reads the six input voxels they need and writes
+        // each (copy, channel) as one 8-byte quad of z.
+        for (int e = threadIdx.x; e < LX * LY * 4 * G4; e += 256) {
+            const int q = e % G4, zq = (e / G4) % 4, vy = (e / (G4 * 4)) % LY,
+                      vx = e / (G4 * 4 * LY);
+            const int ii = x0 - 1 + vx, jj = y0 - 1 + vy;
+            const bool row = ii >= 0 && ii < n && jj >= 0 && jj < n;
+            const int si = a.up ? ii >> 1 : ii, sj = a.up ? jj >> 1 : jj;
+            const long rbase = ((long)si * nin + sj) * nin;
+            v4s u[6];
 #pragma unroll
-            for (int sft = 0; sft < 3; ++sft) {
-                const int z = vz - sft;  // input z = z0 + z + sft - 1
-                if (z >= 0 && z < BZ) {
-#pragma unroll
-                    for (int c = 0; c < 4; ++c)
-                        xs[(sft * CINP + 4 * q + c) * XCH + (vx * LY + vy) * BZ + z] =
-                            (uint16_t)val[c];
-                }
+            for (int t = 0; t < 6; ++t) {
+                const int kz = z0 - 1 + 4 * zq + t;
+                u[t] = v4s{0, 0, 0, 0};
+                if (row && kz >= 0 && kz < n)
+                    u[t] = xin[(rbase + (a.up ? kz >> 1 : kz)) * G4 + q];
             }
+            uint16_t* dst = xs + (4 * q * XCH + (vx * LY + vy) * BZ + 4 * zq);
+#pragma unroll
+            for (int sft = 0; sft < 3; ++sft)
+#pragma unroll
+                for (int c = 0; c < 4; ++c)
+                    *reinterpret_cast<v4s*>(dst + (sft * CINP + c) * XCH) =
+                        v4s{u[sft][c], u[sft + 1][c], u[sft + 2][c], u[sft + 3][c]};
         }
-        // gradient of the brick's voxels, transposed; zero outside the grid
-        for (int e = threadIdx.x; e < 256 * (4 * MT); e += 256) {
-            const int q = e % (4 * MT), v = e / (4 * MT);
+        // gradient of the brick's voxels, transposed; zero outside the grid.
+        // A thread takes one channel group of four consecutive voxels along z.
+        for (int e = threadIdx.x; e < 64 * (4 * MT); e += 256) {
+            const int q = e % (4 * MT), vq = e / (4 * MT);
+            const int v = 4 * vq;
             const int vz = v % BZ, vy = (v / BZ) % BY, vx = v / (BZ * BY);
-            const int i = x0 + vx, j = y0 + vy, k = z0 + vz;
-            v4s val = v4s{0, 0, 0, 0};
-            if (4 * q < a.coutp && i < n && j < n && k < n)
-                val = *reinterpret_cast<const v4s*>(a.g + (((long)i * n + j) * n + k) * a.coutp +
-                                                    4 * q);
+            const int i = x0 + vx, j = y0 + vy;
+            v4s u[4];
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int k = z0 + vz + t;
+                u[t] = v4s{0, 0, 0, 0};
+                if (4 * q < a.coutp && i < n && j < n && k < n)
+                    u[t] = *reinterpret_cast<const v4s*>(a.g + (((long)i * n + j) * n + k) * a.coutp +
+                                                         4 * q);
+            }
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
                 const int co = 4 * q + c;
-                gs[co * GROW + v] = (co < a.cout) ? (uint16_t)val[c] : (uint16_t)0;
+                *reinterpret_cast<v4s*>(gs + co * GROW + v) =
+                    (co < a.cout) ? v4s{u[0][c], u[1][c], u[2][c], u[3][c]} : v4s{0, 0, 0, 0};
             }
         }
         __syncthreads();
-#pragma unroll 4
+#pragma unroll
         for (int ks = 0; ks < 16; ++ks) {  // 16 voxels: (vx, vy) = (ks / 4, ks % 4), z 0..15
             const int vbase = ks * 16 + 4 * g4;
             const int xoff = ((ks >> 2) * LY + (ks & 3)) * BZ + 4 * g4;
-            v4s av[MT];
-#pragma unroll
-            for (int mt = 0; mt < MT; ++mt)
-                av[mt] = *reinterpret_cast<const v4s*>(&gs[(mt * 16 + m) * GROW + vbase]);
+            const v4s av = *reinterpret_cast<const v4s*>(&gs[(mtw * 16 + m) * GROW + vbase]);
 #pragma unroll
             for (int q = 0; q < PW; ++q) {
-                if (wave + 4 * q >= PAIRS) break;
-                const int mt = (wave + 4 * q) % MT;
-                v4s bv = v4s{0, 0, 0, 0};
-                if (bkind[q] == 0)
-                    bv = *reinterpret_cast<const v4s*>(&xs[boff[q] + xoff]);
-                else if (bkind[q] == 1)
-                    bv = ones;
-                acc[q] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av[mt], bv, acc[q], 0, 0, 0);
+                const v4s bv = *reinterpret_cast<const v4s*>(&xs[boff[q] + xoff]);
+                acc[q] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[q], 0, 0, 0);
             }
         }
     }
