@@ -381,6 +381,9 @@ def main():
     ap.add_argument("--selfgravity-arms", default="", metavar="NAMES",
                     help="comma-separated subset of the self-gravity arms (reference_sor, "
                          "reference_sor_sequential_moments, nn_fp32_dc_multigrid)")
+    ap.add_argument("--nn-probe", type=int, default=0, metavar="REPS",
+                    help="only time REPS forward and REPS training passes of the guess network "
+                         "on a 128^3 field, then exit")
     ap.add_argument("--calibrate", action="store_true",
                     help="only run the device layout-conversion copy (40 B read + 40 B written "
                          "per cell) used to scale HBM byte counters, then exit")
@@ -422,6 +425,10 @@ def run(args):
         k = args.snapshot_study
         return {"snapshot_study": snapshot_study(H, ics, n, steps=k, every=max(1, k // 10),
                                                  big=(0, k // 2, k))}
+    if args.nn_probe > 0:
+        from hydrocode_astro1_amd import nn
+        return {"nn_probe": network_timing(H, nn.Model(path=nn.DEFAULT_WEIGHTS), 128,
+                                           args.nn_probe)}
     if args.multigrid_probe > 0:
         import numpy as np
         rho = np.ascontiguousarray(ics.explosion_n3(n)[..., 0])

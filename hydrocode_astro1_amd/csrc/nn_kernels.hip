@@ -15,13 +15,19 @@ namespace nn {
 typedef short v4s __attribute__((ext_vector_type(4)));
 typedef float v4f __attribute__((ext_vector_type(4)));
 
-__device__ __forceinline__ uint16_t f2bf(float f) {
-    uint32_t u = __float_as_uint(f);
-    if ((u & 0x7fffffffu) > 0x7f800000u) return 0x7fc0;  // NaN
-    u += 0x7fffu + ((u >> 16) & 1u);                       // round to nearest even
-    return (uint16_t)(u >> 16);
+typedef float v2f __attribute__((ext_vector_type(2)));
+typedef __bf16 v2bf __attribute__((ext_vector_type(2)));
+typedef unsigned int v2u __attribute__((ext_vector_type(2)));
+
+// FP32 -> BF16 rounded to nearest even: gfx950's v_cvt_pk_bf16_f32, two
+// values per instruction (NaN stays NaN).
+__device__ __forceinline__ uint16_t f2bf(float f) { return __builtin_bit_cast(uint16_t, (__bf16)f); }
+__device__ __forceinline__ uint32_t f2bf2(float lo, float hi) {
+    return __builtin_bit_cast(uint32_t, __builtin_convertvector(v2f{lo, hi}, v2bf));
 }
 __device__ __forceinline__ float bf2f(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+__device__ __forceinline__ float bf_lo(uint32_t w) { return __uint_as_float(w << 16); }
+__device__ __forceinline__ float bf_hi(uint32_t w) { return __uint_as_float(w & 0xffff0000u); }
 
 __device__ __forceinline__ void tap_offset(int t, int& dx, int& dy, int& dz) {
     dx = t / 9 - 1;
@@ -42,7 +48,10 @@ __device__ __forceinline__ void tap_offset(int t, int& dx, int& dy, int& dz) {
 // block walks bricks blockIdx.x, blockIdx.x + gridDim.x, ...
 constexpr int BX = 4, BY = 4, BZ = 16;
 constexpr int LX = BX + 2, LY = BY + 2, LZ = BZ + 2;
-constexpr int CONV_MAX_BLOCKS = 4096;
+#ifndef HYD_CONV_MAX_BLOCKS
+#define HYD_CONV_MAX_BLOCKS 2048  // 8 per CU; 1024 and 4096 measured 2-4% slower at 128^3
+#endif
+constexpr int CONV_MAX_BLOCKS = HYD_CONV_MAX_BLOCKS;
 
 template <int CINP, int NTL>
 __global__ void __launch_bounds__(256) conv_fwd_lds_kernel(ConvArgs a) {
@@ -60,30 +69,49 @@ __global__ void __launch_bounds__(256) conv_fwd_lds_kernel(ConvArgs a) {
     for (int e = threadIdx.x; e < NTL * KSTEPS * 64; e += 256) s_w[e] = wp[e];
     const int g = lane >> 4, m = lane & 15;
 
+    // The input brick, (x, y, z, group) entries with z and group fastest, is
+    // read into registers one brick ahead: the loads of brick b + gridDim.x
+    // are in flight while the MFMAs of brick b run.
+    constexpr int NE = LX * LY * LZ * G4;
+    constexpr int PER = (NE + 255) / 256;
+    v4s stage[PER];
+    auto load_brick = [&](int blk) {
+        const int bz = blk % nbz, by = (blk / nbz) % nby, bx = blk / (nbz * nby);
+#pragma unroll
+        for (int it = 0; it < PER; ++it) {
+            const int e = threadIdx.x + 256 * it;
+            const int q = e % G4, vz = (e / G4) % LZ, vy = (e / (G4 * LZ)) % LY,
+                      vx = e / (G4 * LZ * LY);
+            const int ii = bx * BX - 1 + vx, jj = by * BY - 1 + vy, kz = bz * BZ - 1 + vz;
+            stage[it] = v4s{0, 0, 0, 0};
+            if (e < NE && (unsigned)ii < (unsigned)n && (unsigned)jj < (unsigned)n &&
+                (unsigned)kz < (unsigned)n) {
+                const int si = a.up ? ii >> 1 : ii, sj = a.up ? jj >> 1 : jj,
+                          sk = a.up ? kz >> 1 : kz;
+                stage[it] = xin[(((long)si * nin + sj) * nin + sk) * G4 + q];
+            }
+        }
+    };
+    if (blockIdx.x < nbricks) load_brick(blockIdx.x);
+
     for (int blk = blockIdx.x; blk < nbricks; blk += gridDim.x) {
         const int bz = blk % nbz, by = (blk / nbz) % nby, bx = blk / (nbz * nby);
         const int x0 = bx * BX, y0 = by * BY, z0 = bz * BZ;
         __syncthreads();  // the previous brick's reads are done
-        // stage the input brick: (x, y, z, group) entries, z and group fastest
-        for (int e = threadIdx.x; e < LX * LY * LZ * G4; e += 256) {
-            const int q = e % G4, vz = (e / G4) % LZ, vy = (e / (G4 * LZ)) % LY,
-                      vx = e / (G4 * LZ * LY);
-            const int ii = x0 - 1 + vx, jj = y0 - 1 + vy, kz = z0 - 1 + vz;
-            v4s val = v4s{0, 0, 0, 0};
-            if (ii >= 0 && ii < n && jj >= 0 && jj < n && kz >= 0 && kz < n) {
-                const int si = a.up ? ii >> 1 : ii, sj = a.up ? jj >> 1 : jj,
-                          sk = a.up ? kz >> 1 : kz;
-                val = xin[(((long)si * nin + sj) * nin + sk) * G4 + q];
-            }
-            s_in[e] = val;
+#pragma unroll
+        for (int it = 0; it < PER; ++it) {
+            const int e = threadIdx.x + 256 * it;
+            if (e < NE) s_in[e] = stage[it];
         }
         __syncthreads();
+        if (blk + (int)gridDim.x < nbricks) load_brick(blk + gridDim.x);
 
         v4f acc[4][NTL];
 #pragma unroll
         for (int mt = 0; mt < 4; ++mt)
 #pragma unroll
             for (int nt = 0; nt < NTL; ++nt) acc[mt][nt] = v4f{0.f, 0.f, 0.f, 0.f};
+#pragma unroll  // taps, channels and LDS offsets become constants
         for (int ks = 0; ks < KSTEPS; ++ks) {
             const int kk = (ks * 4 + g) * 4;
             const int t = kk / CINP, ci = kk % CINP;
@@ -118,7 +146,6 @@ __global__ void __launch_bounds__(256) conv_fwd_lds_kernel(ConvArgs a) {
                 const int c0 = nt * 16 + 4 * g;
                 if (c0 >= a.coutp) continue;
                 float val[4];
-                v4s rb;
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
                     float t = acc[mt][nt][i];
@@ -129,17 +156,17 @@ __global__ void __launch_bounds__(256) conv_fwd_lds_kernel(ConvArgs a) {
                         t = 0.f;
                     }
                     val[i] = t;
-                    rb[i] = (short)f2bf(t);
                 }
-                if (a.yr) *reinterpret_cast<v4s*>(a.yr + v * a.coutp + c0) = rb;
+                const v2u rb = v2u{f2bf2(val[0], val[1]), f2bf2(val[2], val[3])};
+                if (a.yr) *reinterpret_cast<v2u*>(a.yr + v * a.coutp + c0) = rb;
                 if (a.yd) {
-                    v4s sk = v4s{0, 0, 0, 0};
-                    if (a.skip) sk = *reinterpret_cast<const v4s*>(a.skip + v * a.coutp + c0);
-                    v4s o;
+                    v2u sk = v2u{0u, 0u};
+                    if (a.skip) sk = *reinterpret_cast<const v2u*>(a.skip + v * a.coutp + c0);
+                    v2u o;
 #pragma unroll
-                    for (int i = 0; i < 4; ++i)
-                        o[i] = (short)f2bf(bf2f((uint16_t)rb[i]) + bf2f((uint16_t)sk[i]));
-                    *reinterpret_cast<v4s*>(a.yd + v * a.coutp + c0) = o;
+                    for (int h = 0; h < 2; ++h)
+                        o[h] = f2bf2(bf_lo(rb[h]) + bf_lo(sk[h]), bf_hi(rb[h]) + bf_hi(sk[h]));
+                    *reinterpret_cast<v2u*>(a.yd + v * a.coutp + c0) = o;
                 }
                 if (a.yf && c0 == 0) a.yf[v] = a.add ? val[0] + a.add[v] : val[0];
             }
