@@ -536,7 +536,36 @@ __global__ void moments_finish_kernel(const double* __restrict__ sums, int nq, i
 // r = f - L phi on interior cells (0 on the boundary), both colours of
 // split entry m of row (i, j); the block's sums of r^2 and f^2 go to
 // part[block] and residual_finish_kernel adds the blocks in a fixed order
-// (no atomics: the result is reproducible).  Planes are grid-strided.
+// (no atomics: the result is reproducible).  A block marches RES_PLANES
+// consecutive x planes.  Each thread keeps the centre values of planes
+// i - 1 .. i + 1 of both colours of its (j, m) in registers (the x
+// neighbours, the centre, and the z neighbour at the same split index), and
+// the loads of plane i + 1's other operands and plane i + 2's centres are
+// issued before plane i is computed, so one plane of loads is in flight.
+constexpr int RES_PLANES = 16;
+
+struct ResOperands {  // one colour of one cell: f, y neighbours, the other z neighbour
+    double f, ym, yp, zx;
+};
+
+__device__ __forceinline__ ResOperands res_load(const GridDims& g, int i, int j, int m, int colour,
+                                                const double* __restrict__ pr,
+                                                const double* __restrict__ pb,
+                                                const double* __restrict__ fr,
+                                                const double* __restrict__ fb) {
+    ResOperands o{0.0, 0.0, 0.0, 0.0};
+    const int p = (i + j + colour) & 1, k = 2 * m + p;
+    if (i >= 1 && i <= g.nx - 2 && j >= 1 && j <= g.ny - 2 && k >= 1 && k <= g.nz - 2) {
+        const long base = ((long)i * g.ny + j) * g.nzh;
+        const double* other = colour ? pr : pb;
+        o.f = (colour ? fb : fr)[base + m];
+        o.ym = other[base - g.nzh + m];
+        o.yp = other[base + g.nzh + m];
+        o.zx = other[base + (p ? m + 1 : m - 1)];  // the z pair is this and other[m]
+    }
+    return o;
+}
+
 __global__ void __launch_bounds__(256) residual_kernel(GridDims g, SorCoef c,
                                                        const double* __restrict__ pr,
                                                        const double* __restrict__ pb,
@@ -547,11 +576,37 @@ __global__ void __launch_bounds__(256) residual_kernel(GridDims g, SorCoef c,
                                                        double* __restrict__ part) {
     const int m = blockIdx.x * 64 + (threadIdx.x & 63);
     const int j = blockIdx.y * SOR_ROWS + (threadIdx.x >> 6);
+    const int i0 = blockIdx.z * RES_PLANES;
+    const int i1 = i0 + RES_PLANES < g.nx ? i0 + RES_PLANES : g.nx;
     double s_rr = 0.0, s_ff = 0.0;
     if (m < g.nzh && j < g.ny) {
         const long sxh = (long)g.ny * g.nzh;
-        for (int i = blockIdx.z; i < g.nx; i += gridDim.z) {
-            const long base = ((long)i * g.ny + j) * g.nzh;
+        const long row = (long)j * g.nzh + m;
+        double r_prev = 0.0, b_prev = 0.0, r_next = 0.0, b_next = 0.0;
+        if (i0 > 0) {
+            r_prev = pr[(long)(i0 - 1) * sxh + row];
+            b_prev = pb[(long)(i0 - 1) * sxh + row];
+        }
+        double r_cur = pr[(long)i0 * sxh + row], b_cur = pb[(long)i0 * sxh + row];
+        if (i0 + 1 < g.nx) {
+            r_next = pr[(long)(i0 + 1) * sxh + row];
+            b_next = pb[(long)(i0 + 1) * sxh + row];
+        }
+        ResOperands o0 = res_load(g, i0, j, m, 0, pr, pb, fr, fb);
+        ResOperands o1 = res_load(g, i0, j, m, 1, pr, pb, fr, fb);
+        for (int i = i0; i < i1; ++i) {
+            // next iteration's loads, in flight while plane i is computed
+            double r_nn = 0.0, b_nn = 0.0;
+            if (i + 2 < g.nx) {
+                r_nn = pr[(long)(i + 2) * sxh + row];
+                b_nn = pb[(long)(i + 2) * sxh + row];
+            }
+            ResOperands n0{0.0, 0.0, 0.0, 0.0}, n1{0.0, 0.0, 0.0, 0.0};
+            if (i + 1 < i1) {
+                n0 = res_load(g, i + 1, j, m, 0, pr, pb, fr, fb);
+                n1 = res_load(g, i + 1, j, m, 1, pr, pb, fr, fb);
+            }
+            const long base = (long)i * sxh + (long)j * g.nzh;
 #pragma unroll
             for (int colour = 0; colour < 2; ++colour) {
                 const int k = 2 * m + ((i + j + colour) & 1);
@@ -561,19 +616,25 @@ __global__ void __launch_bounds__(256) residual_kernel(GridDims g, SorCoef c,
                     if (rout) rout[base + m] = 0.0f;
                     continue;
                 }
-                const double* mine = colour ? pb : pr;
-                const double* other = colour ? pr : pb;
-                const double lap = (other[base - sxh + m] + other[base + sxh + m]) * c.dx2i +
-                                   (other[base - g.nzh + m] + other[base + g.nzh + m]) * c.dy2i +
-                                   (other[base + ((k - 1) >> 1)] + other[base + ((k + 1) >> 1)]) *
-                                       c.dz2i -
-                                   c.factor * mine[base + m];
-                const double f = (colour ? fb : fr)[base + m];
-                const double r = f - lap;
+                const ResOperands& o = colour ? o1 : o0;
+                const double lap = ((colour ? r_prev : b_prev) + (colour ? r_next : b_next)) *
+                                       c.dx2i +
+                                   (o.ym + o.yp) * c.dy2i +
+                                   (o.zx + (colour ? r_cur : b_cur)) * c.dz2i -
+                                   c.factor * (colour ? b_cur : r_cur);
+                const double r = o.f - lap;
                 if (rout) rout[base + m] = (float)r;
                 s_rr += r * r;
-                s_ff += f * f;
+                s_ff += o.f * o.f;
             }
+            r_prev = r_cur;
+            b_prev = b_cur;
+            r_cur = r_next;
+            b_cur = b_next;
+            r_next = r_nn;
+            b_next = b_nn;
+            o0 = n0;
+            o1 = n1;
         }
     }
     __shared__ double sp[2][4];
@@ -868,7 +929,7 @@ hipError_t launch_acceleration(const GridDims& g, const double* phi_r, const dou
 
 static dim3 residual_grid(const GridDims& g) {
     return dim3((unsigned)((g.nzh + 63) / 64), (unsigned)((g.ny + SOR_ROWS - 1) / SOR_ROWS),
-                (unsigned)(g.nx < 64 ? g.nx : 64));
+                (unsigned)((g.nx + RES_PLANES - 1) / RES_PLANES));
 }
 
 long residual_scratch_len(const GridDims& g) {
