@@ -109,7 +109,7 @@ def gaussian_field(n, slope, sigma, seed):
     kk = np.sqrt(k[:, None, None] ** 2 + k[None, :, None] ** 2 + kz[None, None, :] ** 2)
     amp = np.zeros_like(kk)
     amp[kk > 0] = kk[kk > 0] ** (slope / 2.0)
-    field = np.fft.irfftn(fk * amp, s=(n, n, n))
+    field = np.fft.irfftn(fk * amp, s=(n, n, n), axes=(0, 1, 2))
     field -= field.mean()
     field *= sigma / field.std()
     return field
