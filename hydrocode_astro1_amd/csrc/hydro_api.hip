@@ -154,6 +154,7 @@ int step_device(GodunovHLLC3D* s) {
     a.cfl = s->cfl;
     a.gamma = s->gamma;
     a.gravity = grav ? 1 : 0;
+    a.zcopy = hyd::hydro_zcopy(s->nz, s->bc_zmin, s->bc_zmax) ? 1 : 0;
     HYD_CHECK(hipEventRecord(d->record_ev[0] ? d->record_ev[0] : d->ev0, d->stream));
     HYD_CHECK(hyd::launch_hydro_step(a, d->stream));
     if (d->record_ev[1]) HYD_CHECK(hipEventRecord(d->record_ev[1], d->stream));
@@ -166,6 +167,7 @@ int step_device(GodunovHLLC3D* s) {
     bc.bc[2] = s->bc_ymin; bc.bc[3] = s->bc_ymax;
     bc.bc[4] = s->bc_zmin; bc.bc[5] = s->bc_zmax;
     bc.gamma = s->gamma;
+    bc.zcopy = a.zcopy;
     HYD_CHECK(hyd::launch_hydro_bc(bc, d->stream));
     HYD_CHECK(hipEventRecord(d->ev1, d->stream));
     if (d->record_ev[2]) HYD_CHECK(hipEventRecord(d->record_ev[2], d->stream));

@@ -198,6 +198,10 @@ __global__ void __launch_bounds__(NT) hydro_step_kernel(HydroStepArgs a) {
     if (wave >= 1 && wave <= TJ) {
         // ---------------------------------------------------- cell rows
         const double ax = dt / a.dx, ay = dt / a.dy, az = dt / a.dz;
+        // block-uniform: this tile holds the z-min face (lane 0) or the
+        // z-max face (lane nz-1-k0, with its neighbour in the same wave)
+        const bool zlo = a.zcopy && k0 == 0;
+        const bool zhi = a.zcopy && k0 + TK >= nz;
         double fxl[5] = {0, 0, 0, 0, 0};
         unsigned long long smax = 0ull;
         if (ib > 0) {  // F_x(ib-1) for the update of plane ib
@@ -278,6 +282,24 @@ __global__ void __launch_bounds__(NT) hydro_step_kernel(HydroStepArgs a) {
                     const double work = rho * (u * gx + v * gy + w * gz);
                     q[4] = q[4] + dt * work;
                 }
+                // z faces with outflow on both sides: the final value of
+                // (i, j, 0) is that of (i, j, 1) and of (i, j, nz-1) that of
+                // (i, j, nz-2), the neighbouring lane's new state
+                // (hydro_solver_3d.c:535-550 after the update and gravity
+                // source).  Its signal speed equals the neighbour's, which is
+                // already in the maximum.
+                double qz[5];
+                if (zlo) {
+#pragma unroll
+                    for (int f = 0; f < 5; ++f) qz[f] = __shfl_down(q[f], 1, 64);
+                }
+                if (zhi) {
+#pragma unroll
+                    for (int f = 0; f < 5; ++f) {
+                        const double t = __shfl_up(q[f], 1, 64);
+                        qz[f] = (k == nz - 1) ? t : qz[f];
+                    }
+                }
                 if (interior) {
 #pragma unroll
                     for (int f = 0; f < 5; ++f) a.uout[f][idx] = q[f];
@@ -285,6 +307,10 @@ __global__ void __launch_bounds__(NT) hydro_step_kernel(HydroStepArgs a) {
                     unsigned long long sb =
                         dbits(signal_speed_floored(q[0], q[1], q[2], q[3], q[4], kc));
                     smax = sb > smax ? sb : smax;
+                } else if (a.zcopy && (k == 0 || k == nz - 1) && i >= 1 && i <= nx - 2 &&
+                           j >= 1 && j <= ny - 2) {
+#pragma unroll
+                    for (int f = 0; f < 5; ++f) a.uout[f][idx] = qz[f];
                 } else {
 #pragma unroll
                     for (int f = 0; f < 5; ++f) a.face[face_index(nx, ny, nz, i, j, k, f)] = q[f];
@@ -541,11 +567,16 @@ hipError_t launch_hydro_step(const HydroStepArgs& a, hipStream_t st) {
     return hipGetLastError();
 }
 
+bool hydro_zcopy(int nz, int bc_zmin, int bc_zmax) {
+    return bc_zmin == 2 && bc_zmax == 2 && nz >= 3 && (nz - 1) % TK != 0;  // BC_OUTFLOW == 2
+}
+
 hipError_t launch_hydro_bc(const HydroBCArgs& a, hipStream_t st) {
-    // widest inner and outer extents over the three face kinds
-    const int inner = a.nz > a.ny - 2 ? a.nz : a.ny - 2;
+    // widest inner and outer extents over the face kinds launched
+    const int inner = a.zcopy ? a.nz : (a.nz > a.ny - 2 ? a.nz : a.ny - 2);
     const int outer = a.ny > a.nx - 2 ? a.ny : a.nx - 2;
-    hipLaunchKernelGGL(hydro_bc_kernel, dim3((unsigned)((inner + 255) / 256), (unsigned)outer, 6),
+    hipLaunchKernelGGL(hydro_bc_kernel,
+                       dim3((unsigned)((inner + 255) / 256), (unsigned)outer, a.zcopy ? 4 : 6),
                        dim3(256), 0, st, a);
     return hipGetLastError();
 }

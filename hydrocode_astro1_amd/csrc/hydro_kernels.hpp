@@ -19,6 +19,7 @@ struct HydroStepArgs {
     int ntj, ntk, nxc;      // filled by the launcher
     double dx, dy, dz, cfl, gamma;
     int gravity;
+    int zcopy;              // z faces outflow on both sides: the step writes them (hydro_zcopy)
 };
 
 struct HydroBCArgs {
@@ -28,7 +29,14 @@ struct HydroBCArgs {
     int nx, ny, nz;
     int bc[6];              // xmin xmax ymin ymax zmin zmax (BCType values)
     double gamma;
+    int zcopy;              // the step kernel has written the z faces: x and y faces only
 };
+
+// True when both z sides are outflow and each z boundary cell shares a
+// wavefront with its inward neighbour: the step kernel then writes the z
+// faces' final values itself (a lane shuffle of the neighbour's new state)
+// and the boundary kernel leaves them alone.
+bool hydro_zcopy(int nz, int bc_zmin, int bc_zmax);
 
 // face buffer size in doubles for an nx x ny x nz grid
 inline long face_buffer_len(int nx, int ny, int nz) {
