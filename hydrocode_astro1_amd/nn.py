@@ -26,7 +26,8 @@ import time
 
 import numpy as np
 
-from . import (GRAV_BC_ISOLATED, POISSON_FP32_DC, SOR_CRIT_REL_RESID, Gravity, _check, lib)
+from . import (DC_INNER_MULTIGRID, DC_INNER_SOR, GRAV_BC_ISOLATED, POISSON_FP32_DC,
+               SOR_CRIT_REL_RESID, Gravity, _check, lib)
 
 _fp = C.POINTER(C.c_float)
 _dp = C.POINTER(C.c_double)
@@ -189,22 +190,27 @@ def field_params(seed):
     return blobs, modes, floor_rho, sigma
 
 
-def make_solver(n, omega=1.9, tol=1e-8, lmax=6):
+def make_solver(n, omega=1.9, tol=1e-8, lmax=6, inner=None):
     """Isolated n^3 solver on the unit box used to produce training targets:
-    FP32 defect correction to a relative residual `tol`."""
+    FP32 defect correction to a relative residual `tol`, its corrections
+    solved by multigrid V-cycles where the grid allows it (even n >= 8),
+    else by red-black SOR sweeps with factor `omega`."""
+    if inner is None:
+        inner = DC_INNER_MULTIGRID if n % 2 == 0 and n >= 8 else DC_INNER_SOR
     g = Gravity(n, n, n, G=1.0)
     g.set_boundary(GRAV_BC_ISOLATED)
     g.set_lmax(lmax)
     g.set_params(omega, 400000, tol)
     g.set_mode(POISSON_FP32_DC)
+    g.set_dc_inner(inner)
     g.set_criterion(SOR_CRIT_REL_RESID)
     return g
 
 
-def build_dataset(n, count, first_seed=0, omega=1.9, tol=1e-8, log=print):
+def build_dataset(n, count, first_seed=0, omega=1.9, tol=1e-8, log=print, inner=None):
     """Generate and solve fields first_seed .. first_seed + count - 1."""
     ds = Dataset(n, count)
-    grav = make_solver(n, omega, tol)
+    grav = make_solver(n, omega, tol, inner=inner)
     t0 = time.perf_counter()
     bad = 0
     for i in range(count):
@@ -212,7 +218,7 @@ def build_dataset(n, count, first_seed=0, omega=1.9, tol=1e-8, log=print):
         bad += st != 0
         if log and (i + 1) % max(1, count // 10) == 0:
             log(f"[nn] fields {i + 1}/{count} solved in {time.perf_counter() - t0:.1f}s "
-                f"(last: {grav.s.last_iterations} sweeps, rel. residual "
+                f"(last: {grav.s.last_iterations} sweeps or V-cycles, rel. residual "
                 f"{grav.s.last_rel_residual:.2e}, unconverged so far {bad})")
     grav.close()
     return ds
@@ -259,11 +265,14 @@ def main(argv=None):
     ap.add_argument("--batch", type=int, default=4)
     ap.add_argument("--lr", type=float, default=2e-3)
     ap.add_argument("--decay", type=float, default=0.5, help="learning-rate factor per epoch")
-    ap.add_argument("--omega", type=float, default=1.9)
+    ap.add_argument("--omega", type=float, default=1.9, help="SOR factor of the sor inner solve")
+    ap.add_argument("--inner", choices=("auto", "sor", "multigrid"), default="auto",
+                    help="inner solve of the training targets' defect correction")
     ap.add_argument("--seed", type=int, default=0)
     ap.add_argument("--out", default=DEFAULT_WEIGHTS)
     a = ap.parse_args(argv)
-    ds = build_dataset(a.n, a.fields, omega=a.omega)
+    inner = {"auto": None, "sor": DC_INNER_SOR, "multigrid": DC_INNER_MULTIGRID}[a.inner]
+    ds = build_dataset(a.n, a.fields, omega=a.omega, inner=inner)
     model = Model(seed=a.seed)
     losses, vmse = train(model, ds, a.epochs, a.batch, a.lr, val=a.val, seed=a.seed,
                          decay=a.decay)
