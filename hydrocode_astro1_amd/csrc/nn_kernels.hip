@@ -54,7 +54,8 @@ constexpr int LX = BX + 2, LY = BY + 2, LZ = BZ + 2;
 constexpr int CONV_MAX_BLOCKS = HYD_CONV_MAX_BLOCKS;
 
 template <int CINP, int NTL>
-__global__ void __launch_bounds__(256) conv_fwd_lds_kernel(ConvArgs a) {
+__global__ void __launch_bounds__(256, (CINP == 12 ? 3 : 1))  // 12: 3 waves per SIMD, no spills
+conv_fwd_lds_kernel(ConvArgs a) {
     constexpr int KSTEPS = (27 * CINP + 15) / 16;
     constexpr int G4 = CINP / 4;  // 4-channel groups per voxel
     __shared__ v4s s_in[LX * LY * LZ * G4];
@@ -79,7 +80,13 @@ __global__ void __launch_bounds__(256) conv_fwd_lds_kernel(ConvArgs a) {
         const int bz = blk % nbz, by = (blk / nbz) % nby, bx = blk / (nbz * nby);
 #pragma unroll
         for (int it = 0; it < PER; ++it) {
-            const int e = threadIdx.x + 256 * it;
+            int e = threadIdx.x + 256 * it;
+            // From 12 channels up the indices are recomputed per brick: kept
+            // live across the brick loop they cost a wave per SIMD (12
+            // channels at 128^3: 119 us at 2 waves, 102 us at 3).  With 4
+            // and 8 channels occupancy does not change and recomputing only
+            // adds VALU work.
+            if constexpr (CINP >= 12) asm volatile("" : "+v"(e));
             const int q = e % G4, vz = (e / G4) % LZ, vy = (e / (G4 * LZ)) % LY,
                       vx = e / (G4 * LZ * LY);
             const int ii = bx * BX - 1 + vx, jj = by * BY - 1 + vy, kz = bz * BZ - 1 + vz;
