@@ -542,7 +542,10 @@ __global__ void moments_finish_kernel(const double* __restrict__ sums, int nq, i
 // neighbours, the centre, and the z neighbour at the same split index), and
 // the loads of plane i + 1's other operands and plane i + 2's centres are
 // issued before plane i is computed, so one plane of loads is in flight.
-constexpr int RES_PLANES = 16;
+#ifndef HYD_RES_PLANES
+#define HYD_RES_PLANES 8  // 16 and 4 measured 4% slower at 256^3 (kernel + final sum)
+#endif
+constexpr int RES_PLANES = HYD_RES_PLANES;
 
 struct ResOperands {  // one colour of one cell: f, y neighbours, the other z neighbour
     double f, ym, yp, zx;
