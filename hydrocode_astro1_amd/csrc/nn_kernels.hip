@@ -266,62 +266,88 @@ __global__ void __launch_bounds__(256) conv_wgrad_lds_kernel(WgradArgs a) {
         xs[XZERO + e] = 0;
     }
 
-    for (int b = blockIdx.x; b < nbricks; b += gridDim.x) {
+    // Staging, one brick ahead: the loads of brick b + gridDim.x are issued
+    // into registers after brick b is in LDS and complete during its MFMAs.
+    // Input: positions (x0-1 .. x0+4) x (y0-1 .. y0+4) x (z0-1 .. z0+16),
+    // 4-channel groups; position z_in goes to copy s at z = z_in - z0 - s + 1.
+    // An input item is one channel group of one (x, y) row and four output z
+    // of every copy: the six input voxels they need, written as one 8-byte
+    // quad of z per (copy, channel).  A gradient item is one channel group of
+    // four consecutive voxels along z, stored transposed (zero outside the
+    // grid).
+    constexpr int NXI = LX * LY * 4 * G4, PXI = (NXI + 255) / 256;
+    constexpr int NGI = 64 * (4 * MT), PGI = (NGI + 255) / 256;
+    v4s ux[PXI][6], ug[PGI][4];
+    auto load_brick = [&](int b) {
         const int bz = b % nbz, by = (b / nbz) % nby, bx = b / (nbz * nby);
         const int x0 = bx * BX, y0 = by * BY, z0 = bz * BZ;
-        __syncthreads();  // the previous brick's reads are done
-        // input: positions (x0-1 .. x0+4) x (y0-1 .. y0+4) x (z0-1 .. z0+16),
-        // 4-channel groups; position z_in goes to copy s at z = z_in - z0 - s + 1.
-        // A thread takes one channel group of one (x, y) row and four output
-        // z of every copy: it reads the six input voxels they need and writes
-        // each (copy, channel) as one 8-byte quad of z.
-        for (int e = threadIdx.x; e < LX * LY * 4 * G4; e += 256) {
+#pragma unroll
+        for (int it = 0; it < PXI; ++it) {
+            const int e = threadIdx.x + 256 * it;
             const int q = e % G4, zq = (e / G4) % 4, vy = (e / (G4 * 4)) % LY,
                       vx = e / (G4 * 4 * LY);
             const int ii = x0 - 1 + vx, jj = y0 - 1 + vy;
-            const bool row = ii >= 0 && ii < n && jj >= 0 && jj < n;
+            const bool row = e < NXI && ii >= 0 && ii < n && jj >= 0 && jj < n;
             const int si = a.up ? ii >> 1 : ii, sj = a.up ? jj >> 1 : jj;
             const long rbase = ((long)si * nin + sj) * nin;
-            v4s u[6];
 #pragma unroll
             for (int t = 0; t < 6; ++t) {
                 const int kz = z0 - 1 + 4 * zq + t;
-                u[t] = v4s{0, 0, 0, 0};
+                ux[it][t] = v4s{0, 0, 0, 0};
                 if (row && kz >= 0 && kz < n)
-                    u[t] = xin[(rbase + (a.up ? kz >> 1 : kz)) * G4 + q];
+                    ux[it][t] = xin[(rbase + (a.up ? kz >> 1 : kz)) * G4 + q];
             }
+        }
+#pragma unroll
+        for (int it = 0; it < PGI; ++it) {
+            const int e = threadIdx.x + 256 * it;
+            const int q = e % (4 * MT), v = 4 * (e / (4 * MT));
+            const int vz = v % BZ, vy = (v / BZ) % BY, vx = v / (BZ * BY);
+            const int i = x0 + vx, j = y0 + vy;
+#pragma unroll
+            for (int t = 0; t < 4; ++t) {
+                const int k = z0 + vz + t;
+                ug[it][t] = v4s{0, 0, 0, 0};
+                if (e < NGI && 4 * q < a.coutp && i < n && j < n && k < n)
+                    ug[it][t] = *reinterpret_cast<const v4s*>(
+                        a.g + (((long)i * n + j) * n + k) * a.coutp + 4 * q);
+            }
+        }
+    };
+    if (blockIdx.x < nbricks) load_brick(blockIdx.x);
+
+    for (int b = blockIdx.x; b < nbricks; b += gridDim.x) {
+        __syncthreads();  // the previous brick's reads are done
+#pragma unroll
+        for (int it = 0; it < PXI; ++it) {
+            const int e = threadIdx.x + 256 * it;
+            if (e >= NXI) continue;
+            const int q = e % G4, zq = (e / G4) % 4, vy = (e / (G4 * 4)) % LY,
+                      vx = e / (G4 * 4 * LY);
             uint16_t* dst = xs + (4 * q * XCH + (vx * LY + vy) * BZ + 4 * zq);
 #pragma unroll
             for (int sft = 0; sft < 3; ++sft)
 #pragma unroll
                 for (int c = 0; c < 4; ++c)
                     *reinterpret_cast<v4s*>(dst + (sft * CINP + c) * XCH) =
-                        v4s{u[sft][c], u[sft + 1][c], u[sft + 2][c], u[sft + 3][c]};
+                        v4s{ux[it][sft][c], ux[it][sft + 1][c], ux[it][sft + 2][c],
+                            ux[it][sft + 3][c]};
         }
-        // gradient of the brick's voxels, transposed; zero outside the grid.
-        // A thread takes one channel group of four consecutive voxels along z.
-        for (int e = threadIdx.x; e < 64 * (4 * MT); e += 256) {
-            const int q = e % (4 * MT), vq = e / (4 * MT);
-            const int v = 4 * vq;
-            const int vz = v % BZ, vy = (v / BZ) % BY, vx = v / (BZ * BY);
-            const int i = x0 + vx, j = y0 + vy;
-            v4s u[4];
 #pragma unroll
-            for (int t = 0; t < 4; ++t) {
-                const int k = z0 + vz + t;
-                u[t] = v4s{0, 0, 0, 0};
-                if (4 * q < a.coutp && i < n && j < n && k < n)
-                    u[t] = *reinterpret_cast<const v4s*>(a.g + (((long)i * n + j) * n + k) * a.coutp +
-                                                         4 * q);
-            }
+        for (int it = 0; it < PGI; ++it) {
+            const int e = threadIdx.x + 256 * it;
+            if (e >= NGI) continue;
+            const int q = e % (4 * MT), v = 4 * (e / (4 * MT));
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
                 const int co = 4 * q + c;
                 *reinterpret_cast<v4s*>(gs + co * GROW + v) =
-                    (co < a.cout) ? v4s{u[0][c], u[1][c], u[2][c], u[3][c]} : v4s{0, 0, 0, 0};
+                    (co < a.cout) ? v4s{ug[it][0][c], ug[it][1][c], ug[it][2][c], ug[it][3][c]}
+                                  : v4s{0, 0, 0, 0};
             }
         }
         __syncthreads();
+        if (b + (int)gridDim.x < nbricks) load_brick(b + gridDim.x);
 #pragma unroll
         for (int ks = 0; ks < 16; ++ks) {  // 16 voxels: (vx, vy) = (ks / 4, ks % 4), z 0..15
             const int vbase = ks * 16 + 4 * g4;
