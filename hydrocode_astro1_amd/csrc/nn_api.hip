@@ -159,7 +159,8 @@ int conv(Model& M, int l, int n_out, const uint16_t* x, const uint16_t* skip, ui
 // Input gradient of layer l: the adjoint convolution of g (gradient with
 // respect to the pre-activation, output grid) into out (output grid; for an
 // upsampling layer this is the gradient of the upsampled input).
-int conv_adj(Model& M, int l, int n_out, const uint16_t* g, uint16_t* out) {
+int conv_adj(Model& M, int l, int n_out, const uint16_t* g, uint16_t* out,
+             const uint16_t* mask = nullptr) {
     const LayerDef& L = kLayers[l];
     ConvArgs a;
     a.x = g;
@@ -176,6 +177,7 @@ int conv_adj(Model& M, int l, int n_out, const uint16_t* g, uint16_t* out) {
     a.cout = L.cin;
     a.coutp = pad4(L.cin);
     a.relu = 0;
+    a.mask = mask;
     HYD_CHECK(conv_forward(a, M.st));
     return 0;
 }
@@ -248,27 +250,24 @@ int backward(Model& M, int n) {
     HYD_CHECK(copy_bf16(A.gd3, v2 * 16, A.ge3, M.st));
     // c4b, c4a
     HYD_CHECK(relu_mask(A.ge4, A.e4, v3 * 24, A.s0, M.st));
-    if (wgrad(M, 7, n3, A.e4a, A.s0) || conv_adj(M, 7, n3, A.s0, A.ge4a)) return -2;
-    HYD_CHECK(relu_mask(A.ge4a, A.e4a, v3 * 24, A.s0, M.st));
-    if (wgrad(M, 6, n3, A.p3, A.s0) || conv_adj(M, 6, n3, A.s0, A.gp3)) return -2;
+    // ge4a: the input gradient of c4b with c4a's ReLU derivative applied
+    if (wgrad(M, 7, n3, A.e4a, A.s0) || conv_adj(M, 7, n3, A.s0, A.ge4a, A.e4a)) return -2;
+    if (wgrad(M, 6, n3, A.p3, A.ge4a) || conv_adj(M, 6, n3, A.ge4a, A.gp3)) return -2;
     HYD_CHECK(unpool_add(A.gp3, n2, 16, A.ge3, M.st));
     // c3b, c3a
     HYD_CHECK(relu_mask(A.ge3, A.e3, v2 * 16, A.s0, M.st));
-    if (wgrad(M, 5, n2, A.e3a, A.s0) || conv_adj(M, 5, n2, A.s0, A.ge3a)) return -2;
-    HYD_CHECK(relu_mask(A.ge3a, A.e3a, v2 * 16, A.s0, M.st));
-    if (wgrad(M, 4, n2, A.p2, A.s0) || conv_adj(M, 4, n2, A.s0, A.gp2)) return -2;
+    if (wgrad(M, 5, n2, A.e3a, A.s0) || conv_adj(M, 5, n2, A.s0, A.ge3a, A.e3a)) return -2;
+    if (wgrad(M, 4, n2, A.p2, A.ge3a) || conv_adj(M, 4, n2, A.ge3a, A.gp2)) return -2;
     HYD_CHECK(unpool_add(A.gp2, n1, 12, A.ge2, M.st));
     // c2b, c2a
     HYD_CHECK(relu_mask(A.ge2, A.e2, v1 * 12, A.s0, M.st));
-    if (wgrad(M, 3, n1, A.e2a, A.s0) || conv_adj(M, 3, n1, A.s0, A.ge2a)) return -2;
-    HYD_CHECK(relu_mask(A.ge2a, A.e2a, v1 * 12, A.s0, M.st));
-    if (wgrad(M, 2, n1, A.p1, A.s0) || conv_adj(M, 2, n1, A.s0, A.gp1)) return -2;
+    if (wgrad(M, 3, n1, A.e2a, A.s0) || conv_adj(M, 3, n1, A.s0, A.ge2a, A.e2a)) return -2;
+    if (wgrad(M, 2, n1, A.p1, A.ge2a) || conv_adj(M, 2, n1, A.ge2a, A.gp1)) return -2;
     HYD_CHECK(unpool_add(A.gp1, n, 8, A.ge1, M.st));
     // c1b, c1a
     HYD_CHECK(relu_mask(A.ge1, A.e1, v0 * 8, A.s0, M.st));
-    if (wgrad(M, 1, n, A.e1a, A.s0) || conv_adj(M, 1, n, A.s0, A.ge1a)) return -2;
-    HYD_CHECK(relu_mask(A.ge1a, A.e1a, v0 * 8, A.s0, M.st));
-    if (wgrad(M, 0, n, A.x0, A.s0)) return -2;
+    if (wgrad(M, 1, n, A.e1a, A.s0) || conv_adj(M, 1, n, A.s0, A.ge1a, A.e1a)) return -2;
+    if (wgrad(M, 0, n, A.x0, A.ge1a)) return -2;
     return 0;
 }
 

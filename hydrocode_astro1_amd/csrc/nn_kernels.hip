@@ -164,6 +164,14 @@ conv_fwd_lds_kernel(ConvArgs a) {
                     }
                     val[i] = t;
                 }
+                if (a.mask) {  // zeroing commutes with the BF16 rounding below
+                    const v2u mk = *reinterpret_cast<const v2u*>(a.mask + v * a.coutp + c0);
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        if (!(bf_lo(mk[h]) > 0.f)) val[2 * h] = 0.f;
+                        if (!(bf_hi(mk[h]) > 0.f)) val[2 * h + 1] = 0.f;
+                    }
+                }
                 const v2u rb = v2u{f2bf2(val[0], val[1]), f2bf2(val[2], val[3])};
                 if (a.yr) *reinterpret_cast<v2u*>(a.yr + v * a.coutp + c0) = rb;
                 if (a.yd) {

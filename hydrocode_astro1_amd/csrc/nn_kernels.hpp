@@ -34,6 +34,9 @@ struct ConvArgs {
     int up;                // 1: input lives on the n/2 grid (nearest upsampling fused)
     int cinp, cout, coutp;
     int relu;
+    // [n^3][coutp] forward activation: yr is zeroed where it is <= 0 (the
+    // ReLU derivative applied to an input gradient), or nullptr
+    const uint16_t* mask = nullptr;
 };
 
 struct WgradArgs {
