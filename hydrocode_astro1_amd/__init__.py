@@ -77,7 +77,7 @@ class GodunovHLLC3D(C.Structure):
         ("gravity_solver", C.POINTER(GravitySolver3D)),
         ("enable_gravity", C.c_bool), ("G_const", C.c_double),
         ("dev", C.c_void_p), ("max_steps", C.c_int), ("output_enable", C.c_bool),
-        ("steps_taken", C.c_long),
+        ("steps_taken", C.c_long), ("ic_status", C.c_int), ("checkpoint_enable", C.c_bool),
     ]
 
 
@@ -114,12 +114,13 @@ EXPORTS = [
     ("solver3d_upload", C.c_int, [_SP]),
     ("solver3d_download", C.c_int, [_SP]),
     ("solver3d_set_state", C.c_int, [_SP, _dp]),
+    ("solver3d_write_checkpoint", C.c_int, [_SP, C.c_char_p, C.c_double, C.c_int]),
+    ("solver3d_read_checkpoint", C.c_int, [_SP, C.c_char_p, _dp, _ip]),
     ("solver3d_step", C.c_int, [_SP]),
     ("solver3d_run_steps", C.c_int, [_SP, C.c_int, _dp]),
     ("solver3d_run", C.c_int, [_SP, C.c_double, C.c_int, _dp, _dp, _ip, C.c_int]),
     ("solver3d_last_timing", C.c_int, [_SP, _dp, _dp]),
     ("solver3d_benchmark_steps", C.c_int, [_SP, C.c_int, _dp, _dp, _dp]),
-    ("hydro_check_division", C.c_long, [_dp, _dp, C.c_long]),
     ("hydro_sequential_sum", C.c_double, [_dp, C.c_long, C.c_int, C.POINTER(C.c_longlong), _ip]),
     ("hydro_fill_initial_state", C.c_int, [C.c_int, C.c_int, C.c_int] + [C.c_double] * 8
      + [C.c_int, C.c_void_p, _dp]),
@@ -153,7 +154,6 @@ EXPORTS = [
     ("nn_dataset_get", C.c_int, [C.c_void_p, C.c_int, _fp, _fp, _dp]),
     ("nn_train", C.c_int, [C.c_void_p, C.c_void_p, _ip, C.c_int, C.c_int, C.c_float, _fp]),
     ("nn_evaluate", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, _fp]),
-    ("nn_probe_mfma", C.c_int, [_fp]),
     ("nn_benchmark", C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _dp]),
     ("nn_forward_flops", C.c_double, [C.c_int]),
 ]
@@ -398,6 +398,20 @@ class Solver:
         _check(self.L.solver3d_download(self.h), "solver3d_download")
         return np.ctypeslib.as_array(self.s.U, shape=(5 * self.n,)).reshape(
             self.shape + (5,)).copy()
+
+    def write_checkpoint(self, path, t=0.0, output_count=0):
+        rc = self.L.solver3d_write_checkpoint(self.h, str(path).encode(), float(t),
+                                              int(output_count))
+        if rc != 0:
+            raise RuntimeError(f"solver3d_write_checkpoint({path}) failed with {rc}")
+
+    def read_checkpoint(self, path):
+        """Load a checkpoint; returns (t, output_count) from its header."""
+        t, n = C.c_double(0.0), C.c_int(0)
+        rc = self.L.solver3d_read_checkpoint(self.h, str(path).encode(), C.byref(t), C.byref(n))
+        if rc != 0:
+            raise RuntimeError(f"solver3d_read_checkpoint({path}) failed with {rc}")
+        return t.value, n.value
 
     def g(self):
         _check(self.L.solver3d_download(self.h), "solver3d_download")

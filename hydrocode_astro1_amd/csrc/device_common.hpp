@@ -19,48 +19,18 @@ namespace hyd {
 __device__ __forceinline__ double tmin(double a, double b) { return (a < b) ? a : b; }
 __device__ __forceinline__ double tmax(double a, double b) { return (a > b) ? a : b; }
 
-// Quotients a / b that share a positive denominator b, with yb = RN(1/b)
-// (an IEEE division 1.0 / b), by one Markstein correction each: q0 = RN(a yb)
-// is within one ulp of a/b, the remainder a - b q0 is exact in a fused
-// multiply-add, and RN(q0 + r yb) is RN(a/b) (Markstein 1990; Muller et al.,
-// Handbook of Floating-Point Arithmetic, the quotient theorem).  The theorem
-// needs the remainder clear of underflow and q0 in range: `ok` turns false
-// for a tiny nonzero dividend or an extreme quotient, and the caller then
-// recomputes the whole group with IEEE divisions, so every result equals
-// a / b bit for bit.  A zero dividend returns itself (b > 0 keeps its sign).
-__device__ __forceinline__ double div_rcp_q(double a, double b, double yb, bool& ok) {
-    const double q0 = a * yb;
-    const double r = fma(-q0, b, a);
-    const double q = fma(r, yb, q0);
-    const double m = fabs(q0), ma = fabs(a);
-    ok = ok && ((ma == 0.0) || (ma >= 0x1p-900 && m >= 0x1p-900 && m <= 0x1p+900));
-    return (ma == 0.0) ? a : q;
-}
-
-// Single quotient with its own IEEE fallback (used by the self-check).
-__device__ __forceinline__ double div_rcp(double a, double b, double yb) {
-    bool ok = true;
-    double q = div_rcp_q(a, b, yb, ok);
-    if (!ok) q = a / b;
-    return q;
-}
-
-// Off by default: on gfx950 the grouped Markstein form plus its range
-// checks issues about as many VALU operations as the IEEE divisions it
-// replaces, and measured slower in the step kernel.
-#ifndef HYD_FASTDIV
-#define HYD_FASTDIV 0
-#endif
 // Keeps the rare IEEE recomputation a real branch: the compiler may not
 // speculate across a volatile asm statement, so the common path never
-// executes the full divisions.
+// executes the full division.
 #define HYD_COLD_PATH() asm volatile("")
 
 // x / d for a divisor d that is a compile-time constant after unrolling (a
 // small integer here), correctly rounded: x (1/d) for a power of two (exact),
-// else the Markstein correction of x RN(1/d) as in div_rcp_q, whose quotient
-// theorem holds while x stays clear of underflow and overflow; outside that
-// range the IEEE division runs.
+// else one Markstein correction of q0 = RN(x RN(1/d)): the remainder x - d q0
+// is exact in a fused multiply-add and RN(q0 + r RN(1/d)) is RN(x/d)
+// (Markstein 1990; Muller et al., Handbook of Floating-Point Arithmetic, the
+// quotient theorem), which holds while x stays clear of underflow and
+// overflow; outside that range the IEEE division runs.
 __device__ __forceinline__ double div_by_const(double x, double d) {
     if (d == 2.0 || d == 4.0 || d == 8.0) return x * (1.0 / d);
     const double y = 1.0 / d;
@@ -87,26 +57,6 @@ __device__ __forceinline__ double rho_floor(double d) { return tmax(d, HYD_RHO_F
 // cons_to_prim_3d, hydro_solver_3d.c:162-175; sound_speed_3d :191-193.
 __device__ __forceinline__ void prims(Cell& q, double gamma) {
     const double r = rho_floor(q.d);
-#if HYD_FASTDIV
-    const double yr = 1.0 / r;
-    bool ok = true;
-    q.u = div_rcp_q(q.mx, r, yr, ok);
-    q.v = div_rcp_q(q.my, r, yr, ok);
-    q.w = div_rcp_q(q.mz, r, yr, ok);
-    double ke = 0.5 * r * (q.u * q.u + q.v * q.v + q.w * q.w);
-    q.p = tmax((gamma - 1.0) * (q.e - ke), HYD_P_FLOOR);
-    double c2 = div_rcp_q(gamma * q.p, r, yr, ok);
-    if (!ok) {
-        HYD_COLD_PATH();
-        q.u = q.mx / r;
-        q.v = q.my / r;
-        q.w = q.mz / r;
-        ke = 0.5 * r * (q.u * q.u + q.v * q.v + q.w * q.w);
-        q.p = tmax((gamma - 1.0) * (q.e - ke), HYD_P_FLOOR);
-        c2 = gamma * q.p / r;
-    }
-    q.c = sqrt(c2);
-#else
     q.u = q.mx / r;
     q.v = q.my / r;
     q.w = q.mz / r;
@@ -114,7 +64,6 @@ __device__ __forceinline__ void prims(Cell& q, double gamma) {
     double p = (gamma - 1.0) * (q.e - ke);
     q.p = tmax(p, HYD_P_FLOOR);
     q.c = sqrt(gamma * q.p / r);
-#endif
 }
 
 // Constants of one step shared by every cell.
@@ -137,29 +86,10 @@ __device__ __forceinline__ EosConst eos_const(double gamma) {
 __device__ __forceinline__ void floor_roundtrip(double& d, double& mx, double& my, double& mz,
                                                 double& e, const EosConst& k) {
     const double r = tmax(d, HYD_RHO_FLOOR);
-#if HYD_FASTDIV
-    const double yr = 1.0 / r;
-    bool ok = true;
-    double u = div_rcp_q(mx, r, yr, ok), v = div_rcp_q(my, r, yr, ok),
-           w = div_rcp_q(mz, r, yr, ok);
-    double ke = 0.5 * r * (u * u + v * v + w * w);
-    double p = tmax(k.gm1 * (e - ke), HYD_P_FLOOR);
-    double ep = div_rcp_q(p, k.gm1, k.ygm1, ok);
-    if (!ok) {
-        HYD_COLD_PATH();
-        u = mx / r;
-        v = my / r;
-        w = mz / r;
-        ke = 0.5 * r * (u * u + v * v + w * w);
-        p = tmax(k.gm1 * (e - ke), HYD_P_FLOOR);
-        ep = p / k.gm1;
-    }
-#else
     const double u = mx / r, v = my / r, w = mz / r;
     const double ke = 0.5 * r * (u * u + v * v + w * w);
     const double p = tmax(k.gm1 * (e - ke), HYD_P_FLOOR);
     const double ep = p / k.gm1;
-#endif
     d = r;
     mx = r * u;
     my = r * v;
@@ -206,29 +136,10 @@ __device__ __forceinline__ double signal_speed(double d, double mx, double my, d
 // The same for a state whose density is already at or above the floor.
 __device__ __forceinline__ double signal_speed_floored(double r, double mx, double my, double mz,
                                                        double e, const EosConst& k) {
-#if HYD_FASTDIV
-    const double yr = 1.0 / r;
-    bool ok = true;
-    double u = div_rcp_q(mx, r, yr, ok), v = div_rcp_q(my, r, yr, ok),
-           w = div_rcp_q(mz, r, yr, ok);
-    double ke = 0.5 * r * (u * u + v * v + w * w);
-    double p = tmax(k.gm1 * (e - ke), HYD_P_FLOOR);
-    double c2 = div_rcp_q(k.gamma * p, r, yr, ok);
-    if (!ok) {
-        HYD_COLD_PATH();
-        u = mx / r;
-        v = my / r;
-        w = mz / r;
-        ke = 0.5 * r * (u * u + v * v + w * w);
-        p = tmax(k.gm1 * (e - ke), HYD_P_FLOOR);
-        c2 = k.gamma * p / r;
-    }
-#else
     const double u = mx / r, v = my / r, w = mz / r;
     const double ke = 0.5 * r * (u * u + v * v + w * w);
     const double p = tmax(k.gm1 * (e - ke), HYD_P_FLOOR);
     const double c2 = k.gamma * p / r;
-#endif
     const double m = tmax(tmax(fabs(u), fabs(v)), fabs(w));
     return m + sqrt(c2);
 }
@@ -236,17 +147,6 @@ __device__ __forceinline__ double signal_speed_floored(double r, double mx, doub
 // Velocities of the gravity source (hydro_solver_3d.c:612-614): no floor.
 __device__ __forceinline__ void velocities(double rho, double mx, double my, double mz, double& u,
                                            double& v, double& w) {
-#if HYD_FASTDIV
-    if (rho > 0.0) {
-        const double y = 1.0 / rho;
-        bool ok = true;
-        u = div_rcp_q(mx, rho, y, ok);
-        v = div_rcp_q(my, rho, y, ok);
-        w = div_rcp_q(mz, rho, y, ok);
-        if (ok) return;
-        HYD_COLD_PATH();
-    }
-#endif
     u = mx / rho;
     v = my / rho;
     w = mz / rho;

@@ -67,7 +67,7 @@ void dev_free(GravDevice* d) {
     void* ptrs[] = {d->phi_r, d->phi_b, d->rhs_r, d->rhs_b, d->gx,  d->gy, d->gz,
                     d->rho_tmp, d->nat_tmp, d->mom, d->mom_scratch, d->mom_seq, d->wlm, d->ctl, d->acc, d->res_part,
                     d->e_r, d->e_b, d->r_r, d->r_b, d->guess_r, d->guess_b,
-                    d->phi2_r, d->phi2_b, d->e2_r, d->e2_b};
+};
     for (void* p : ptrs)
         if (p) (void)hipFree(p);
     for (auto* v : {&d->mg_u, &d->mg_f, &d->mg_q})
@@ -95,7 +95,6 @@ int dev_create(GravitySolver3D* s) {
     d->nsplit = (long)s->nx * s->ny * g.nzh;
     d->stream = hyd_stream();
     if (!d->stream) return -2;
-    if (const char* e = getenv("HYDRO_MI355X_SOR_FUSED")) d->fused = atoi(e) != 0;
     if (const char* e = getenv("HYDRO_MI355X_MG_TAIL")) d->mg_use_tail = atoi(e) != 0;
     if (const char* e = getenv("HYDRO_MI355X_SEQSUM"))
         d->seq_method = strcmp(e, "chain") == 0 ? hyd::SEQSUM_CHAIN : hyd::SEQSUM_COMPOSITE;
@@ -190,24 +189,14 @@ int apply_boundary(GravitySolver3D* s) {
     return 0;
 }
 
-// One launch of the FP64 iteration: a fused pair (buffers in -> out) or a
-// single iteration in place (in == out).  Kept so that the host can tell,
-// after the fact, which buffer holds the state after a given iteration.
-struct SorLaunch {
-    int first;         // first iteration
-    int count;         // 1 or 2
-    int in_b, out_b;   // buffer ids: 0 the solve's starting pair, 1 the spare
-};
-
 // Red-black SOR in FP64 with the reference's iteration structure
 // (gravity_solver_3d.c:566-746): red, black, boundary, check.  The boundary
 // values depend only on the density, so they are written once, after the
-// first iteration, exactly where the reference first writes them.  After
-// that, iterations run either as two colour sweeps in place or, with
-// d->fused, two per pass (sor2) from one copy of the potential into the
-// other; a pass whose first iteration already met the tolerance is undone by
-// recomputing that iteration alone from the pass's input, so the result and
-// the iteration count are those of one-iteration sweeps either way.
+// first iteration, exactly where the reference first writes them.  Each
+// iteration is two colour sweeps in place; under the max-update rule a
+// one-wavefront check before each iteration evaluates the stop rule on the
+// device, and sweeps past convergence return at their first instruction, so
+// the host reads the state back only once per batch of iterations.
 int sor_fp64(GravitySolver3D* s) {
     GravDevice* d = s->dev;
     const SorCoef c = coefficients(s);
@@ -220,50 +209,19 @@ int sor_fp64(GravitySolver3D* s) {
         s->last_residual = 0.0;
         return -1;
     }
-    bool fused = d->fused;
-    if (fused && !d->phi2_r && (dev_alloc(&d->phi2_r, d->nsplit) || dev_alloc(&d->phi2_b, d->nsplit)))
-        return -2;
-    double* buf_r[2] = {d->phi_r, d->phi2_r};
-    double* buf_b[2] = {d->phi_b, d->phi2_b};
-    int cur = 0;
-    std::vector<SorLaunch> launches;
-
-    auto single = [&](int it) -> int {
-        // the check before a single iteration clears the slots of the
-        // iteration after it; only a fused pair leaves the slots of the next
-        // iteration to be cleared
-        if (!rel && !launches.empty() && launches.back().count == 2)
-            HYD_CHECK(hyd::launch_sor_prepare(d->ctl, it, d->stream));
+    auto iteration = [&](int it) -> int {
         // the stop rule of iteration it, evaluated once for both sweeps
         if (!rel) HYD_CHECK(hyd::launch_sor_check(d->ctl, it, tol, d->stream));
-        HYD_CHECK(hyd::launch_sor_sweep(d->g, c, 0, it, stol, buf_r[cur], buf_b[cur], d->rhs_r,
+        HYD_CHECK(hyd::launch_sor_sweep(d->g, c, 0, it, stol, d->phi_r, d->phi_b, d->rhs_r,
                                         d->rhs_b, d->ctl, d->stream));
-        HYD_CHECK(hyd::launch_sor_sweep(d->g, c, 1, it, stol, buf_r[cur], buf_b[cur], d->rhs_r,
+        HYD_CHECK(hyd::launch_sor_sweep(d->g, c, 1, it, stol, d->phi_r, d->phi_b, d->rhs_r,
                                         d->rhs_b, d->ctl, d->stream));
-        launches.push_back({it, 1, cur, cur});
         return 0;
     };
     // iteration 0, then the boundary values
-    if (single(0)) return -2;
-    d->phi_r = buf_r[cur];
-    d->phi_b = buf_b[cur];
+    if (iteration(0)) return -2;
     if (apply_boundary(s)) return -2;
     int it = 1;
-    if (fused && s->max_iterations > 1) {
-        // the spare copy carries the same boundary values
-        HYD_CHECK(hipMemcpyAsync(buf_r[1], buf_r[0], sizeof(double) * d->nsplit,
-                                 hipMemcpyDeviceToDevice, d->stream));
-        HYD_CHECK(hipMemcpyAsync(buf_b[1], buf_b[0], sizeof(double) * d->nsplit,
-                                 hipMemcpyDeviceToDevice, d->stream));
-    }
-    auto finish = [&](int keep) {  // buffer `keep` becomes the solver's potential
-        d->phi_r = buf_r[keep];
-        d->phi_b = buf_b[keep];
-        if (fused) {
-            d->phi2_r = buf_r[1 - keep];
-            d->phi2_b = buf_b[1 - keep];
-        }
-    };
     // Under the max-update rule a warm start usually needs about as many
     // iterations as the last solve, and iterations past convergence return
     // at once: one batch of that size ends most solves with a single host
@@ -275,23 +233,9 @@ int sor_fp64(GravitySolver3D* s) {
     int it_prev = 0;
     while (true) {
         const int nb = std::min(batch, s->max_iterations - it);
-        for (int q = 0; q < nb;) {
-            if (fused && nb - q >= 2) {
-                HYD_CHECK(hyd::launch_sor2(d->g, c, buf_r[cur], buf_b[cur], buf_r[1 - cur],
-                                           buf_b[1 - cur], d->rhs_r, d->rhs_b,
-                                           rel ? nullptr : d->ctl, it, tol, d->stream));
-                launches.push_back({it, 2, cur, 1 - cur});
-                cur = 1 - cur;
-                it += 2;
-                q += 2;
-            } else {
-                if (single(it)) return -2;
-                ++it;
-                ++q;
-            }
-        }
+        for (int q = 0; q < nb; ++q, ++it)
+            if (iteration(it)) return -2;
         if (rel) {
-            finish(cur);
             double r;
             if (rel_residual_dev(s, nullptr, nullptr, &r)) return -2;
             s->last_rel_residual = r;
@@ -319,34 +263,10 @@ int sor_fp64(GravitySolver3D* s) {
         HYD_CHECK(hipStreamSynchronize(d->stream));
         const SorCtl& h = *d->ctl_host;
         if (h.converged) {
-            const int det = h.iters;  // first iteration that did not run
-            const double rm2 = det >= 2 ? bitsd(hyd::sor_slot_max(h, (det - 2) & 3)) : 1.0;
-            // the launch that ran iteration det - 1
-            const SorLaunch* L = nullptr;
-            for (auto r = launches.rbegin(); r != launches.rend(); ++r)
-                if (r->first <= det - 1 && det - 1 < r->first + r->count) {
-                    L = &*r;
-                    break;
-                }
-            if (!L) return -2;
-            if (L->count == 2 && L->first == det - 2 && rm2 < tol) {
-                // iteration det - 2 met tol inside a pair: redo it alone
-                cur = L->in_b;
-                HYD_CHECK(hyd::launch_sor_sweep(d->g, c, 0, det - 2, -1.0, buf_r[cur], buf_b[cur],
-                                                d->rhs_r, d->rhs_b, d->ctl, d->stream));
-                HYD_CHECK(hyd::launch_sor_sweep(d->g, c, 1, det - 2, -1.0, buf_r[cur], buf_b[cur],
-                                                d->rhs_r, d->rhs_b, d->ctl, d->stream));
-                finish(cur);
-                s->last_iterations = det - 1;
-                s->last_residual = rm2;
-                return 0;
-            }
-            finish(L->out_b);
-            s->last_iterations = det;
+            s->last_iterations = h.iters;  // first iteration that did not run
             s->last_residual = h.last_res;
             return 0;
         }
-        finish(cur);
         const double r1 = bitsd(hyd::sor_slot_max(h, (it - 1) & 3));
         if (it >= s->max_iterations) {
             s->last_iterations = it;
@@ -355,17 +275,12 @@ int sor_fp64(GravitySolver3D* s) {
         }
         const double r2 = it >= 2 ? bitsd(hyd::sor_slot_max(h, (it - 2) & 3)) : 0.0;
         batch = next_batch(r1, r2, tol, batch);
-        launches.erase(launches.begin(),
-                       launches.end() - std::min<size_t>(launches.size(), 2));
     }
 }
 
 int ensure_dc_buffers(GravDevice* d) {
     if (!d->e_r && (dev_alloc(&d->e_r, d->nsplit) || dev_alloc(&d->e_b, d->nsplit) ||
                     dev_alloc(&d->r_r, d->nsplit) || dev_alloc(&d->r_b, d->nsplit)))
-        return -2;
-    // the second correction copy keeps e = 0 on the boundary for good
-    if (d->fused && !d->e2_r && (dev_alloc(&d->e2_r, d->nsplit) || dev_alloc(&d->e2_b, d->nsplit)))
         return -2;
     return 0;
 }
@@ -384,20 +299,11 @@ int sor_fp32_dc(GravitySolver3D* s) {
     const double tol = s->tolerance;
     for (int outer = 0; outer < 64 && rel > tol && sweeps < s->max_iterations; ++outer) {
         const int nk = std::min(k, s->max_iterations - sweeps);
-        for (int q = 0; q < nk;) {
-            if (d->fused && nk - q >= 2) {
-                HYD_CHECK(hyd::launch_sor2_f32(d->g, c, d->e_r, d->e_b, d->e2_r, d->e2_b, d->r_r,
-                                               d->r_b, d->stream));
-                std::swap(d->e_r, d->e2_r);
-                std::swap(d->e_b, d->e2_b);
-                q += 2;
-            } else {
-                HYD_CHECK(hyd::launch_sor_sweep_f32(d->g, c, 0, d->e_r, d->e_b, d->r_r, d->r_b,
-                                                    d->stream));
-                HYD_CHECK(hyd::launch_sor_sweep_f32(d->g, c, 1, d->e_r, d->e_b, d->r_r, d->r_b,
-                                                    d->stream));
-                ++q;
-            }
+        for (int q = 0; q < nk; ++q) {
+            HYD_CHECK(hyd::launch_sor_sweep_f32(d->g, c, 0, d->e_r, d->e_b, d->r_r, d->r_b,
+                                                d->stream));
+            HYD_CHECK(hyd::launch_sor_sweep_f32(d->g, c, 1, d->e_r, d->e_b, d->r_r, d->r_b,
+                                                d->stream));
         }
         sweeps += nk;
         HYD_CHECK(hyd::launch_correct(d->g, d->phi_r, d->phi_b, d->e_r, d->e_b, d->stream));
@@ -662,9 +568,12 @@ int potential_device(GravitySolver3D* s, const double* d_rho) {
     HYD_CHECK(hipEventElapsedTime(&ms, d->ev0, d->ev1));
     s->last_solve_ms = ms;
     s->last_status = st;
-    if (st == -1)  // gravity_solver_3d.c:740-742
+    if (st == -1) {  // gravity_solver_3d.c:733-735
         fprintf(stderr, "Warning: Gravity solver did not converge in %d iterations.\n",
                 s->last_iterations);
+        fprintf(stderr, "         Final residual: %.3e (tolerance: %.3e)\n", s->last_residual,
+                s->tolerance);
+    }
     return st;
 }
 
@@ -934,6 +843,26 @@ int grav_internal_solve(GravitySolver3D* s, const double* d_rho, hipStream_t) {
     int st = potential_device(s, d_rho);
     if (st == 0 && accel_device(s)) return -2;
     return st;
+}
+
+int grav_internal_restore(GravitySolver3D* s, const double* phi, const double* gx,
+                          const double* gy, const double* gz) {
+    GravDevice* d = s->dev;
+    const size_t b = sizeof(double) * d->n;
+    if (phi) {
+        if (!d->nat_tmp && dev_alloc(&d->nat_tmp, d->n)) return -2;
+        HYD_CHECK(hipMemcpyAsync(d->nat_tmp, phi, b, hipMemcpyHostToDevice, d->stream));
+        HYD_CHECK(hyd::launch_natural_to_split(d->g, d->nat_tmp, 1.0, d->phi_r, d->phi_b,
+                                               d->stream));
+    }
+    if (gx && gy && gz) {
+        HYD_CHECK(hipMemcpyAsync(d->gx, gx, b, hipMemcpyHostToDevice, d->stream));
+        HYD_CHECK(hipMemcpyAsync(d->gy, gy, b, hipMemcpyHostToDevice, d->stream));
+        HYD_CHECK(hipMemcpyAsync(d->gz, gz, b, hipMemcpyHostToDevice, d->stream));
+        d->g_valid = true;
+    }
+    HYD_CHECK(hipStreamSynchronize(d->stream));
+    return 0;
 }
 
 const double* grav_internal_g(GravitySolver3D* s, int comp) {

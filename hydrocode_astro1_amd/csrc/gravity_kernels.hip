@@ -62,13 +62,6 @@ __device__ __forceinline__ bool sor_stop_check(SorCtl* ctl, int it, double tol, 
         prev = bitsd(sor_wave_slot_max(ctl, (it - 1) & 3));
         stop = prev < tol;
     }
-    if (!stop && it > 1) {
-        const double prev2 = bitsd(sor_wave_slot_max(ctl, (it - 2) & 3));
-        if (prev2 < tol) {
-            stop = true;
-            prev = prev2;
-        }
-    }
     if (stop && writer) {
         ctl->converged = 1;
         ctl->iters = it;
@@ -120,15 +113,6 @@ __global__ void sor_reset_kernel(SorCtl* ctl) {
         ctl->converged = 0;
         ctl->iters = 0;
         ctl->last_res = 0.0;
-    }
-}
-
-// one wavefront
-__global__ void sor_prepare_kernel(SorCtl* ctl, int it) {
-    if (ctl->converged) return;
-    for (int s = threadIdx.x; s < SOR_SLOTS; s += 64) {
-        ctl->slot[it & 3][s][0] = 0ull;
-        ctl->slot[(it + 1) & 3][s][0] = 0ull;
     }
 }
 
@@ -895,11 +879,6 @@ hipError_t launch_sor_sweep(const GridDims& g, const SorCoef& c, int colour, int
     const double* rhs = colour ? rhs_b : rhs_r;
     hipLaunchKernelGGL(sor_sweep_kernel, sor_grid(g), dim3(256), 0, st, g, c, colour, it, tol,
                        mine, other, rhs, ctl);
-    return hipGetLastError();
-}
-
-hipError_t launch_sor_prepare(SorCtl* ctl, int it, hipStream_t st) {
-    hipLaunchKernelGGL(sor_prepare_kernel, dim3(1), dim3(64), 0, st, ctl, it);
     return hipGetLastError();
 }
 

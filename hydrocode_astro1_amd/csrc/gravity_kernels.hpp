@@ -22,11 +22,10 @@ struct GridDims {
 // over SOR_SLOTS slots of slot[t & 3]; a block adds its maximum to slot
 // (block & (SOR_SLOTS - 1)), each slot on its own 128-byte line, so the
 // atomics of one sweep spread over SOR_SLOTS addresses.  Iteration t runs
-// only while the residuals of iterations t-1 and t-2 are both >= tol: a check
-// kernel before the single sweeps of t (or the fused pass itself) evaluates
-// that rule, clears the slots about to be filled, and on the first residual
-// below tol sets converged and iters = t; the host works out which iteration
-// met the tolerance (t-1, or t-2 when t-2 and t-1 ran as a pair).
+// only while the residual of iteration t-1 is >= tol: a check kernel before
+// the two colour sweeps of t evaluates that rule, clears the slots of t+1,
+// and on the first residual below tol sets converged and iters = t (the
+// reference's iteration count, gravity_solver_3d.c:714-727).
 constexpr int SOR_SLOTS = 256;
 struct SorCtl {
     int converged;
@@ -114,22 +113,8 @@ hipError_t launch_sor_sweep(const GridDims& g, const SorCoef& c, int colour, int
 hipError_t launch_sor_boundary(const GridDims& g, int bc, double G, int lmax, const double* mom,
                                const double* wlm, double* phi_r, double* phi_b, hipStream_t st);
 hipError_t launch_sor_reset(SorCtl* ctl, hipStream_t st);
-// Clear the residual slots of iterations it and it + 1 (unless converged).
-hipError_t launch_sor_prepare(SorCtl* ctl, int it, hipStream_t st);
 // The stop check of iteration it alone (after the last launch of a batch).
 hipError_t launch_sor_check(SorCtl* ctl, int it, double tol, hipStream_t st);
-// Two iterations in one pass, out of place: (in_r, in_b) -> (out_r, out_b)
-// on the interior (boundary cells of out must already hold the boundary
-// values).  With ctl, the residuals of iterations it and it + 1 go to their
-// slots and the pass is skipped once an earlier iteration met tol.
-hipError_t launch_sor2(const GridDims& g, const SorCoef& c, const double* in_r,
-                       const double* in_b, double* out_r, double* out_b, const double* rhs_r,
-                       const double* rhs_b, SorCtl* ctl, int it, double tol, hipStream_t st);
-// The same for the FP32 correction equation L e = r.
-hipError_t launch_sor2_f32(const GridDims& g, const SorCoef& c, const float* in_r,
-                           const float* in_b, float* out_r, float* out_b, const float* rhs_r,
-                           const float* rhs_b, hipStream_t st);
-
 // g = -grad phi with the boundary copies of gravity_solver_3d.c:820-897.
 hipError_t launch_acceleration(const GridDims& g, const double* phi_r, const double* phi_b,
                                double* gx, double* gy, double* gz, hipStream_t st);

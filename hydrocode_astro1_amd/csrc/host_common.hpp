@@ -55,12 +55,6 @@ struct GravDevice {
     double *acc_host = nullptr;       // pinned readback
     float *e_r = nullptr, *e_b = nullptr, *r_r = nullptr, *r_b = nullptr;
     double *guess_r = nullptr, *guess_b = nullptr;  // learned guess (split layout)
-    double *phi2_r = nullptr, *phi2_b = nullptr;    // second potential for fused passes
-    float *e2_r = nullptr, *e2_b = nullptr;         // second correction for fused passes
-    // two SOR iterations per pass (sor2), HYDRO_MI355X_SOR_FUSED=1; off by
-    // default: at 256^3 it measured 107 us per FP64 iteration against 105 for
-    // two colour sweeps, and 82 against 70 in FP32
-    bool fused = false;
     bool owns_nn = false;  // the network was created from an input deck
     // multigrid inner solve of the defect correction (DC_INNER_MULTIGRID):
     // pre- and post-smoothing sweeps per level and their SOR factor; V(1,1)
@@ -81,6 +75,8 @@ struct GravDevice {
     bool g_valid = false;  // device g has been computed at least once
 };
 
+struct OutputPipe;  // hydro_io.hip: snapshot, copy stream and writer thread
+
 // Device context of a GodunovHLLC3D.
 struct HydroDevice {
     int nx = 0, ny = 0, nz = 0;
@@ -100,6 +96,7 @@ struct HydroDevice {
     double last_hydro_ms = 0.0;
     hipEvent_t record_ev[3] = {nullptr, nullptr, nullptr};  // benchmark brackets
     bool skip_gravity = false;  // gravity already solved by the caller
+    OutputPipe *out = nullptr;  // asynchronous output of the run loops
     static constexpr int DT_LOG = 1 << 16;
 };
 
@@ -112,3 +109,17 @@ int nn_guess_device(void *model, const double *d_rho, int n, const double *d_mom
 struct GravitySolver3D;
 int grav_internal_solve(struct GravitySolver3D *s, const double *d_rho, hipStream_t st);
 const double *grav_internal_g(struct GravitySolver3D *s, int comp);
+// Device potential (natural order) and g from host arrays; NULL leaves that
+// part as it is.  The next solve warm-starts from the potential, and g is
+// what a step uses when its solve does not converge.
+int grav_internal_restore(struct GravitySolver3D *s, const double *phi, const double *gx,
+                          const double *gy, const double *gz);
+
+// Between the hydro driver (hydro_api.hip) and its files (hydro_io.hip).
+struct GodunovHLLC3D;
+int hyd_download(struct GodunovHLLC3D *s);  // device state (and g) to the host mirror
+int hyd_output_async(struct GodunovHLLC3D *s, int count);
+void hyd_output_drain(HydroDevice *d);  // wait for the pending output files
+void hyd_output_free(HydroDevice *d);
+// HYDIC001 / HYDCK001 file into s; t and count from a checkpoint (else 0).
+int hyd_read_state_file(struct GodunovHLLC3D *s, const char *path, double *t, int *count);

@@ -10,6 +10,8 @@
 #include <sys/stat.h>
 #include <sys/types.h>
 
+#include <algorithm>
+
 #include "../../include/hydro_mi355x.h"
 #include "host_common.hpp"
 
@@ -30,32 +32,9 @@ hipStream_t hyd_stream() {
 
 namespace {
 
-// cons_to_prim_3d, hydro_solver_3d.c:162-175 (host side, for the writers)
-Primitive3D c2p(const State3D& U, double gamma) {
-    Primitive3D P;
-    P.rho = (U.rho > RHO_FLOOR) ? U.rho : RHO_FLOOR;
-    P.u = U.rhou / P.rho;
-    P.v = U.rhov / P.rho;
-    P.w = U.rhow / P.rho;
-    double ke = 0.5 * P.rho * (P.u * P.u + P.v * P.v + P.w * P.w);
-    P.p = (gamma - 1.0) * (U.E - ke);
-    P.p = (P.p > P_FLOOR) ? P.p : P_FLOOR;
-    return P;
-}
-// prim_to_cons_3d, hydro_solver_3d.c:177-189
-State3D p2c(const Primitive3D& P, double gamma) {
-    State3D U;
-    U.rho = P.rho;
-    U.rhou = P.rho * P.u;
-    U.rhov = P.rho * P.v;
-    U.rhow = P.rho * P.w;
-    double ke = 0.5 * P.rho * (P.u * P.u + P.v * P.v + P.w * P.w);
-    U.E = P.p / (gamma - 1.0) + ke;
-    return U;
-}
-
 void dev_free(HydroDevice* d) {
     if (!d) return;
+    hyd_output_free(d);
     for (int b = 0; b < 2; ++b)
         for (int f = 0; f < 5; ++f)
             if (d->U[b][f]) (void)hipFree(d->U[b][f]);
@@ -185,6 +164,21 @@ int read_dt(GodunovHLLC3D* s, long step, double* dt) {
     return 0;
 }
 
+// dt of `count` consecutive steps from `first` (count <= DT_LOG): one or two
+// contiguous pieces of the ring.
+int read_dt_range(GodunovHLLC3D* s, long first, int count, double* out) {
+    HydroDevice* d = s->dev;
+    const long head = first % HydroDevice::DT_LOG;
+    const long a = std::min<long>(count, HydroDevice::DT_LOG - head);
+    HYD_CHECK(hipMemcpyAsync(out, d->dt_log + head, sizeof(double) * a, hipMemcpyDeviceToHost,
+                             d->stream));
+    if (count > a)
+        HYD_CHECK(hipMemcpyAsync(out + a, d->dt_log, sizeof(double) * (count - a),
+                                 hipMemcpyDeviceToHost, d->stream));
+    HYD_CHECK(hipStreamSynchronize(d->stream));
+    return 0;
+}
+
 int ensure_uploaded(GodunovHLLC3D* s) {
     if (!s->dev->uploaded) return upload(s);
     return 0;
@@ -192,9 +186,23 @@ int ensure_uploaded(GodunovHLLC3D* s) {
 
 void mkdir_output() { mkdir("output", 0755); }
 
-// run_simulation_3d / run_simulation_3d_restart, hydro_solver_3d.c:2241-2330
+void write_checkpoint_file(GodunovHLLC3D* s, double t, int count) {
+    char name[256];
+    snprintf(name, sizeof name, "output/checkpoint_%06d.bin", count);
+    (void)solver3d_write_checkpoint(s, name, t, count);
+}
+
+// run_simulation_3d / run_simulation_3d_restart, hydro_solver_3d.c:2241-2330.
+// The output at the start comes from the host mirror (it may hold g read
+// from a restart file); outputs after steps are device snapshots written
+// while the steps go on; the final one follows the last download.
 void run_loop(GodunovHLLC3D* s, double t_start, double t_end, double output_dt, int output_count,
               double next_output_time) {
+    if (s->ic_status != 0) {
+        fprintf(stderr,
+                "Error: the initial state was not loaded (ic_file); the run does not start.\n");
+        return;
+    }
     double t = t_start;
     int step_counter = 0;
     if (upload(s)) return;
@@ -202,18 +210,18 @@ void run_loop(GodunovHLLC3D* s, double t_start, double t_end, double output_dt, 
         mkdir_output();
         write_output_to_file_3d(s, output_count);
         write_output_vtk(s, output_count);
+        if (s->checkpoint_enable) write_checkpoint_file(s, t, output_count);
     }
     output_count++;
     while (t < t_end) {
-        if (step_device(s) < -1) return;
-        if (read_dt(s, s->dev->step - 1, &s->dt)) return;
+        if (step_device(s) < -1) break;
+        if (read_dt(s, s->dev->step - 1, &s->dt)) break;
         t += s->dt;
         step_counter++;
         if (t >= next_output_time) {
             if (s->output_enable) {
-                if (download(s)) return;
-                write_output_to_file_3d(s, output_count);
-                write_output_vtk(s, output_count);
+                if (hyd_output_async(s, output_count)) break;
+                if (s->checkpoint_enable) write_checkpoint_file(s, t, output_count);
             }
             printf("Step %d, t = %.5f, dt = %.5e, Output %d\n", step_counter, t, s->dt, output_count);
             output_count++;
@@ -222,42 +230,19 @@ void run_loop(GodunovHLLC3D* s, double t_start, double t_end, double output_dt, 
         if (step_counter % 10 == 0) printf("Step %d, t = %.5f, dt = %.5e\n", step_counter, t, s->dt);
         if (s->max_steps > 0 && step_counter >= s->max_steps) break;
     }
+    hyd_output_drain(s->dev);
     if (download(s)) return;
     if (s->output_enable && t - (next_output_time - output_dt) > 1e-6) {
         write_output_to_file_3d(s, output_count);
         write_output_vtk(s, output_count);
+        if (s->checkpoint_enable) write_checkpoint_file(s, t, output_count);
     }
     printf("Simulation completed: %d steps, final time: %.5f\n", step_counter, t);
 }
 
-int load_ic_file(GodunovHLLC3D* s, const char* path) {
-    FILE* fp = fopen(path, "rb");
-    if (!fp) {
-        fprintf(stderr, "Error: Could not open initial-state file '%s'\n", path);
-        return -1;
-    }
-    char magic[8];
-    int dims[3];
-    if (fread(magic, 1, 8, fp) != 8 || memcmp(magic, "HYDIC001", 8) != 0 ||
-        fread(dims, sizeof(int), 3, fp) != 3 || dims[0] != s->nx || dims[1] != s->ny ||
-        dims[2] != s->nz) {
-        fprintf(stderr, "Error: '%s' is not an initial-state file for a %d x %d x %d grid\n", path,
-                s->nx, s->ny, s->nz);
-        fclose(fp);
-        return -1;
-    }
-    const long n = (long)s->nx * s->ny * s->nz;
-    const size_t got = fread(s->U, sizeof(State3D), (size_t)n, fp);
-    fclose(fp);
-    if ((long)got != n) {
-        fprintf(stderr, "Error: '%s' is truncated\n", path);
-        return -1;
-    }
-    s->dev->uploaded = false;
-    return 0;
-}
-
 }  // namespace
+
+int hyd_download(GodunovHLLC3D* s) { return download(s); }
 
 extern "C" {
 
@@ -340,6 +325,7 @@ GodunovHLLC3D* solver3d_create_from_deck(const InputDeck* deck) {
     s->bc_zmin = deck->bc_zmin; s->bc_zmax = deck->bc_zmax;
     s->max_steps = deck->max_steps;
     s->output_enable = deck->output_enable;
+    s->checkpoint_enable = deck->checkpoint;
     if (s->gravity_solver) {
         gravity_set_solver_params(s->gravity_solver, deck->sor_omega, deck->sor_max_iter,
                                   deck->sor_tol);
@@ -372,6 +358,7 @@ GodunovHLLC3D* solver3d_create_from_deck(const InputDeck* deck) {
 void set_initial_conditions_3d(GodunovHLLC3D* s, ProblemType problem) {
     s->problem = problem;
     s->dev->uploaded = false;
+    s->ic_status = 0;
     if (hydro_fill_initial_state(s->nx, s->ny, s->nz, s->xmin, s->xmax, s->ymin, s->ymax, s->zmin,
                                  s->zmax, s->gamma, s->G_const, (int)problem, NULL, s->U) != 0)
         fprintf(stderr,
@@ -380,12 +367,17 @@ void set_initial_conditions_3d(GodunovHLLC3D* s, ProblemType problem) {
                 (int)problem);
 }
 
+// With an ic_file the state comes from that file alone: when it cannot be
+// loaded U is left as it was, ic_status is set to -1 and the run loops refuse
+// to start (the problem generator does not stand in for the file).
 void set_initial_conditions_3d_from_deck(GodunovHLLC3D* s, const InputDeck* deck) {
-    if (deck->ic_file[0]) {
-        if (load_ic_file(s, deck->ic_file) == 0) return;
-    }
     s->problem = deck->problem;
+    if (deck->ic_file[0]) {
+        s->ic_status = hyd_read_state_file(s, deck->ic_file, nullptr, nullptr) == 0 ? 0 : -1;
+        return;
+    }
     s->dev->uploaded = false;
+    s->ic_status = 0;
     if (hydro_fill_initial_state(s->nx, s->ny, s->nz, s->xmin, s->xmax, s->ymin, s->ymax, s->zmin,
                                  s->zmax, s->gamma, s->G_const, (int)deck->problem, deck,
                                  s->U) != 0)
@@ -405,125 +397,8 @@ void run_simulation_3d_restart(GodunovHLLC3D* s, double t_start, double t_end, d
                                int start_output_count) {
     printf("Restarting 3D simulation from t=%.5f to t=%.5f\n", t_start, t_end);
     printf("Starting output count: %d\n", start_output_count);
+    printf("Grid: %d x %d x %d = %d cells\n", s->nx, s->ny, s->nz, s->nx * s->ny * s->nz);
     run_loop(s, t_start, t_end, output_dt, start_output_count, t_start + output_dt);
-}
-
-void write_output_to_file_3d(const GodunovHLLC3D* s, int step) {
-    // hydro_solver_3d.c:2336-2372, from the host mirror
-    char filename[256];
-    snprintf(filename, sizeof filename, "output/output_%06d.csv", step);
-    FILE* fp = fopen(filename, "w");
-    if (!fp) {
-        fprintf(stderr, "Error: Could not open file %s for writing.\n", filename);
-        return;
-    }
-    fprintf(fp, "x,y,z,rho,u,v,w,p,gx,gy,gz\n");
-    int skip = 1;
-    if (s->nx > 128) skip = 2;
-    if (s->nx > 256) skip = 4;
-    for (int i = 0; i < s->nx; i += skip)
-        for (int j = 0; j < s->ny; j += skip)
-            for (int k = 0; k < s->nz; k += skip) {
-                const long idx = IDX3D(i, j, k, (long)s->ny, s->nz);
-                const double x = s->xmin + (i + 0.5) * s->dx;
-                const double y = s->ymin + (j + 0.5) * s->dy;
-                const double z = s->zmin + (k + 0.5) * s->dz;
-                Primitive3D P = c2p(s->U[idx], s->gamma);
-                const double gx = s->enable_gravity ? s->gx[idx] : 0.0;
-                const double gy = s->enable_gravity ? s->gy[idx] : 0.0;
-                const double gz = s->enable_gravity ? s->gz[idx] : 0.0;
-                fprintf(fp, "%.6f,%.6f,%.6f,%.6e,%.6e,%.6e,%.6e,%.6e,%.6e,%.6e,%.6e\n", x, y, z,
-                        P.rho, P.u, P.v, P.w, P.p, gx, gy, gz);
-            }
-    fclose(fp);
-}
-
-void write_output_vtk(const GodunovHLLC3D* s, int step) {
-    // hydro_solver_3d.c:2376-2434 (legacy ASCII structured points)
-    char filename[256];
-    snprintf(filename, sizeof filename, "output/output_%06d.vtk", step);
-    FILE* fp = fopen(filename, "w");
-    if (!fp) {
-        fprintf(stderr, "Error: Could not open file %s for writing.\n", filename);
-        return;
-    }
-    const long n = (long)s->nx * s->ny * s->nz;
-    fprintf(fp, "# vtk DataFile Version 3.0\n3D Hydro Simulation Output\nASCII\n");
-    fprintf(fp, "DATASET STRUCTURED_POINTS\n");
-    fprintf(fp, "DIMENSIONS %d %d %d\n", s->nx, s->ny, s->nz);
-    fprintf(fp, "ORIGIN %f %f %f\n", s->xmin, s->ymin, s->zmin);
-    fprintf(fp, "SPACING %f %f %f\n", s->dx, s->dy, s->dz);
-    fprintf(fp, "POINT_DATA %ld\n", n);
-    fprintf(fp, "SCALARS density float 1\nLOOKUP_TABLE default\n");
-    for (long c = 0; c < n; ++c) fprintf(fp, "%e\n", c2p(s->U[c], s->gamma).rho);
-    fprintf(fp, "SCALARS pressure float 1\nLOOKUP_TABLE default\n");
-    for (long c = 0; c < n; ++c) fprintf(fp, "%e\n", c2p(s->U[c], s->gamma).p);
-    fprintf(fp, "VECTORS velocity float\n");
-    for (long c = 0; c < n; ++c) {
-        Primitive3D P = c2p(s->U[c], s->gamma);
-        fprintf(fp, "%e %e %e\n", P.u, P.v, P.w);
-    }
-    fclose(fp);
-}
-
-int load_from_csv_restart(GodunovHLLC3D* s, const char* filename) {
-    // hydro_solver_3d.c:2139-2235
-    FILE* fp = fopen(filename, "r");
-    if (!fp) {
-        fprintf(stderr, "Error: Could not open restart file '%s'\n", filename);
-        return -1;
-    }
-    printf("Loading restart data from: %s\n", filename);
-    char line[1024];
-    if (!fgets(line, sizeof line, fp)) {
-        fprintf(stderr, "Error: Could not read header from restart file\n");
-        fclose(fp);
-        return -1;
-    }
-    if (strstr(line, "rho") == NULL || strstr(line, "u") == NULL) {
-        fprintf(stderr, "Error: Invalid header format in restart file\n");
-        fprintf(stderr, "Expected: x,y,z,rho,u,v,w,p,gx,gy,gz\n");
-        fclose(fp);
-        return -1;
-    }
-    const long ncells = (long)s->nx * s->ny * s->nz;
-    memset(s->U, 0, sizeof(State3D) * ncells);
-    long points = 0;
-    double x, y, z, rho, u, v, w, p, gx, gy, gz;
-    while (fgets(line, sizeof line, fp)) {
-        int parsed = sscanf(line, "%lf,%lf,%lf,%lf,%lf,%lf,%lf,%lf,%lf,%lf,%lf", &x, &y, &z, &rho,
-                            &u, &v, &w, &p, &gx, &gy, &gz);
-        if (parsed < 8) {
-            parsed = sscanf(line, "%lf,%lf,%lf,%lf,%lf,%lf,%lf,%lf", &x, &y, &z, &rho, &u, &v, &w, &p);
-            gx = gy = gz = 0.0;
-        }
-        if (parsed < 8) continue;
-        const int i = (int)((x - s->xmin) / s->dx);
-        const int j = (int)((y - s->ymin) / s->dy);
-        const int k = (int)((z - s->zmin) / s->dz);
-        if (i >= 0 && i < s->nx && j >= 0 && j < s->ny && k >= 0 && k < s->nz) {
-            const long idx = IDX3D(i, j, k, (long)s->ny, s->nz);
-            Primitive3D P = {rho, u, v, w, p};
-            s->U[idx] = p2c(P, s->gamma);
-            if (s->enable_gravity && parsed >= 11) {
-                s->gx[idx] = gx;
-                s->gy[idx] = gy;
-                s->gz[idx] = gz;
-            }
-            points++;
-        }
-    }
-    fclose(fp);
-    printf("Successfully loaded %ld data points from restart file\n", points);
-    if (points < ncells / 10) {
-        fprintf(stderr, "Warning: Only loaded %ld points out of %ld cells. This may indicate:\n",
-                points, ncells);
-        fprintf(stderr, "  - The restart file was subsampled\n");
-        fprintf(stderr, "  - Different grid resolution\n");
-        fprintf(stderr, "  - Incomplete or corrupted file\n");
-    }
-    s->dev->uploaded = false;
-    return 0;
 }
 
 // ------------------------------------------------------------- extensions --
@@ -532,6 +407,7 @@ int solver3d_download(GodunovHLLC3D* s) { return download(s); }
 
 int solver3d_set_state(GodunovHLLC3D* s, const double* U_aos) {
     memcpy(s->U, U_aos, sizeof(State3D) * (size_t)s->nx * s->ny * s->nz);
+    s->ic_status = 0;
     return upload(s);
 }
 
@@ -543,21 +419,20 @@ int solver3d_step(GodunovHLLC3D* s) {
     return 0;
 }
 
+// The device keeps the last DT_LOG steps' dt in a ring: steps go in rounds
+// of at most DT_LOG, each round's dt read back before the next round starts.
 int solver3d_run_steps(GodunovHLLC3D* s, int nsteps, double* dt_log) {
     if (ensure_uploaded(s)) return -2;
     HydroDevice* d = s->dev;
-    const long first = d->step;
-    for (int q = 0; q < nsteps; ++q)
-        if (step_device(s) < -1) return -2;
-    if (nsteps > 0) {
-        if (dt_log) {
-            for (int q = 0; q < nsteps; ++q)
-                if (read_dt(s, first + q, &dt_log[q])) return -2;
-            s->dt = dt_log[nsteps - 1];
-        } else if (read_dt(s, d->step - 1, &s->dt)) {
-            return -2;
-        }
+    for (int done = 0; done < nsteps;) {
+        const int round = std::min(nsteps - done, (int)HydroDevice::DT_LOG);
+        const long first = d->step;
+        for (int q = 0; q < round; ++q)
+            if (step_device(s) < -1) return -2;
+        if (dt_log && read_dt_range(s, first, round, dt_log + done)) return -2;
+        done += round;
     }
+    if (nsteps > 0 && read_dt(s, d->step - 1, &s->dt)) return -2;
     HYD_CHECK(hipStreamSynchronize(d->stream));
     return 0;
 }
@@ -631,29 +506,6 @@ int solver3d_benchmark_steps(GodunovHLLC3D* s, int nsteps, double* wall_ms, doub
     for (int q = 0; q < 3 * nsteps + 1; ++q)
         if (ev[q]) (void)hipEventDestroy(ev[q]);
     free(ev);
-    return rc;
-}
-
-long hydro_check_division(const double* a, const double* b, long n) {
-    hipStream_t st = hyd_stream();
-    if (!st || n <= 0) return -2;
-    double *da = nullptr, *db = nullptr;
-    unsigned long long* dm = nullptr;
-    unsigned long long hm = 0;
-    long rc = -2;
-    if (hipMalloc((void**)&da, sizeof(double) * n) == hipSuccess &&
-        hipMalloc((void**)&db, sizeof(double) * n) == hipSuccess &&
-        hipMalloc((void**)&dm, sizeof(unsigned long long)) == hipSuccess &&
-        hipMemcpyAsync(da, a, sizeof(double) * n, hipMemcpyHostToDevice, st) == hipSuccess &&
-        hipMemcpyAsync(db, b, sizeof(double) * n, hipMemcpyHostToDevice, st) == hipSuccess &&
-        hipMemsetAsync(dm, 0, sizeof(unsigned long long), st) == hipSuccess &&
-        hyd::launch_check_division(da, db, n, dm, st) == hipSuccess &&
-        hipMemcpyAsync(&hm, dm, sizeof hm, hipMemcpyDeviceToHost, st) == hipSuccess &&
-        hipStreamSynchronize(st) == hipSuccess)
-        rc = (long)hm;
-    if (da) (void)hipFree(da);
-    if (db) (void)hipFree(db);
-    if (dm) (void)hipFree(dm);
     return rc;
 }
 

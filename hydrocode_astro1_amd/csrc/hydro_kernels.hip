@@ -504,29 +504,6 @@ __global__ void soa_to_aos_kernel(double* __restrict__ aos, const double* d, con
     }
 }
 
-// Diagnostic: count quotients where the reciprocal-corrected division of the
-// step kernels differs in any bit from the IEEE division.
-__global__ void check_division_kernel(const double* __restrict__ a, const double* __restrict__ b,
-                                      long n, unsigned long long* mismatches) {
-    unsigned long long bad = 0;
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-         t += (long)gridDim.x * blockDim.x) {
-        const double x = a[t], y = b[t];
-        const double q1 = x / y;
-        const double q2 = div_rcp(x, y, 1.0 / y);
-        bad += (dbits(q1) != dbits(q2)) ? 1ull : 0ull;
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) bad += __shfl_xor(bad, o, 64);
-    if ((threadIdx.x & 63) == 0 && bad) atomicAdd(mismatches, bad);
-}
-
-hipError_t launch_check_division(const double* a, const double* b, long n,
-                                 unsigned long long* mismatches, hipStream_t st) {
-    hipLaunchKernelGGL(check_division_kernel, dim3(1024), dim3(256), 0, st, a, b, n, mismatches);
-    return hipGetLastError();
-}
-
 // ---------------------------------------------------------------- launchers
 // Planes per block along x when the caller leaves it open.  A block fills
 // a CU's LDS, so the grid runs in rounds of one block per CU: pick the chunk

@@ -47,8 +47,6 @@ hipError_t launch_hydro_step(const HydroStepArgs& a, hipStream_t st);
 hipError_t launch_hydro_bc(const HydroBCArgs& a, hipStream_t st);
 hipError_t launch_hydro_maxspeed(double* const u[5], long n, double gamma,
                                  unsigned long long* out, hipStream_t st);
-hipError_t launch_check_division(const double* a, const double* b, long n,
-                                 unsigned long long* mismatches, hipStream_t st);
 hipError_t launch_aos_to_soa(const double* aos, double* const u[5], long n, hipStream_t st);
 hipError_t launch_soa_to_aos(double* aos, double* const u[5], long n, hipStream_t st);
 

@@ -61,7 +61,7 @@ const Key kKeys[] = {
     KB(enable_torus), KD(torus_major_radius), KD(torus_minor_radius), KD(torus_density),
     KD(torus_pressure), KD(torus_rotation_velocity),
     KI(max_steps), KD(sor_omega), KI(sor_max_iter), KD(sor_tol), KI(multipole_lmax),
-    KS(nn_guess), KB(output_enable), KS(ic_file),
+    KS(nn_guess), KB(output_enable), KS(ic_file), KB(checkpoint),
 };
 
 const char* bc_name(BCType b) {
@@ -151,6 +151,7 @@ InputDeck* input_deck_create(const char* filename) {
     d->dc_inner = DC_INNER_SOR;
     d->moment_order = MOMENT_ORDER_TREE;
     d->output_enable = true;
+    d->checkpoint = false;
 
     if (filename == NULL) {
         printf("No input file provided, using defaults.\n");
@@ -254,6 +255,7 @@ void input_deck_print(const InputDeck* d) {
     printf("  moment_order = %s\n",
            d->moment_order == MOMENT_ORDER_SEQUENTIAL ? "sequential" : "tree");
     printf("\nOutput:\n  Directory: %s\n", d->output_dir);
+    if (d->checkpoint) printf("  checkpoints: output/checkpoint_NNNNNN.bin\n");
     printf("==============================================\n\n");
 }
 

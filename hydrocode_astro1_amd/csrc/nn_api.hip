@@ -714,19 +714,6 @@ double nn_forward_flops(int n) {
     return f;
 }
 
-int nn_probe_mfma(float* out256) {
-    hipStream_t st = hyd_stream();
-    float* d = nullptr;
-    HYD_CHECK(hipMalloc((void**)&d, sizeof(float) * 256));
-    int rc = 0;
-    if (hyd::nn::probe_mfma(d, st) ||
-        hipMemcpyAsync(out256, d, sizeof(float) * 256, hipMemcpyDeviceToHost, st) ||
-        hipStreamSynchronize(st))
-        rc = -2;
-    (void)hipFree(d);
-    return rc;
-}
-
 }  // extern "C"
 
 // Potential guess for a device density on a cubic n^3 grid with the box

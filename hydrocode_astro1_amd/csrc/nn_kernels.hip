@@ -6,7 +6,8 @@
 //
 // MFMA operand layout of v_mfma_f32_16x16x16bf16_1k (wave64): lane l holds
 // A[l % 16][4 (l / 16) + i], B[4 (l / 16) + i][l % 16] for i = 0..3 and
-// receives D[4 (l / 16) + i][l % 16]; probe_mfma checks this on the device.
+// receives D[4 (l / 16) + i][l % 16]; the forward and backward tests against
+// a CPU reference fail on any other layout.
 #include "nn_kernels.hpp"
 
 namespace hyd {
@@ -660,19 +661,6 @@ __global__ void adam_kernel(float* __restrict__ w, const float* __restrict__ g,
     }
 }
 
-__global__ void probe_mfma_kernel(float* d) {
-    const int lane = threadIdx.x & 63;
-    const int g = lane >> 4, m = lane & 15;
-    v4s a, b;
-    for (int i = 0; i < 4; ++i) {
-        a[i] = (short)f2bf((float)(16 * m + 4 * g + i));  // A[m][4g + i]
-        b[i] = (short)f2bf((4 * g + i) == m ? 1.f : 0.f);  // B[4g + i][m] (column m)
-    }
-    v4f acc = v4f{0.f, 0.f, 0.f, 0.f};
-    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(a, b, acc, 0, 0, 0);
-    for (int i = 0; i < 4; ++i) d[(4 * g + i) * 16 + m] = acc[i];
-}
-
 __global__ void restrict2_kernel(const double* __restrict__ f, int n, double* __restrict__ c) {
     const int nc = n / 2;
     const long total = (long)nc * nc * nc;
@@ -802,10 +790,6 @@ hipError_t adam_step(float* w, const float* g, float* m, float* v, long count, f
                      float b2, float eps, float c1, float c2, hipStream_t st) {
     hipLaunchKernelGGL(adam_kernel, dim3(grid_n(count)), dim3(256), 0, st, w, g, m, v, count, lr,
                        b1, b2, eps, c1, c2);
-    return hipGetLastError();
-}
-hipError_t probe_mfma(float* d, hipStream_t st) {
-    hipLaunchKernelGGL(probe_mfma_kernel, dim3(1), dim3(64), 0, st, d);
     return hipGetLastError();
 }
 hipError_t restrict2_f64(const double* fine, int n, double* coarse, hipStream_t st) {

@@ -91,10 +91,6 @@ hipError_t mse_loss(const float* pred, const float* target, long nvox, float sca
 hipError_t adam_step(float* w, const float* g, float* m, float* v, long count, float lr,
                      float b1, float b2, float eps, float c1, float c2, hipStream_t st);
 
-// probe of the MFMA operand layout: D = A B with A[m][k] = 16 m + k and
-// B = identity; writes D[m][n] (256 floats)
-hipError_t probe_mfma(float* d, hipStream_t st);
-
 // 2x coarsening of a density (mean of 8 cells, mass preserving)
 hipError_t restrict2_f64(const double* fine, int n, double* coarse, hipStream_t st);
 // Trilinear interpolation of a cell-centred nc^3 FP32 field onto the n^3

@@ -35,6 +35,26 @@ int solver3d_download(GodunovHLLC3D *solver);
 /* Replace the state: U_aos (nx*ny*nz*5 doubles) into solver->U, then upload. */
 int solver3d_set_state(GodunovHLLC3D *solver, const double *U_aos);
 
+/* ---- checkpoints (no reference counterpart: the reference restarts from
+ * its %.6e CSV output, hydro_solver_3d.c:2139-2235, which drops the
+ * potential, so SOR starts cold, gravity_solver_3d.c:87) ------------------ */
+
+/* Full-precision checkpoint "HYDCK001": a 40-byte header (magic, int32 nx
+ * ny nz, int32 flags: 1 potential, 2 g; double t; int32 output_count; int32
+ * 0), U as the reference's AoS doubles, then with gravity the potential in
+ * natural order and the gx, gy, gz planes.  Written from the device state. */
+int solver3d_write_checkpoint(GodunovHLLC3D *solver, const char *path, double t,
+                              int output_count);
+/* Load a checkpoint (or a U-only "HYDIC001" initial-state file) into the
+ * solver: U, and with gravity the potential the next solve warm-starts from
+ * and the g a step uses when its solve does not converge, so a run resumed
+ * from it continues bit for bit.  t and output_count (may be NULL) receive
+ * the header's values (0 for an initial-state file).  The solver is left
+ * unchanged on error (-1).  load_from_csv_restart and the deck's ic_file
+ * accept the same files. */
+int solver3d_read_checkpoint(GodunovHLLC3D *solver, const char *path, double *t,
+                             int *output_count);
+
 /* ---- stepping ---------------------------------------------------------- */
 
 /* One step of the reference step function (godunov_step_3d,
@@ -59,11 +79,6 @@ int solver3d_benchmark_steps(GodunovHLLC3D *solver, int nsteps, double *wall_ms,
 /* Device milliseconds of the last hydro kernels (step + boundary) and of
  * the last gravity solve. */
 int solver3d_last_timing(const GodunovHLLC3D *solver, double *hydro_ms, double *gravity_ms);
-
-/* Diagnostic: number of the n quotients a[i] / b[i] (host arrays) for which
- * the reciprocal-corrected division used by the step kernels differs in any
- * bit from the IEEE division on the device (0 expected), < 0 on error. */
-long hydro_check_division(const double *a, const double *b, long n);
 
 /* The reference's initial-condition problems into a host state array U
  * (nx*ny*nz cells, the reference's layout) without a solver or a device:
@@ -165,8 +180,6 @@ int nn_evaluate(void *model, void *dataset, int first, int count, float *mse);
  * flops of one forward pass (2 * 27 * cin * cout per output voxel). */
 int nn_benchmark(void *model, int n, int reps, double *fwd_ms, double *fwdbwd_ms);
 double nn_forward_flops(int n);
-/* MFMA operand-layout probe: D = A I with A[m][k] = 16 m + k (256 floats). */
-int nn_probe_mfma(float *out256);
 /* Attach a network to a solver for POISSON_NN_FP32_DC (NULL detaches; the
  * caller keeps ownership). */
 int gravity_set_nn_model(GravitySolver3D *solver, void *model);

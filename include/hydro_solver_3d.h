@@ -84,6 +84,11 @@ typedef struct GodunovHLLC3D {
     int max_steps;        /* run loops stop after this many steps (0: none) */
     bool output_enable;   /* run loops write CSV/VTK at output times        */
     long steps_taken;     /* steps since the state was last uploaded        */
+    int ic_status;        /* 0, or -1 when the deck's ic_file failed to load:
+                             U is left as it was and the run loops refuse
+                             to start from it                               */
+    bool checkpoint_enable; /* run loops also write output/checkpoint_N.bin
+                               (HYDCK001) at output times                  */
 } GodunovHLLC3D;
 
 /* hydro_solver_3d.h:69-73 */

@@ -158,6 +158,8 @@ typedef struct {
     char ic_file[256];        /* binary initial state written by this build */
     int dc_inner;             /* DcInner of the FP32 modes, default SOR   */
     int moment_order;         /* MomentOrder, default TREE                */
+    bool checkpoint;          /* also write HYDCK001 checkpoints at output
+                                 times (U, potential, g), default false   */
 } InputDeck;
 
 /* input_deck.h:127-131 */

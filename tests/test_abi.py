@@ -69,12 +69,22 @@ OFFSET_FIELDS = {
 }
 
 
-def _offsets(incdir, tmp, tag):
+# fields this build appends (not in the reference headers)
+APPENDED_FIELDS = {
+    "GodunovHLLC3D": ["dev", "max_steps", "output_enable", "steps_taken", "ic_status",
+                      "checkpoint_enable"],
+    "GravitySolver3D": ["dev", "multipole_lmax", "poisson_mode", "criterion", "last_status",
+                        "last_rel_residual", "last_solve_ms", "nn_model", "dc_inner",
+                        "moment_order"],
+}
+
+
+def _offsets(incdir, tmp, tag, table=OFFSET_FIELDS):
     src = os.path.join(tmp, f"off_{tag}.c")
     with open(src, "w") as f:
         f.write("#include <stdio.h>\n#include <stddef.h>\n#include \"hydro_solver_3d.h\"\n")
         f.write("int main(void){\n")
-        for st, fields in OFFSET_FIELDS.items():
+        for st, fields in table.items():
             for fld in fields:
                 f.write(f'printf("{st}.{fld} %zu\\n", offsetof({st}, {fld}));\n')
         f.write('printf("State3D.size %zu\\n", sizeof(State3D));\nreturn 0;}\n')
@@ -97,9 +107,12 @@ def test_struct_layout_matches_reference_headers():
 def test_ctypes_mirror_matches_header_offsets():
     with tempfile.TemporaryDirectory() as tmp:
         ours = _offsets(INC, tmp, "ours")
+        extra = _offsets(INC, tmp, "extra", APPENDED_FIELDS)
     for st, cls in (("GodunovHLLC3D", H.GodunovHLLC3D), ("GravitySolver3D", H.GravitySolver3D)):
         for fld in OFFSET_FIELDS[st]:
             assert getattr(cls, fld).offset == int(ours[f"{st}.{fld}"]), (st, fld)
+        for fld in APPENDED_FIELDS[st]:
+            assert getattr(cls, fld).offset == int(extra[f"{st}.{fld}"]), (st, fld)
 
 
 DECK = """# synthetic deck in the reference's key = value format
@@ -195,22 +208,25 @@ def test_deck_added_keys(tmp_path):
     with open(path, "w") as f:
         f.write("nx = 32\nmax_steps = 200\nsor_tol = 1e-8\nmultipole_lmax = 6\n"
                 "poisson_mode = fp32dc\nsor_criterion = rel_resid\noutput_enable = false\n"
-                "ic_file = state.bin\ndc_inner = multigrid\nmoment_order = sequential\n")
+                "ic_file = state.bin\ndc_inner = multigrid\nmoment_order = sequential\n"
+                "checkpoint = true\n")
     class Deck(C.Structure):
         _fields_ = _PREFIX_FIELDS + [
             ("max_steps", C.c_int), ("sor_omega", C.c_double), ("sor_max_iter", C.c_int),
             ("sor_tol", C.c_double), ("sor_criterion", C.c_int), ("multipole_lmax", C.c_int),
             ("poisson_mode", C.c_int), ("nn_guess", C.c_char * 256), ("output_enable", C.c_bool),
-            ("ic_file", C.c_char * 256), ("dc_inner", C.c_int), ("moment_order", C.c_int)]
+            ("ic_file", C.c_char * 256), ("dc_inner", C.c_int), ("moment_order", C.c_int),
+            ("checkpoint", C.c_bool)]
 
     lib = H.lib()
     p = lib.input_deck_create(path.encode())
     d = Deck.from_address(p)
     got = (d.nx, d.max_steps, d.sor_omega, d.sor_max_iter, d.sor_tol, d.sor_criterion,
-           d.multipole_lmax, d.poisson_mode, d.output_enable, d.ic_file, d.dc_inner, d.moment_order)
+           d.multipole_lmax, d.poisson_mode, d.output_enable, d.ic_file, d.dc_inner, d.moment_order,
+           d.checkpoint)
     lib.input_deck_destroy(p)
     assert got == (32, 200, 1.5, 10000, 1e-8, H.SOR_CRIT_REL_RESID, 6, H.POISSON_FP32_DC, False,
-                   b"state.bin", H.DC_INNER_MULTIGRID, H.MOMENT_ORDER_SEQUENTIAL)
+                   b"state.bin", H.DC_INNER_MULTIGRID, H.MOMENT_ORDER_SEQUENTIAL, True)
 
 
 def test_initial_state_file_roundtrip(tmp_path):

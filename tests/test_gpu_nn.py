@@ -1,4 +1,4 @@
-"""The learned Poisson guess on the device: MFMA operand layout, forward and
+"""The learned Poisson guess on the device: forward and
 backward passes against the torch CPU reference (tests/nn_reference.py),
 training, and the guess-or-warm-start rule of POISSON_NN_FP32_DC."""
 import numpy as np
@@ -13,15 +13,6 @@ pytestmark = pytest.mark.gpu
 
 def rel_l2(a, b):
     return float(np.linalg.norm(np.ravel(a) - np.ravel(b)) / max(np.linalg.norm(np.ravel(b)), 1e-30))
-
-
-def test_mfma_operand_layout():
-    """D = A B with A[m][k] = 16 m + k and B = I must return A exactly: the
-    lane layout the convolution kernels assume."""
-    out = np.zeros(256, dtype=np.float32)
-    assert H.lib().nn_probe_mfma(out.ctypes.data_as(nn._fp)) == 0
-    expect = np.arange(256, dtype=np.float32).reshape(16, 16)
-    np.testing.assert_array_equal(out.reshape(16, 16), expect)
 
 
 def sample(n, seed):
