@@ -135,6 +135,9 @@ EXPORTS = [
     ("gravity_boundary_potential", C.c_int, [_GP, _dp, _dp]),
     ("gravity_rel_residual", C.c_int, [_GP, _dp, _dp]),
     ("gravity_solve_device", C.c_int, [_GP, C.c_void_p]),
+    ("gravity_moments_noalloc_scratch", C.c_long, [C.c_int] * 4),
+    ("gravity_moments_noalloc", C.c_int,
+     [C.c_int] * 3 + [C.c_double] * 6 + [C.c_int, C.c_void_p, C.c_void_p]),
     ("gravity_set_nn_model", C.c_int, [_GP, C.c_void_p]),
     ("nn_model_create", C.c_void_p, [C.c_ulonglong]),
     ("nn_model_destroy", None, [C.c_void_p]),

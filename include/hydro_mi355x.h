@@ -138,6 +138,29 @@ int gravity_rel_residual(GravitySolver3D *solver, const double *rho, double *rel
  * the potential and (when converged) g stay on the device. */
 int gravity_solve_device(GravitySolver3D *solver, const double *d_rho);
 
+/* ---- no device allocation (no counterpart in the reference; for callers
+ * that cannot allocate device memory: code under HIP graph capture, tight
+ * serving loops, harnesses that forbid hipMalloc) ------------------------- */
+
+/* Largest scratch (doubles) gravity_moments_noalloc holds: 64 MiB per GPU,
+ * kept for the life of the process; enough for 256^3 at lmax <= 6 and for
+ * 128^3 at any lmax. */
+#define GRAVITY_MOMENTS_NOALLOC_MAX_SCRATCH (8L << 20)
+/* Scratch doubles a grid needs (-1 for bad arguments); the call below
+ * rejects grids needing more than the maximum above. */
+long gravity_moments_noalloc_scratch(int nx, int ny, int nz, int lmax);
+/* The moments gravity_moments returns for MOMENT_ORDER_TREE (same kernels,
+ * same bits) of a device density d_rho (nx*ny*nz doubles) into the device
+ * array d_mom: M, xcm, ycm, zcm, Q[3][3], then A_lm, B_lm for l = 3..lmax
+ * (1 + (lmax+1)(lmax+2) doubles).  The grid spacing is (max - min)/n as in
+ * gravity_solver_create.  Allocates nothing: the partial sums live in a
+ * static device array shared by every call on that GPU, so calls that
+ * overlap on one GPU race.  Synchronous; returns 0, -1 for bad arguments or
+ * a grid beyond the scratch maximum, -2 on a device error. */
+int gravity_moments_noalloc(int nx, int ny, int nz, double xmin, double xmax, double ymin,
+                            double ymax, double zmin, double zmax, int lmax,
+                            const double *d_rho, double *d_mom);
+
 /* ---- learned initial guess of the Poisson solve (no reference counterpart;
  * the reference starts SOR from the previous potential or from zero,
  * gravity_solver_3d.c:87 and :566) ---------------------------------------- */
