@@ -118,30 +118,56 @@ __device__ __forceinline__ void reflect(double& d, double& mx, double& my, doubl
     e = p / (gamma - 1.0) + ke2;
 }
 
+// True when one of a cell's three signal speeds |u|+c, |v|+c, |w|+c is NaN:
+// the terms are >= 0 or NaN, so their sum is NaN exactly then.
+__device__ __forceinline__ bool speeds_have_nan(double a, double b, double z, double c) {
+    const double t = (a + b) + (z + c);
+    return t != t;
+}
+
 // Largest signal speed of a cell as compute_dt_3d sees it
 // (hydro_solver_3d.c:428-440): max(|u|,|v|,|w|) + c.  Adding c is monotone
-// under rounding, so this equals the max of the three sums bit for bit.
+// under rounding, so this equals the max of the three sums bit for bit when
+// none of them is NaN; `nan` is set when one is (the reduction then takes
+// the reference's order into account, nan_fold_block).
 __device__ __forceinline__ double signal_speed(double d, double mx, double my, double mz,
-                                               double e, double gamma) {
+                                               double e, double gamma, bool& nan) {
     double r = rho_floor(d);
     double u = mx / r, v = my / r, w = mz / r;
     double ke = 0.5 * r * (u * u + v * v + w * w);
     double p = tmax((gamma - 1.0) * (e - ke), HYD_P_FLOOR);
     double c = sqrt(gamma * p / r);
     double a = fabs(u), b = fabs(v), z = fabs(w);
+    nan = nan || speeds_have_nan(a, b, z, c);
     double m = tmax(tmax(a, b), z);
     return m + c;
 }
 
 // The same for a state whose density is already at or above the floor.
 __device__ __forceinline__ double signal_speed_floored(double r, double mx, double my, double mz,
-                                                       double e, const EosConst& k) {
+                                                       double e, const EosConst& k, bool& nan) {
     const double u = mx / r, v = my / r, w = mz / r;
     const double ke = 0.5 * r * (u * u + v * v + w * w);
     const double p = tmax(k.gm1 * (e - ke), HYD_P_FLOOR);
-    const double c2 = k.gamma * p / r;
-    const double m = tmax(tmax(fabs(u), fabs(v)), fabs(w));
-    return m + sqrt(c2);
+    const double c = sqrt(k.gamma * p / r);
+    const double a = fabs(u), b = fabs(v), z = fabs(w);
+    nan = nan || speeds_have_nan(a, b, z, c);
+    const double m = tmax(tmax(a, b), z);
+    return m + c;
+}
+
+// The three speeds |u|+c, |v|+c, |w|+c of a cell in the reference's order
+// (hydro_solver_3d.c:428-440).
+__device__ __forceinline__ void cell_speeds(double d, double mx, double my, double mz, double e,
+                                            double gamma, double s[3]) {
+    const double r = rho_floor(d);
+    const double u = mx / r, v = my / r, w = mz / r;
+    const double ke = 0.5 * r * (u * u + v * v + w * w);
+    const double p = tmax((gamma - 1.0) * (e - ke), HYD_P_FLOOR);
+    const double c = sqrt(gamma * p / r);
+    s[0] = fabs(u) + c;
+    s[1] = fabs(v) + c;
+    s[2] = fabs(w) + c;
 }
 
 // Velocities of the gravity source (hydro_solver_3d.c:612-614): no floor.

@@ -84,7 +84,9 @@ struct HydroDevice {
     double *U[2][5] = {{nullptr}};
     int cur = 0;
     double *face = nullptr;
-    unsigned long long *ms = nullptr;  // [3] signal-speed slots
+    // [0..2] signal-speed slots, [3..5] their NaN flags, [6] the boundary
+    // kernel's block ticket
+    unsigned long long *ms = nullptr;
     double *dt_log = nullptr;          // [DT_LOG] dt per step (ring)
     double *g_zero = nullptr;          // zero g planes when gravity is off
     double *aos = nullptr;             // staging for AoS transfers
@@ -98,6 +100,7 @@ struct HydroDevice {
     bool skip_gravity = false;  // gravity already solved by the caller
     OutputPipe *out = nullptr;  // asynchronous output of the run loops
     static constexpr int DT_LOG = 1 << 16;
+    static constexpr int MS_WORDS = 8;
 };
 
 // Learned potential guess of a cubic device density (nn_api.hip); -1 when

@@ -58,13 +58,14 @@ int dev_create(GodunovHLLC3D* s) {
     for (int b = 0; b < 2; ++b)
         for (int f = 0; f < 5; ++f) HYD_CHECK(hipMalloc((void**)&d->U[b][f], sizeof(double) * d->n));
     HYD_CHECK(hipMalloc((void**)&d->face, sizeof(double) * hyd::face_buffer_len(d->nx, d->ny, d->nz)));
-    HYD_CHECK(hipMalloc((void**)&d->ms, sizeof(unsigned long long) * 3));
+    HYD_CHECK(hipMalloc((void**)&d->ms, sizeof(unsigned long long) * HydroDevice::MS_WORDS));
     HYD_CHECK(hipMalloc((void**)&d->dt_log, sizeof(double) * HydroDevice::DT_LOG));
     HYD_CHECK(hipMalloc((void**)&d->aos, sizeof(double) * 5 * d->n));
     for (int b = 0; b < 2; ++b)
         for (int f = 0; f < 5; ++f)
             HYD_CHECK(hipMemsetAsync(d->U[b][f], 0, sizeof(double) * d->n, d->stream));
-    HYD_CHECK(hipMemsetAsync(d->ms, 0, sizeof(unsigned long long) * 3, d->stream));
+    HYD_CHECK(hipMemsetAsync(d->ms, 0, sizeof(unsigned long long) * HydroDevice::MS_WORDS,
+                             d->stream));
     HYD_CHECK(hipEventCreate(&d->ev0));
     HYD_CHECK(hipEventCreate(&d->ev1));
     HYD_CHECK(hipStreamSynchronize(d->stream));
@@ -75,8 +76,10 @@ int upload(GodunovHLLC3D* s) {
     HydroDevice* d = s->dev;
     HYD_CHECK(hipMemcpyAsync(d->aos, s->U, sizeof(State3D) * d->n, hipMemcpyHostToDevice, d->stream));
     HYD_CHECK(hyd::launch_aos_to_soa(d->aos, d->U[d->cur], d->n, d->stream));
-    HYD_CHECK(hipMemsetAsync(d->ms, 0, sizeof(unsigned long long) * 3, d->stream));
-    HYD_CHECK(hyd::launch_hydro_maxspeed(d->U[d->cur], d->n, s->gamma, d->ms, d->stream));
+    HYD_CHECK(hipMemsetAsync(d->ms, 0, sizeof(unsigned long long) * HydroDevice::MS_WORDS,
+                             d->stream));
+    HYD_CHECK(hyd::launch_hydro_maxspeed(d->U[d->cur], d->n, s->gamma, d->ms, d->ms + 3,
+                                         d->stream));
     HYD_CHECK(hipStreamSynchronize(d->stream));
     d->step = 0;
     d->uploaded = true;
@@ -84,8 +87,12 @@ int upload(GodunovHLLC3D* s) {
     return 0;
 }
 
+// Device state to the host mirror.  Until the mirror has been uploaded (after
+// set_initial_conditions_3d, load_from_csv_restart, a checkpoint read) the
+// mirror is the state, and this leaves it alone.
 int download(GodunovHLLC3D* s) {
     HydroDevice* d = s->dev;
+    if (!d->uploaded) return 0;
     HYD_CHECK(hyd::launch_soa_to_aos(d->aos, d->U[d->cur], d->n, d->stream));
     HYD_CHECK(hipMemcpyAsync(s->U, d->aos, sizeof(State3D) * d->n, hipMemcpyDeviceToHost, d->stream));
     if (s->enable_gravity && s->gravity_solver) {
@@ -125,6 +132,9 @@ int step_device(GodunovHLLC3D* s) {
     a.ms_in = d->ms + (d->step % 3);
     a.ms_out = d->ms + ((d->step + 1) % 3);
     a.ms_clear = d->ms + ((d->step + 2) % 3);
+    a.nan_in = a.ms_in + 3;
+    a.nan_out = a.ms_out + 3;
+    a.nan_clear = a.ms_clear + 3;
     a.dt_out = d->dt_log + (d->step % HydroDevice::DT_LOG);
     a.nx = s->nx; a.ny = s->ny; a.nz = s->nz;
     a.cx = d->cx;
@@ -141,6 +151,8 @@ int step_device(GodunovHLLC3D* s) {
     for (int f = 0; f < 5; ++f) bc.u[f] = d->U[1 - d->cur][f];
     bc.face = d->face;
     bc.ms_out = a.ms_out;
+    bc.nan_out = a.nan_out;
+    bc.ticket = reinterpret_cast<unsigned int*>(d->ms + 6);
     bc.nx = s->nx; bc.ny = s->ny; bc.nz = s->nz;
     bc.bc[0] = s->bc_xmin; bc.bc[1] = s->bc_xmax;
     bc.bc[2] = s->bc_ymin; bc.bc[3] = s->bc_ymax;

@@ -49,10 +49,12 @@ constexpr int NF = 10;            // LDS fields per cell: d mx my mz e u v w p c
 constexpr int FS = LR * LC;       // LDS stride between fields of one cell
 }  // namespace
 
-__device__ __forceinline__ double dt_from_speed(unsigned long long bits, double dx, double dy,
-                                                double dz, double cfl) {
-    // compute_dt_3d, hydro_solver_3d.c:423 and :444-449
-    double ms = tmax(1.0e-10, bitsd(bits));
+__device__ __forceinline__ double dt_from_speed(unsigned long long bits, unsigned long long nan,
+                                                double dx, double dy, double dz, double cfl) {
+    // compute_dt_3d, hydro_solver_3d.c:423 and :444-449.  After a NaN speed
+    // the reference's running MAX restarts from the next speed, so its
+    // 1e-10 start value no longer takes part (nan_fold_block).
+    double ms = nan ? bitsd(bits) : tmax(1.0e-10, bitsd(bits));
     double tx = dx / ms, ty = dy / ms, tz = dz / ms;
     return cfl * tmin(tx, tmin(ty, tz));
 }
@@ -174,11 +176,13 @@ __global__ void __launch_bounds__(NT) hydro_step_kernel(HydroStepArgs a) {
     const long cidx = ((long)ib * ny + j) * nz + k;  // own cell at plane ib
     const long hidx = ((long)ib * ny + hj) * nz + hk;
 
-    const double dt = dt_from_speed(*a.ms_in, a.dx, a.dy, a.dz, a.cfl);
+    const double dt = dt_from_speed(*a.ms_in, *a.nan_in, a.dx, a.dy, a.dz, a.cfl);
     if (blockIdx.x == 0 && threadIdx.x == 0) {
         *a.dt_out = dt;
         *a.ms_clear = 0ull;
+        *a.nan_clear = 0ull;
     }
+    bool nan_speed = false;  // a NaN signal speed among this lane's new cells
     const double gamma = a.gamma;
     const EosConst kc = eos_const(gamma);
 
@@ -305,7 +309,7 @@ __global__ void __launch_bounds__(NT) hydro_step_kernel(HydroStepArgs a) {
                     for (int f = 0; f < 5; ++f) a.uout[f][idx] = q[f];
                     // the interior density is at or above the floor here
                     unsigned long long sb =
-                        dbits(signal_speed_floored(q[0], q[1], q[2], q[3], q[4], kc));
+                        dbits(signal_speed_floored(q[0], q[1], q[2], q[3], q[4], kc, nan_speed));
                     smax = sb > smax ? sb : smax;
                 } else if (a.zcopy && (k == 0 || k == nz - 1) && i >= 1 && i <= nx - 2 &&
                            j >= 1 && j <= ny - 2) {
@@ -374,12 +378,59 @@ __global__ void __launch_bounds__(NT) hydro_step_kernel(HydroStepArgs a) {
             __syncthreads();  // B2
         }
     }
-    // one atomic per block for the signal speed of the updated interior
+    // one atomic per block for the signal speed of the updated interior; a
+    // NaN speed only raises the flag (the boundary kernel's last block then
+    // reduces the whole state in the reference's order)
+    if (nan_speed) *a.nan_out = 1ull;
     __syncthreads();
     if (threadIdx.x == 0) {
         unsigned long long m = 0ull;
         for (int w = 1; w <= TJ; ++w) m = s_smax[w] > m ? s_smax[w] : m;
         if (m != 0ull && m > __atomic_load_n(a.ms_out, __ATOMIC_RELAXED)) atomicMax(a.ms_out, m);
+    }
+}
+
+// compute_dt_3d's reduction (hydro_solver_3d.c:423-441) when a speed is NaN,
+// by one block over the whole state: the reference folds the speeds of cell
+// after cell (u, v, w each) with MAX(a, b) = a > b ? a : b, so a NaN speed
+// replaces the running value and the next speed replaces a NaN.  The result
+// is NaN when the last speed is NaN, else the largest speed after the last
+// NaN one (the 1e-10 start value drops out).  Writes it to *ms_out and sets
+// the flag to 2; leaves both alone when the state has no NaN speed.
+__device__ void nan_fold_block(const double* const u[5], long n, double gamma,
+                               unsigned long long* ms_out, unsigned long long* nan_flag) {
+    __shared__ unsigned long long s_last;
+    unsigned long long last = 0ull;  // position + 1 of the last NaN speed, position 3 c + q
+    for (long c = threadIdx.x; c < n; c += blockDim.x) {
+        double s[3];
+        cell_speeds(u[0][c], u[1][c], u[2][c], u[3][c], u[4][c], gamma, s);
+#pragma unroll
+        for (int q = 0; q < 3; ++q)
+            if (s[q] != s[q]) last = 3ull * (unsigned long long)c + q + 1ull;
+    }
+    last = block_max_u64(last);
+    if (threadIdx.x == 0) s_last = last;
+    __syncthreads();
+    last = s_last;
+    if (last == 0ull) return;  // uniform across the block
+    unsigned long long m = 0ull;
+    for (long c = (long)(last / 3) + threadIdx.x; c < n; c += blockDim.x) {
+        double s[3];
+        cell_speeds(u[0][c], u[1][c], u[2][c], u[3][c], u[4][c], gamma, s);
+#pragma unroll
+        for (int q = 0; q < 3; ++q) {
+            const unsigned long long pos = 3ull * (unsigned long long)c + q;
+            if (pos >= last) {
+                const unsigned long long b = dbits(s[q]);
+                m = b > m ? b : m;
+            }
+        }
+    }
+    __syncthreads();  // s_part of block_max_u64 is reused
+    m = block_max_u64(m);
+    if (threadIdx.x == 0) {
+        *ms_out = (last == 3ull * (unsigned long long)n) ? dbits(__builtin_nan("")) : m;
+        *nan_flag = 2ull;
     }
 }
 
@@ -459,12 +510,34 @@ __global__ void __launch_bounds__(256) hydro_bc_kernel(HydroBCArgs a) {
         const long idx = ((long)i * ny + j) * nz + k;
 #pragma unroll
         for (int f = 0; f < 5; ++f) a.u[f][idx] = q[f];
-        const unsigned long long sb = dbits(signal_speed(q[0], q[1], q[2], q[3], q[4], a.gamma));
+        bool nan = false;
+        const unsigned long long sb =
+            dbits(signal_speed(q[0], q[1], q[2], q[3], q[4], a.gamma, nan));
         smax = sb > smax ? sb : smax;
+        if (nan) *a.nan_out = 1ull;
     }
     smax = block_max_u64(smax);
     if (threadIdx.x == 0 && smax != 0ull && smax > __atomic_load_n(a.ms_out, __ATOMIC_RELAXED))
         atomicMax(a.ms_out, smax);
+    // The last block to finish sees every cell's final value (each block
+    // makes its writes visible before it takes a ticket) and, when a NaN
+    // speed was flagged, redoes the reduction in the reference's order.
+    __shared__ unsigned int s_ticket;
+    __shared__ unsigned long long s_flag;
+    __threadfence();
+    __syncthreads();
+    if (threadIdx.x == 0) s_ticket = atomicAdd(a.ticket, 1u);
+    __syncthreads();
+    if (s_ticket != gridDim.x * gridDim.y * gridDim.z - 1) return;
+    __threadfence();
+    if (threadIdx.x == 0) {
+        *a.ticket = 0u;  // for the next launch
+        s_flag = __atomic_load_n(a.nan_out, __ATOMIC_RELAXED);
+    }
+    __syncthreads();
+    if (s_flag == 0ull) return;
+    const double* u[5] = {a.u[0], a.u[1], a.u[2], a.u[3], a.u[4]};
+    nan_fold_block(u, (long)nx * ny * nz, a.gamma, a.ms_out, a.nan_out);
 }
 
 // Signal speed of a whole state (after an upload).
@@ -474,16 +547,31 @@ __global__ void __launch_bounds__(256) hydro_maxspeed_kernel(const double* __res
                                                              const double* __restrict__ mz,
                                                              const double* __restrict__ e,
                                                              long n, double gamma,
-                                                             unsigned long long* out) {
+                                                             unsigned long long* out,
+                                                             unsigned long long* nan_flag) {
     unsigned long long smax = 0ull;
+    bool nan = false;
     for (long c = (long)blockIdx.x * blockDim.x + threadIdx.x; c < n;
          c += (long)gridDim.x * blockDim.x) {
-        unsigned long long sb = dbits(signal_speed(d[c], mx[c], my[c], mz[c], e[c], gamma));
+        unsigned long long sb = dbits(signal_speed(d[c], mx[c], my[c], mz[c], e[c], gamma, nan));
         smax = sb > smax ? sb : smax;
     }
+    if (nan) *nan_flag = 1ull;
     smax = block_max_u64(smax);
     if (threadIdx.x == 0 && smax != 0ull && smax > __atomic_load_n(out, __ATOMIC_RELAXED))
         atomicMax(out, smax);
+}
+
+// One block: the reference-order reduction when hydro_maxspeed_kernel
+// flagged a NaN speed.
+__global__ void __launch_bounds__(256) hydro_nan_fold_kernel(const double* d, const double* mx,
+                                                             const double* my, const double* mz,
+                                                             const double* e, long n, double gamma,
+                                                             unsigned long long* out,
+                                                             unsigned long long* nan_flag) {
+    if (*nan_flag == 0ull) return;
+    const double* u[5] = {d, mx, my, mz, e};
+    nan_fold_block(u, n, gamma, out, nan_flag);
 }
 
 // AoS (host layout, 5 doubles per cell) <-> SoA planes.
@@ -559,11 +647,13 @@ hipError_t launch_hydro_bc(const HydroBCArgs& a, hipStream_t st) {
 }
 
 hipError_t launch_hydro_maxspeed(double* const u[5], long n, double gamma, unsigned long long* out,
-                                 hipStream_t st) {
+                                 unsigned long long* nan_flag, hipStream_t st) {
     long nb = (n + 255) / 256;
     if (nb > 1024) nb = 1024;
     hipLaunchKernelGGL(hydro_maxspeed_kernel, dim3((unsigned)nb), dim3(256), 0, st, u[0], u[1],
-                       u[2], u[3], u[4], n, gamma, out);
+                       u[2], u[3], u[4], n, gamma, out, nan_flag);
+    hipLaunchKernelGGL(hydro_nan_fold_kernel, dim3(1), dim3(256), 0, st, u[0], u[1], u[2], u[3],
+                       u[4], n, gamma, out, nan_flag);
     return hipGetLastError();
 }
 

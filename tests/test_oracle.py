@@ -8,6 +8,7 @@ import os
 import numpy as np
 import pytest
 
+import fault_cases as FC
 import oracle_lib as OL
 from hydrocode_astro1_amd import ics
 
@@ -373,3 +374,49 @@ def test_ddtree_moments_agree_with_sequential_to_rounding():
     a, _ = OL.OracleSolver(g, moment_mode=OL.MOM_SEQUENTIAL).moments(rho)
     b, _ = OL.OracleSolver(g, moment_mode=OL.MOM_DDTREE).moments(rho)
     assert np.allclose(a, b, rtol=1e-13, atol=1e-16)
+
+
+@pytest.mark.parametrize("name", FC.NAMES)
+def test_dt_fault_states_match_reference(name):
+    """compute_dt_3d on states with NaN, infinite and negative cells: the
+    oracle's dt equals the compiled reference's (NaN where it is NaN)."""
+    R = OL.ref_lib()
+    if R is None:
+        pytest.skip("reference not built here")
+    U = np.ascontiguousarray(FC.case(name))
+    args = (*FC.SHAPE, 1 / 12, 1 / 10, 1 / 8, 0.4, FC.GAMMA)
+    a = OL.oracle_lib().or_compute_dt(OL.dptr(U), *args)
+    b = R.ref_compute_dt(OL.dptr(U), *args)
+    assert FC.same_bits_or_nan([a], [b])
+
+
+def test_dt_nan_order_closed_forms():
+    """What the ternary MAX fold fixes, without the reference: a NaN last
+    speed gives a NaN dt; a NaN v in the last cell leaves only that cell's w
+    speed; after the last NaN the 1e-10 start value no longer applies."""
+    L = OL.oracle_lib()
+    args = (*FC.SHAPE, 1 / 12, 1 / 10, 1 / 8, 0.4, FC.GAMMA)
+    g = FC.GAMMA
+
+    def speeds(cell):
+        r = max(cell[0], 1e-5)
+        u, v, w = cell[1] / r, cell[2] / r, cell[3] / r
+        ke = 0.5 * r * (u * u + v * v + w * w)
+        p = (g - 1.0) * (cell[4] - ke)
+        p = p if p > 1e-6 else 1e-6
+        c = math.sqrt(g * p / r)
+        return abs(u) + c, abs(v) + c, abs(w) + c
+
+    def dt_of(ms):
+        tx, ty, tz = (1 / 12) / ms, (1 / 10) / ms, (1 / 8) / ms
+        m = ty if ty < tz else tz
+        return 0.4 * (tx if tx < m else m)
+
+    U = np.ascontiguousarray(FC.case("nan_last_w"))
+    assert math.isnan(L.or_compute_dt(OL.dptr(U), *args))
+    U = np.ascontiguousarray(FC.case("nan_last_v"))
+    assert L.or_compute_dt(OL.dptr(U), *args) == dt_of(speeds(U[-1, -1, -1])[2])
+    U = np.ascontiguousarray(FC.case("tiny_tail"))
+    tail = max(speeds(U[-1, -1, -1]))
+    assert tail < 1e-10
+    assert L.or_compute_dt(OL.dptr(U), *args) == dt_of(tail)
