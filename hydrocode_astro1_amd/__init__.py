@@ -207,16 +207,16 @@ def initial_state(problem, shape, box=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0), gamma=5.0 
     return U if rc == 0 else None
 
 
-SEQSUM_STATS = ("blocks16384", "blocks1024", "segments", "chunks", "chunks_added",
-                "blocks1024_opened", "cycles", "cycles_adding", "cycles_opened")
+SEQSUM_STATS = ("chunks", "chunks_p1_3", "chunks_p4_6", "chunks_added", "verify_failed",
+                "cycles_stitch", "cycles_adding", "chain_redo", "chunks_zero")
 
 
 def sequential_sum(x, method=1, stats=False):
     """((0 + x[0]) + x[1]) + ... rounded after every addition, computed on
-    the device: method 0 adds the terms in one chain, method 1 moves by
-    block increments checked exactly (the moment sums of
-    MOMENT_ORDER_SEQUENTIAL use method 1).  stats=True also returns method
-    1's walk counters as a dict."""
+    the device: method 0 adds the terms in one chain, method 1 by chunks
+    with tabulated increments, a serial stitch and a parallel verification
+    (the moment sums of MOMENT_ORDER_SEQUENTIAL use method 1).  stats=True
+    also returns method 1's counters as a dict."""
     x = np.ascontiguousarray(x, dtype=np.float64)
     st = C.c_int(-1)
     cnt = (C.c_longlong * 9)()

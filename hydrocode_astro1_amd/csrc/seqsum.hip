@@ -1,107 +1,125 @@
 // seqsum.hip -- sequential-order sums (see seqsum.hpp for the method).
 //
-// Composite method, per sequence q of npad terms:
-//   seq_blocksum_kernel  approximate sum of every 1024-term block
-//   seq_scan_kernel      approximate exclusive prefix of those sums
-//   seq_prep_kernel      per chunk: predicted binade, D, lo, hi; per run of
-//                        chunks sharing a binade (a segment), the same
-//                        composed; per block of 1024: its segment when one
-//                        spans it
-//   seq_level2_kernel    the same for blocks of 16384
-//   seq_walk_kernel      one wavefront moves the running sum through the
-//                        blocks, applying a block's increment when its
-//                        conditions hold and adding the terms of a chunk one
-//                        by one when they do not
-// The approximate sums only choose which binade to try; every application
-// is checked against the exact running sum, so a wrong guess costs time, not
-// bits.
+// SEQSUM_PEAKS, per sequence q of npad terms (nb = npad / 1024 blocks,
+// nc = npad / 4096 chunks):
+//   pk_blocksum_kernel  double-double sum of every 1024-term block
+//   pk_scan_kernel      exclusive prefix of the block sums (double-double)
+//   pk_peak_kernel      per 4096-term block: the predicted running sum after
+//                       each term, where its magnitude peaks (chunk starts),
+//                       and whether every term is zero
+//   pk_tab_kernel       one start value per residue of each chunk, run over
+//                       the chunk's terms: 2 lanes per chunk for P = 1, 8
+//                       for P = 2, 3, a wavefront for P = 4..6
+//   pk_stitch_kernel    one wavefront per sequence carries the exact sum
+//                       across the chunks (term by term where a chunk fails)
+//   pk_verify_kernel    one lane per chunk runs it again from its exact start
+//   pk_redo_kernel      the chain, only for a sequence whose check failed
 #include "seqsum.hpp"
-
-#include <climits>
 
 namespace hyd {
 
 namespace {
 
-constexpr long long SEQ_LIM = 1LL << 55;  // any |D|, lo, hi beyond this never applies
-constexpr long long SEQ_2P52 = 1LL << 52;
-constexpr long long SEQ_2P53 = 1LL << 53;
 constexpr int SEQ_CHAIN_TILE = 2048;
 constexpr int SEQ_CHAIN_ROW = SEQ_CHAIN_TILE + 32;  // a tile and the overread of its last reads
+constexpr int TAB = 1 << SEQ_PMAX;                  // tabulated end values per chunk
+constexpr int BADP = 99;                            // a chunk that is added term by term
 
-__device__ __forceinline__ int rl(int v, int lane) { return __builtin_amdgcn_readlane(v, lane); }
-
-__device__ __forceinline__ long long rl64(long long v, int lane) {
-    const unsigned long long u = (unsigned long long)v;
-    const unsigned lo = (unsigned)rl((int)(unsigned)(u & 0xffffffffull), lane);
-    const unsigned hi = (unsigned)rl((int)(unsigned)(u >> 32), lane);
-    return (long long)(((unsigned long long)hi << 32) | lo);
+__device__ __forceinline__ unsigned long long ubits(double d) {
+    return (unsigned long long)__double_as_longlong(d);
+}
+__device__ __forceinline__ double dfrom(unsigned long long u) {
+    return __longlong_as_double((long long)u);
+}
+// sign and exponent field: equal keys = same sign and binade
+__device__ __forceinline__ unsigned key_of(double d) { return (unsigned)(ubits(d) >> 52); }
+__device__ __forceinline__ int expo(double d) { return (int)((ubits(d) >> 52) & 0x7ff); }
+__device__ __forceinline__ bool normal_finite(double d) {
+    const int e = expo(d);
+    return e != 0 && e != 0x7ff;
+}
+// s = I ulp(s): the signed integer I (2^52 <= |I| < 2^53) of a normal double
+__device__ __forceinline__ long long imant(double d) {
+    const unsigned long long u = ubits(d);
+    const long long m = (long long)((u & ((1ull << 52) - 1)) | (1ull << 52));
+    return (u >> 63) ? -m : m;
+}
+__device__ __forceinline__ double with_imant(long long i, double like) {
+    const unsigned long long a = (unsigned long long)(i < 0 ? -i : i);
+    return dfrom((ubits(like) & (0x7ffull << 52)) | (a & ((1ull << 52) - 1)) |
+                 (i < 0 ? (1ull << 63) : 0ull));
+}
+// The start value of residue r (I mod 2^P) nearest below V's integer, kept
+// inside V's binade.
+__device__ __forceinline__ double variant_start(double V, int P, int r) {
+    const long long mod = 1ll << P;
+    long long i = imant(V);
+    i = i - (i & (mod - 1)) + r;
+    const long long a = i < 0 ? -i : i;
+    if (a < (1ll << 52)) i += i > 0 ? mod : -mod;
+    else if (a >= (1ll << 53)) i -= i > 0 ? mod : -mod;
+    return with_imant(i, V);
+}
+// Residue bits of a chunk from its predicted start (peak) V and the next
+// chunk's predicted peak V1: 1 + how far V1's binade lies above V's, or BADP.
+__device__ __forceinline__ int chunk_bits(double V, double V1) {
+    if (!normal_finite(V) || !normal_finite(V1)) return BADP;
+    const int d = expo(V1) - expo(V);
+    const int P = (d > 0 ? d : 0) + 1;
+    return P <= SEQ_PMAX ? P : BADP;
 }
 
-__device__ __forceinline__ SeqDesc rldesc(const SeqDesc& d, int lane) {
-    SeqDesc r;
-    r.D = rl64(d.D, lane);
-    r.lo = rl64(d.lo, lane);
-    r.hi = rl64(d.hi, lane);
-    r.k = rl(d.k, lane);
-    r.flags = rl(d.flags, lane);
+// Reads of one lane's value into scalar registers (the lane index uniform).
+__device__ __forceinline__ double rl_d(double v, int lane) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b & 0xffffffffll), lane);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(b >> 32), lane);
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+__device__ __forceinline__ long rl_l(long v, int lane) {
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v & 0xffffffffl), lane);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(v >> 32), lane);
+    return (long)(((unsigned long)hi << 32) | lo);
+}
+__device__ __forceinline__ unsigned long long rfl_u(unsigned long long v) {
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v & 0xffffffffull));
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(v >> 32));
+    return ((unsigned long long)hi << 32) | lo;
+}
+__device__ __forceinline__ double rfl_d(double v) {
+    const long long b = __double_as_longlong(v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b & 0xffffffffll));
+    const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b >> 32));
+    return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
+
+struct DD {
+    double hi, lo;
+};
+__device__ __forceinline__ DD two_sum(double a, double b) {
+    DD r;
+    r.hi = a + b;
+    const double bb = r.hi - a;
+    r.lo = (a - (r.hi - bb)) + (b - bb);
     return r;
 }
-
-// Move the running sum (its bits sb) over a block described by d when that
-// is exact; false leaves it.  Works on the bits: s in the block's binade k
-// has exponent field k + 1023 and S = 2^52 + mantissa, or is 2^(k+1) itself
-// (S = 2^53); S + D lies in [2^52, 2^53] when the checks pass.  Only
-// integer arithmetic on values every lane shares, so it stays in scalar
-// registers.
-__device__ __forceinline__ bool seq_apply(const SeqDesc& d, long long& sb) {
-    if (d.flags & SEQ_ZERO) return true;  // s + (+-0) = s for every s but -0, which never occurs
-    if (!(d.flags & SEQ_VALID)) return false;
-    const unsigned long long u = (unsigned long long)sb;
-    const unsigned long long sign = u >> 63;
-    if (sign != (unsigned long long)((d.flags & SEQ_NEG) ? 1 : 0)) return false;
-    const int ex = (int)((u >> 52) & 0x7ff);
-    const long long man = (long long)(u & ((1ull << 52) - 1));
-    long long S;
-    if (ex == d.k + 1023) {
-        S = man | SEQ_2P52;
-    } else if (ex == d.k + 1024 && man == 0) {
-        S = SEQ_2P53;
-    } else {
-        return false;
-    }
-    if (S + d.lo < SEQ_2P52 || S + d.hi >= SEQ_2P53) return false;
-    const long long T = S + d.D;
-    const unsigned long long r =
-        T == SEQ_2P53 ? (unsigned long long)(d.k + 1024) << 52
-                      : ((unsigned long long)(d.k + 1023) << 52) | (unsigned long long)(T - SEQ_2P52);
-    sb = (long long)(r | (sign << 63));
-    return true;
+// double-double addition for predictions (not error-free in the last bits)
+__device__ __forceinline__ DD dd_add(DD x, DD y) {
+    const DD s = two_sum(x.hi, y.hi);
+    return two_sum(s.hi, s.lo + (x.lo + y.lo));
+}
+__device__ __forceinline__ DD dd_shfl_xor(DD x, int o) {
+    return DD{__shfl_xor(x.hi, o, 64), __shfl_xor(x.lo, o, 64)};
+}
+__device__ __forceinline__ DD dd_shfl_up(DD x, int o) {
+    return DD{__shfl_up(x.hi, o, 64), __shfl_up(x.lo, o, 64)};
 }
 
-__device__ __forceinline__ double wave_sum(double v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) v += __shfl_xor(v, o, 64);
-    return v;
-}
-
-__device__ __forceinline__ long long wave_min64(long long v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const long long w = __shfl_xor(v, o, 64);
-        v = w < v ? w : v;
-    }
-    return v;
-}
-
-__device__ __forceinline__ long long wave_max64(long long v) {
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const long long w = __shfl_xor(v, o, 64);
-        v = w > v ? w : v;
-    }
-    return v;
-}
+struct Peak {
+    double V;  // predicted running sum at the peak (start of the chunk)
+    int p;     // position of the peak's term in its 4096-term block
+    int zero;  // 1 when every term of the block is +0 or -0
+};
 
 // One wavefront adds the npad terms at x in order; the other three of the
 // 256-thread workgroup stage the next tile in LDS.  Returns the sum (valid in
@@ -178,443 +196,476 @@ __device__ __forceinline__ void load_chunk(const double* __restrict__ p, double 
     }
 }
 
-// Approximate sum of each 1024-term block (wavefront per block).
-__global__ void __launch_bounds__(256) seq_blocksum_kernel(const double* __restrict__ terms,
-                                                           long npad, double* __restrict__ bs) {
-    const long n1 = npad / SEQ_BLOCK1;
-    const long b1 = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+// Double-double sum of each 1024-term block (a wavefront per block).
+__global__ void __launch_bounds__(256) pk_blocksum_kernel(const double* __restrict__ terms,
+                                                          long npad, double* __restrict__ bsum) {
+    const long nb = npad / SEQ_BLOCK1;
+    const long j = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
     const int lane = threadIdx.x & 63;
-    if (b1 >= n1) return;
+    if (j >= nb) return;
     const int q = blockIdx.y;
     double v[SEQ_CHUNK];
-    load_chunk(terms + (long)q * npad + b1 * SEQ_BLOCK1 + lane * SEQ_CHUNK, v);
-    double c = 0.0;
+    load_chunk(terms + (long)q * npad + j * SEQ_BLOCK1 + lane * SEQ_CHUNK, v);
+    DD acc{0.0, 0.0};
 #pragma unroll
-    for (int e = 0; e < SEQ_CHUNK; ++e) c += v[e];
-    c = wave_sum(c);
-    if (lane == 0) bs[(long)q * n1 + b1] = c;
+    for (int e = 0; e < SEQ_CHUNK; ++e) acc = dd_add(acc, DD{v[e], 0.0});
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) acc = dd_add(acc, dd_shfl_xor(acc, o));
+    if (lane == 0) {
+        bsum[2 * ((long)q * nb + j)] = acc.hi;
+        bsum[2 * ((long)q * nb + j) + 1] = acc.lo;
+    }
 }
 
-// Exclusive prefix of the n1 block sums of sequence blockIdx.x (approximate).
-__global__ void __launch_bounds__(1024) seq_scan_kernel(const double* __restrict__ bs, long n1,
-                                                        double* __restrict__ pre) {
-    __shared__ double s_tot[1024];
+// Exclusive prefix of the nb block sums of sequence blockIdx.x.
+__global__ void __launch_bounds__(1024) pk_scan_kernel(const double* __restrict__ bsum, long nb,
+                                                       double* __restrict__ bpre) {
+    __shared__ double sh[1024], sl[1024];
     const int tid = threadIdx.x;
-    const long per = (n1 + 1023) / 1024;
+    const long per = (nb + 1023) / 1024;
     const long b = (long)tid * per;
-    const long e = b + per < n1 ? b + per : n1;
-    const double* in = bs + (long)blockIdx.x * n1;
-    double* out = pre + (long)blockIdx.x * n1;
-    double t = 0.0;
-    for (long i = b; i < e; ++i) t += in[i];
-    s_tot[tid] = t;
+    const long e = b + per < nb ? b + per : nb;
+    const double* in = bsum + 2 * (long)blockIdx.x * nb;
+    double* out = bpre + 2 * (long)blockIdx.x * nb;
+    DD t{0.0, 0.0};
+    for (long i = b; i < e; ++i) t = dd_add(t, DD{in[2 * i], in[2 * i + 1]});
+    sh[tid] = t.hi;
+    sl[tid] = t.lo;
     __syncthreads();
     for (int o = 1; o < 1024; o <<= 1) {
-        const double add = tid >= o ? s_tot[tid - o] : 0.0;
+        DD add{0.0, 0.0};
+        if (tid >= o) add = DD{sh[tid - o], sl[tid - o]};
         __syncthreads();
-        s_tot[tid] += add;
+        if (tid >= o) {
+            const DD c = dd_add(DD{sh[tid], sl[tid]}, add);
+            sh[tid] = c.hi;
+            sl[tid] = c.lo;
+        }
         __syncthreads();
     }
-    double run = s_tot[tid] - t;
+    DD run{0.0, 0.0};
+    if (tid > 0) run = DD{sh[tid - 1], sl[tid - 1]};
     for (long i = b; i < e; ++i) {
-        out[i] = run;
-        run += in[i];
+        out[2 * i] = run.hi;
+        out[2 * i + 1] = run.lo;
+        run = dd_add(run, DD{in[2 * i], in[2 * i + 1]});
     }
 }
 
-// Chunk and 1024-block descriptors (wavefront per block, lane per chunk).
-__global__ void __launch_bounds__(256) seq_prep_kernel(const double* __restrict__ terms, long npad,
-                                                       const double* __restrict__ pre,
-                                                       SeqDesc* __restrict__ dseg,
-                                                       SeqDesc* __restrict__ d1) {
-    const long n1 = npad / SEQ_BLOCK1;
-    const long b1 = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+// Per 4096-term block (a workgroup, a wavefront per 1024 terms): the
+// predicted running sum after each term (block prefix plus the in-block
+// prefix), the first position of its largest magnitude, and whether all the
+// terms are zero.
+__global__ void __launch_bounds__(256) pk_peak_kernel(const double* __restrict__ terms, long npad,
+                                                      const double* __restrict__ bpre,
+                                                      Peak* __restrict__ pk) {
+    __shared__ double s_best[4], s_v[4];
+    __shared__ int s_k[4], s_zero[4];
+    const long nb = npad / SEQ_BLOCK1;
+    const int w = threadIdx.x >> 6;
+    const long j = (long)blockIdx.x * 4 + w;  // < nb: nb is a multiple of 16
     const int lane = threadIdx.x & 63;
-    if (b1 >= n1) return;
     const int q = blockIdx.y;
     double v[SEQ_CHUNK];
-    load_chunk(terms + (long)q * npad + b1 * SEQ_BLOCK1 + lane * SEQ_CHUNK, v);
-    double c = 0.0;
+    load_chunk(terms + (long)q * npad + j * SEQ_BLOCK1 + lane * SEQ_CHUNK, v);
+    double pre[SEQ_CHUNK];
+    double run = 0.0;
     bool zero = true;
 #pragma unroll
     for (int e = 0; e < SEQ_CHUNK; ++e) {
-        c += v[e];
+        run = run + v[e];
+        pre[e] = run;
         zero = zero && v[e] == 0.0;
     }
-    // approximate running sum at the chunk's start: its binade is the guess
-    double incl = c;
+    DD incl{run, 0.0};
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
-        const double w = __shfl_up(incl, o, 64);
-        if (lane >= o) incl += w;
+        const DD u = dd_shfl_up(incl, o);
+        if (lane >= o) incl = dd_add(incl, u);
     }
-    const double P = pre[(long)q * n1 + b1] + (incl - c);
-    int ex = 0;
-    (void)frexp(P, &ex);
-    const int k = ex - 1;  // |P| in [2^k, 2^(k+1))
-    const bool neg = P < 0.0;
-    bool ok = P != 0.0 && P - P == 0.0 && k >= -960 && k <= 1000;
-    long long E = 0, lo = SEQ_LIM, hi = -SEQ_LIM;
+    DD excl = dd_shfl_up(incl, 1);
+    if (lane == 0) excl = DD{0.0, 0.0};
+    const long bi = (long)q * nb + j;
+    const DD base = dd_add(DD{bpre[2 * bi], bpre[2 * bi + 1]}, excl);
+    double best = -1.0, bv = 0.0;
+    int bk = 0;
 #pragma unroll
     for (int e = 0; e < SEQ_CHUNK; ++e) {
-        const double sx = neg ? -v[e] : v[e];
-        const double t = ldexp(sx, 52 - k);  // exact unless subnormal (then |t| < 0.5)
-        const double at = fabs(t);
-        long long f = 0, d = 0;
-        if (!(at < 0x1p54)) {
-            ok = false;
-        } else if (at < 0.5) {
-            f = sx < 0.0 ? -1 : 0;
-        } else {
-            const double fl = floor(t);
-            if (t - fl == 0.5) ok = false;  // a tie: rounds by the parity of S
-            f = (long long)fl;
-            d = (long long)rint(t);
-        }
-        lo = E + f < lo ? E + f : lo;
-        hi = E + f > hi ? E + f : hi;
-        E += d;
-    }
-    if (E >= SEQ_LIM || E <= -SEQ_LIM || lo <= -SEQ_LIM || hi >= SEQ_LIM) ok = false;
-    if (!ok || zero) E = lo = hi = 0;
-    // Segments: maximal runs of chunks that share a binade and sign, every
-    // non-zero one valid (zero chunks join the run they fall in; an invalid
-    // chunk ends a run and forms its own).  Lane l's segment entry (kept at
-    // the lane where a segment starts) composes the chunks [l, end) and
-    // carries end in flags >> 8.  The 1024-block is the segment at lane 0
-    // when it spans all 64 chunks.
-    const bool nz = !zero;
-    const int code = k * 2 + (neg ? 1 : 0);
-    int lastnz = nz ? lane : -1;  // last non-zero lane <= l
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const int w = __shfl_up(lastnz, o, 64);
-        if (lane >= o) lastnz = w > lastnz ? w : lastnz;
-    }
-    int prev = __shfl_up(lastnz, 1, 64);  // last non-zero lane < l
-    if (lane == 0) prev = -1;
-    const int okp = __shfl(ok ? 1 : 0, prev < 0 ? 0 : prev, 64);
-    const int codep = __shfl(code, prev < 0 ? 0 : prev, 64);
-    const bool start = lane == 0 || (nz && prev >= 0 && (!ok || !okp || code != codep));
-    const unsigned long long starts = __ballot(start);
-    // end: the next start after l (64 when none)
-    const unsigned long long after = lane == 63 ? 0ull : (starts >> (lane + 1)) << (lane + 1);
-    const int end = after ? __ffsll((long long)after) - 1 : 64;
-    // first lane of l's segment
-    const unsigned long long upto = starts & (lane == 63 ? ~0ull : ((2ull << lane) - 1));
-    const int first = 63 - __clzll((long long)upto);
-    long long inc = nz ? E : 0;
-    const long long own = inc;
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const long long w = __shfl_up(inc, o, 64);
-        if (lane >= o) inc += w;
-    }
-    const long long excl = inc - own;
-    const long long base = __shfl(excl, first, 64);  // increments before the segment
-    const long long rel = excl - base;
-    long long slo = nz ? rel + lo : SEQ_LIM, shi = nz ? rel + hi : -SEQ_LIM;
-    int cmin = nz ? code : INT_MAX, cmax = nz ? code : INT_MIN, allok = (!nz || ok) ? 1 : 0;
-    // suffix reductions inside the segment: lane l folds in lane l + o when
-    // that lane is still in its segment
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const long long wl = __shfl_down(slo, o, 64), wh = __shfl_down(shi, o, 64);
-        const int a = __shfl_down(cmin, o, 64), bb = __shfl_down(cmax, o, 64);
-        const int w = __shfl_down(allok, o, 64);
-        if (lane + o < end) {
-            slo = wl < slo ? wl : slo;
-            shi = wh > shi ? wh : shi;
-            cmin = a < cmin ? a : cmin;
-            cmax = bb > cmax ? bb : cmax;
-            allok &= w;
+        const double pv = base.hi + (base.lo + pre[e]);
+        const double a = fabs(pv);
+        if (a > best) {
+            best = a;
+            bv = pv;
+            bk = w * SEQ_BLOCK1 + lane * SEQ_CHUNK + e;
         }
     }
-    const long long Dseg = __shfl(inc, end - 1, 64) - base;
-    SeqDesc sg;
-    if (cmin == INT_MAX) {  // only zero chunks
-        sg.D = sg.lo = sg.hi = 0;
-        sg.k = 0;
-        sg.flags = SEQ_ZERO;
-    } else {
-        const bool v = allok && cmin == cmax && Dseg < SEQ_LIM && Dseg > -SEQ_LIM &&
-                       slo > -SEQ_LIM && shi < SEQ_LIM;
-        sg.D = v ? Dseg : 0;
-        sg.lo = v ? slo : 0;
-        sg.hi = v ? shi : 0;
-        sg.k = cmin >> 1;  // floor division keeps k for negative codes
-        sg.flags = (v ? SEQ_VALID : 0) | ((cmin & 1) ? SEQ_NEG : 0);
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double ob = __shfl_xor(best, o, 64), ov = __shfl_xor(bv, o, 64);
+        const int ok = __shfl_xor(bk, o, 64);
+        if (ob > best || (ob == best && ok < bk)) {
+            best = ob;
+            bv = ov;
+            bk = ok;
+        }
     }
-    sg.flags |= (end << 8) | (start ? SEQ_START : 0);
-    dseg[((long)q * n1 + b1) * 64 + lane] = sg;
+    const int wzero = __all(zero);
     if (lane == 0) {
-        SeqDesc b = sg;
-        if (end != 64) {
-            b.D = b.lo = b.hi = 0;
-            b.flags = 0;
+        s_best[w] = best;
+        s_v[w] = bv;
+        s_k[w] = bk;
+        s_zero[w] = wzero;
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        // in term order: a later wavefront wins only with a larger magnitude
+        Peak r;
+        r.V = s_v[0];
+        r.p = s_k[0];
+        r.zero = s_zero[0];
+        double b = s_best[0];
+        for (int u = 1; u < 4; ++u) {
+            if (s_best[u] > b) {
+                b = s_best[u];
+                r.V = s_v[u];
+                r.p = s_k[u];
+            }
+            r.zero = r.zero && s_zero[u];
         }
-        b.flags &= SEQ_VALID | SEQ_NEG | SEQ_ZERO;
-        d1[(long)q * n1 + b1] = b;
+        pk[(long)q * gridDim.x + blockIdx.x] = r;
     }
 }
 
-// 16384-blocks from their sixteen 1024-blocks (thread per block).
-__global__ void __launch_bounds__(256) seq_level2_kernel(const SeqDesc* __restrict__ d1, long n2,
-                                                         int nq, SeqDesc* __restrict__ d2) {
-    const long id = (long)blockIdx.x * 256 + threadIdx.x;
-    if (id >= n2 * nq) return;
-    const SeqDesc* c = d1 + id * 16;
-    bool valid = true, any = false;
-    int k = 0, flags = 0;
-    long long E = 0, lo = SEQ_LIM, hi = -SEQ_LIM;
-    for (int j = 0; j < 16; ++j) {
-        const SeqDesc d = c[j];
-        if (d.flags & SEQ_ZERO) continue;
-        if (!(d.flags & SEQ_VALID)) valid = false;
-        if (!any) {
-            k = d.k;
-            flags = d.flags & SEQ_NEG;
-            any = true;
-        } else if (d.k != k || (d.flags & SEQ_NEG) != flags) {
-            valid = false;
-        }
-        if (!valid) break;
-        lo = E + d.lo < lo ? E + d.lo : lo;
-        hi = E + d.hi > hi ? E + d.hi : hi;
-        E += d.D;
-        if (E >= SEQ_LIM || E <= -SEQ_LIM || lo <= -SEQ_LIM || hi >= SEQ_LIM) valid = false;
-    }
-    SeqDesc r;
-    if (!any) {
-        r.D = r.lo = r.hi = 0;
-        r.k = 0;
-        r.flags = SEQ_ZERO;
-    } else {
-        r.D = valid ? E : 0;
-        r.lo = valid ? lo : 0;
-        r.hi = valid ? hi : 0;
-        r.k = k;
-        r.flags = (valid ? SEQ_VALID : 0) | flags;
-    }
-    d2[id] = r;
-}
-
-// What the walk needs of one 1024-block: each lane's segment entry and 16
-// terms.
-struct SeqBlockData {
-    SeqDesc sg;
-    double v[SEQ_CHUNK];
+// Chunk c of a sequence: terms [first, last], residue bits P, predicted
+// start V, and whether all its terms are zero (both 4096-blocks it touches).
+struct Chunk {
+    long first, last;
+    double V;
+    int P;
+    int zero;
 };
+__device__ __forceinline__ Chunk chunk_of(const Peak* __restrict__ pkq, long nc, long npad,
+                                          long c) {
+    const Peak a = pkq[c];
+    Chunk k;
+    k.first = c * SEQ_CBLK + a.p + 1;
+    k.V = a.V;
+    if (c + 1 < nc) {
+        const Peak b = pkq[c + 1];
+        k.last = (c + 1) * SEQ_CBLK + b.p;
+        k.P = chunk_bits(a.V, b.V);
+        k.zero = a.zero && b.zero;
+    } else {
+        k.last = npad - 1;
+        k.P = chunk_bits(a.V, a.V);
+        k.zero = a.zero;
+    }
+    return k;
+}
 
-__device__ __forceinline__ void load_block(const double* __restrict__ terms, long npad,
-                                           const SeqDesc* __restrict__ dseg, int q, long n1,
-                                           long b1, SeqBlockData& b) {
+__device__ __forceinline__ long wave_max_long(long v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const long w = __shfl_xor(v, o, 64);
+        v = w > v ? w : v;
+    }
+    return v;
+}
+
+// s + x[first] + ... + x[first + len - 1], one lane's own run, the next 8
+// terms loaded while the current 8 are added.  Every lane of the wavefront
+// calls it: minlen and maxlen are the shortest and longest runs among the
+// lanes (a lane with nothing to add passes len = maxlen and any start).
+// Reads reach x[first + maxlen + 15]: the term arrays are followed by at
+// least that much readable memory (the workspace).
+__device__ __forceinline__ double run_lane(const double* __restrict__ x, long first, long len,
+                                           long minlen, long maxlen, double s) {
+    const double* p = x + first;
+    double cur[8], nxt[8];
+#pragma unroll
+    for (int i = 0; i < 8; ++i) cur[i] = p[i];
+    long base = 0;
+    for (; base + 8 <= minlen; base += 8) {  // every lane adds all 8
+#pragma unroll
+        for (int i = 0; i < 8; ++i) nxt[i] = p[base + 8 + i];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) s = s + cur[i];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) cur[i] = nxt[i];
+    }
+    for (; base < maxlen; base += 8) {  // the tails, predicated per lane
+#pragma unroll
+        for (int i = 0; i < 8; ++i) nxt[i] = p[base + 8 + i];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const double t = s + cur[i];
+            s = (base + i < len) ? t : s;
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) cur[i] = nxt[i];
+    }
+    return s;
+}
+
+__device__ __forceinline__ long wave_min_long(long v) {
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const long w = __shfl_xor(v, o, 64);
+        v = w < v ? w : v;
+    }
+    return v;
+}
+
+// End values of the chunks whose residue bits P lie in [PLO, PHI]: L lanes
+// per chunk (lane % L = residue), 64 / L chunks per wavefront.
+template <int L, int PLO, int PHI>
+__global__ void __launch_bounds__(256) pk_tab_kernel(const double* __restrict__ terms, long npad,
+                                                     const Peak* __restrict__ pk,
+                                                     double* __restrict__ tab) {
+    constexpr int CPW = 64 / L;
+    const long nc = npad / SEQ_CBLK;
     const int lane = threadIdx.x & 63;
-    b.sg = dseg[((long)q * n1 + b1) * 64 + lane];
-    load_chunk(terms + (long)q * npad + b1 * SEQ_BLOCK1 + lane * SEQ_CHUNK, b.v);
-}
-
-// Terms [a, b) of the block staged in LDS (a, b multiples of 16), added one
-// by one: all 16 reads of a chunk are issued before its adds.
-__device__ __forceinline__ void seq_chain_lds(const double* blk, int a, int b, long long& sb) {
-    double s = __longlong_as_double(sb);
-    for (int e = a; e < b; e += SEQ_CHUNK) {
-        double2 w[SEQ_CHUNK / 2];
-#pragma unroll
-        for (int u = 0; u < SEQ_CHUNK / 2; ++u) w[u] = reinterpret_cast<const double2*>(blk + e)[u];
-#pragma unroll
-        for (int u = 0; u < SEQ_CHUNK / 2; ++u) {
-            s = s + w[u].x;
-            s = s + w[u].y;
-        }
+    const long c = ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * CPW + lane / L;
+    const int r = lane % L;
+    const int q = blockIdx.y;
+    bool active = false;
+    Chunk k{0, 0, 0.0, BADP, 0};
+    if (c < nc) {
+        k = chunk_of(pk + (long)q * nc, nc, npad, c);
+        active = k.P >= PLO && k.P <= PHI && r < (1 << k.P);
     }
-    sb = __double_as_longlong(s);
+    const long len = active ? k.last - k.first + 1 : 0;
+    const long maxlen = wave_max_long(len);
+    if (maxlen == 0) return;  // uniform across the wavefront
+    const long minlen = wave_min_long(active ? len : maxlen);
+    double s = active ? variant_start(k.V, k.P, r) : 0.0;
+    s = run_lane(terms + (long)q * npad, active ? k.first : 0, active ? len : maxlen, minlen,
+                 maxlen, s);
+    if (active) tab[((long)q * nc + c) * TAB + r] = s;
 }
 
-// Stage an opened block's terms in LDS.  Only one wavefront of the
-// workgroup walks, and its LDS accesses run in order, so ordering them needs
-// no barrier (a barrier would also wait for the load of the next block).
-__device__ __forceinline__ void seq_stage(const SeqBlockData& b, double* blk) {
+// s + x[first] + ... in order: the wavefront loads 64 terms at a time (the
+// next 64 while the current ones are added) and the chain of additions runs
+// on the values read from lane i.  Returns the same value in every lane.
+__device__ __noinline__ double run_wave(const double* __restrict__ x, long first, long len,
+                                        double s) {
     const int lane = threadIdx.x & 63;
-    __builtin_amdgcn_wave_barrier();
+    double v = lane < len ? x[first + lane] : 0.0;
+    for (long base = 0; base < len; base += 64) {
+        const long cnt = len - base < 64 ? len - base : 64;
+        const double w = base + 64 + lane < len ? x[first + base + 64 + lane] : 0.0;
+        if (cnt == 64) {
 #pragma unroll
-    for (int e = 0; e < SEQ_CHUNK / 2; ++e)
-        reinterpret_cast<double2*>(blk + lane * SEQ_CHUNK)[e] = make_double2(b.v[2 * e], b.v[2 * e + 1]);
-    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-    __builtin_amdgcn_wave_barrier();
-    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-}
-
-// All 1024 staged terms added one by one, the reads of the next 16 in
-// flight while the current 16 are added.
-__device__ __forceinline__ void seq_chain_block(const double* blk, long long& sb) {
-    double s = __longlong_as_double(sb);
-    const double2* b2 = reinterpret_cast<const double2*>(blk);
-    double2 a[SEQ_CHUNK / 2], b[SEQ_CHUNK / 2];
-#pragma unroll
-    for (int u = 0; u < SEQ_CHUNK / 2; ++u) a[u] = b2[u];
-    for (int e = 0; e < SEQ_BLOCK1 / 2; e += SEQ_CHUNK) {
-#pragma unroll
-        for (int u = 0; u < SEQ_CHUNK / 2; ++u) b[u] = b2[e + SEQ_CHUNK / 2 + u];
-#pragma unroll
-        for (int u = 0; u < SEQ_CHUNK / 2; ++u) {
-            s = s + a[u].x;
-            s = s + a[u].y;
-        }
-        if (e + SEQ_CHUNK < SEQ_BLOCK1 / 2) {
-#pragma unroll
-            for (int u = 0; u < SEQ_CHUNK / 2; ++u) a[u] = b2[e + SEQ_CHUNK + u];
-        }
-#pragma unroll
-        for (int u = 0; u < SEQ_CHUNK / 2; ++u) {
-            s = s + b[u].x;
-            s = s + b[u].y;
-        }
-    }
-    sb = __double_as_longlong(s);
-}
-
-// Walk a staged block by its segments: a segment of three chunks or more is
-// tried as a whole; one that does not apply, and any shorter one, is added
-// term by term (a shorter check costs more than the adds it saves).  A
-// block cut into more than four segments is added term by term outright:
-// its segments are short and their checks would cost more than the adds.
-__device__ __forceinline__ void seq_walk_segments(const SeqDesc& sgl, const double* blk,
-                                                  long long& sb, long long (&cnt)[9], bool want) {
-    if (__popcll(__ballot((sgl.flags & SEQ_START) != 0)) > 4) {
-        const long long t_chain = want ? clock64() : 0;
-        seq_chain_block(blk, sb);
-        cnt[4] += 64;
-        if (want) cnt[7] += clock64() - t_chain;
-        return;
-    }
-    for (int c = 0; c < 64;) {
-        SeqDesc sg = rldesc(sgl, c);
-        const int end = sg.flags >> 8;
-        sg.flags &= SEQ_VALID | SEQ_NEG | SEQ_ZERO;
-        if (end - c >= 3 && seq_apply(sg, sb)) {
-            cnt[2] += 1;
+            for (int i = 0; i < 64; ++i) s = s + rl_d(v, i);
         } else {
-            const long long t_chain = want ? clock64() : 0;
-            seq_chain_lds(blk, c * SEQ_CHUNK, end * SEQ_CHUNK, sb);
-            cnt[4] += end - c;
-            if (want) cnt[7] += clock64() - t_chain;
+            for (int i = 0; i < (int)cnt; ++i) s = s + rl_d(v, i);
         }
-        c = end;
+        v = w;
     }
+    return rfl_d(s);
 }
 
-// The running sum of sequence blockIdx.x.  A sequence whose 1024-blocks
-// mostly leave their binade is added by the plain chain (all four
-// wavefronts); otherwise one wavefront walks it.  Every lane holds the same
-// s, kept as its bits; descriptors are
-// loaded a row at a time and read lane by lane.  A 1024-block whose own
-// entry does not apply is staged in LDS and walked by segments, then
-// chunks, then single terms.  Inside a 16384-block that does not apply, the
-// next 1024-block is loaded into one of two register sets while the block in
-// the other is tried and walked.
-__global__ void __launch_bounds__(256) seq_walk_kernel(const double* __restrict__ terms, long npad,
-                                                       const SeqDesc* __restrict__ dseg,
-                                                       const SeqDesc* __restrict__ d1,
-                                                       const SeqDesc* __restrict__ d2,
+constexpr int STITCH_G = 16;        // chunks per load round of the stitch
+// LDS bytes asked for each stitch workgroup (dynamic, unused): more than half
+// of a CU's 160 KB, so no two stitches share a CU
+constexpr int STITCH_LDS_BYTES = 96 * 1024;
+
+// What the stitch needs of a chunk: its start key (sign and exponent field),
+// residue bits, the start values' common base I_v - (I_v mod 2^P), and
+// whether base + r stays in the binade for every residue r.
+struct StitchInfo {
+    unsigned key;
+    int P;
+    long long base;
+    int ok;
+    int zero;
+};
+__device__ __forceinline__ StitchInfo stitch_info(const Chunk& k) {
+    StitchInfo i;
+    i.key = key_of(k.V);
+    i.P = k.P;
+    i.zero = k.zero;
+    i.ok = k.P <= SEQ_PMAX;
+    i.base = 0;
+    if (i.ok) {
+        const long long mod = 1ll << k.P;
+        const long long iv = imant(k.V);
+        i.base = iv - (iv & (mod - 1));
+        const long long lo = i.base, hi = i.base + (mod - 1);
+        const long long alo = lo < 0 ? -lo : lo, ahi = hi < 0 ? -hi : hi;
+        i.ok = alo >= (1ll << 52) && alo < (1ll << 53) && ahi >= (1ll << 52) && ahi < (1ll << 53);
+    }
+    return i;
+}
+
+// One chunk of the stitch in integer arithmetic on the bits u of the exact
+// start: r = I mod 2^P, delta = I - (base + r) (a multiple of 2^P), and the
+// end E[r] + delta ulp(start) as the integer of E[r]'s binade.  False when
+// the start is not in the predicted binade or the end leaves E[r]'s.
+__device__ __forceinline__ bool stitch_step(unsigned long long& u, unsigned key, int P,
+                                            long long base, double Eg) {
+    if ((unsigned)(u >> 52) != key) return false;
+    const long long m = (long long)((u & ((1ull << 52) - 1)) | (1ull << 52));
+    const long long I = (u >> 63) ? -m : m;
+    const int r = (int)(I & ((1ll << P) - 1));
+    const long long delta = I - base - r;
+    const unsigned long long e = ubits(rl_d(Eg, r));
+    const int ee = (int)((e >> 52) & 0x7ff), es = (int)((u >> 52) & 0x7ff);
+    if (ee == 0 || ee == 0x7ff) return false;
+    const long long me = (long long)((e & ((1ull << 52) - 1)) | (1ull << 52));
+    const long long IE = (e >> 63) ? -me : me;
+    const int k = es - ee;
+    long long D;
+    if (k >= 0) {
+        if (k > 10 || delta >= (1ll << 50) || delta <= -(1ll << 50)) return false;
+        D = delta * (1ll << k);
+    } else {
+        D = delta >> (-k);  // exact: delta is a multiple of 2^P and -k < P
+    }
+    const long long Ie = IE + D;
+    if ((Ie < 0) != (IE < 0)) return false;
+    const long long a = Ie < 0 ? -Ie : Ie;
+    if (a < (1ll << 52) || a >= (1ll << 53)) return false;
+    u = rfl_u((e & (0xfffull << 52)) | ((unsigned long long)a & ((1ull << 52) - 1)));
+    return true;
+}
+
+// One wavefront per sequence carries the exact running sum from chunk to
+// chunk.  Lane r holds tab[c][r] of the STITCH_G chunks in hand, so E[r] is a
+// read of lane r; the next round's tables load meanwhile.  A chunk of zeros
+// leaves the sum as it is: s + 0 and s - 0 are s for every s but -0 and NaN,
+// and the sum, which starts at +0, is never -0 (the last bits are checked by
+// the verification anyway).  The launch asks
+// for STITCH_LDS_BYTES of LDS so that each stitch has a CU of its own (they
+// are serial chains of scalar work that would otherwise share a CU's scalar
+// unit).
+__global__ void __launch_bounds__(64) pk_stitch_kernel(const double* __restrict__ terms, long npad,
+                                                       const Peak* __restrict__ pk,
+                                                       const double* __restrict__ tab,
+                                                       double* __restrict__ starts,
+                                                       unsigned long long* __restrict__ flags,
                                                        double* __restrict__ sums,
                                                        long long* __restrict__ stats) {
-    __shared__ double buf[2][SEQ_CHAIN_ROW];
-    __shared__ int s_bad[4];
-    double* blk = buf[0];
+    const long nc = npad / SEQ_CBLK;
     const int q = blockIdx.x;
-    // how many of the sequence's 1024-blocks are not of one binade
-    {
-        const long nb1 = npad / SEQ_BLOCK1;
-        int bad = 0;
-        for (long b = threadIdx.x; b < nb1; b += 256)
-            bad += (d1[(long)q * nb1 + b].flags & (SEQ_VALID | SEQ_ZERO)) == 0;
-#pragma unroll
-        for (int o = 32; o > 0; o >>= 1) bad += __shfl_xor(bad, o, 64);
-        if ((threadIdx.x & 63) == 0) s_bad[threadIdx.x >> 6] = bad;
-        __syncthreads();
-    }
-    const long n1_ = npad / SEQ_BLOCK1;
-    if (2L * (s_bad[0] + s_bad[1] + s_bad[2] + s_bad[3]) >= n1_) {
-        // at least half the 1024-blocks leave their binade: the walk would
-        // add most terms one by one at a slower pace than the plain chain
-        const long long t0 = stats ? clock64() : 0;
-        const double s = seq_chain_group(terms + (long)q * npad, npad, buf);
-        if (threadIdx.x == 0) {
-            sums[2 * q] = s;
-            sums[2 * q + 1] = 0.0;
-            if (stats) {
-                for (int u = 0; u < 9; ++u) stats[9 * q + u] = 0;
-                stats[9 * q + 4] = npad / SEQ_CHUNK;
-                stats[9 * q + 6] = clock64() - t0;
-            }
-        }
-        return;
-    }
-    if (threadIdx.x >= 64) return;  // the walk is one wavefront's
     const int lane = threadIdx.x;
-    // counters (only when stats != nullptr): applied 16384-blocks, 1024-blocks,
-    // segments, chunks; chunks added term by term; 1024-blocks opened; clock
-    // cycles in total, in term-by-term adds, in opened 1024-blocks
-    long long cnt[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    const double* x = terms + (long)q * npad;
+    const Peak* pkq = pk + (long)q * nc;
+    const double* tq = tab + (long)q * nc * TAB;
+    double* sq = starts + (long)q * (nc + 1);
+    if (lane == 0) flags[q] = 0ull;
     const bool want = stats != nullptr;
-    const long long t_start = want ? clock64() : 0;
-    const long n1 = npad / SEQ_BLOCK1, n2 = npad / SEQ_BLOCK2;
-    SeqDesc none;
-    none.D = none.lo = none.hi = 0;
-    none.k = 0;
-    none.flags = 0;
-    long long sb = 0;  // +0.0
-    SeqBlockData bufA, bufB;
-    for (long base2 = 0; base2 < n2; base2 += 64) {
-        const SeqDesc my2 = base2 + lane < n2 ? d2[(long)q * n2 + base2 + lane] : none;
-        const int cnt2 = n2 - base2 < 64 ? (int)(n2 - base2) : 64;
-        for (int i = 0; i < cnt2; ++i) {
-            if (seq_apply(rldesc(my2, i), sb)) {
-                cnt[0] += 1;
-                continue;
-            }
-            const long base1 = (base2 + i) * 16;
-            const SeqDesc my1 = lane < 16 ? d1[(long)q * n1 + base1 + lane] : none;
-            // block j's data is in bufA when inA, else in bufB; block j + 1 is
-            // loaded into the other set before block j is tried
-            load_block(terms, npad, dseg, q, n1, base1, bufA);
-            bool inA = true;
-            for (int j = 0; j < 16; ++j) {
-                if (j + 1 < 16) {
-                    if (inA)
-                        load_block(terms, npad, dseg, q, n1, base1 + j + 1, bufB);
-                    else
-                        load_block(terms, npad, dseg, q, n1, base1 + j + 1, bufA);
+    const long long t0 = want ? (long long)clock64() : 0;
+    long long n_small = 0, n_big = 0, n_fall = 0, cyc_fall = 0, n_zero = 0;
+    // the head: the terms up to the first block's peak, from 0
+    // (run_wave returns the same value in every lane; readfirstlane tells the
+    // compiler so, which keeps the stitch's integer work on the scalar unit)
+    unsigned long long u = rfl_u(ubits(run_wave(x, 0, pkq[0].p + 1, 0.0)));
+    double E[STITCH_G], En[STITCH_G];
+#pragma unroll
+    for (int g = 0; g < STITCH_G; ++g) E[g] = g < nc ? tq[(long)g * TAB + lane] : 0.0;
+    Chunk mine = chunk_of(pkq, nc, npad, lane < nc ? lane : 0);
+    for (long c0 = 0; c0 < nc; c0 += STITCH_G) {
+        const int g_n = nc - c0 < STITCH_G ? (int)(nc - c0) : STITCH_G;
+        const long c1 = c0 + STITCH_G;
+        // the next round's tables and chunk bounds, in flight during this round
+#pragma unroll
+        for (int g = 0; g < STITCH_G; ++g) En[g] = c1 + g < nc ? tq[(c1 + g) * TAB + lane] : 0.0;
+        const Chunk next = chunk_of(pkq, nc, npad, c1 + lane < nc ? c1 + lane : 0);
+        const StitchInfo info = stitch_info(mine);
+        double mystart = 0.0;
+#pragma unroll
+        for (int g = 0; g < STITCH_G; ++g) {
+            if (g < g_n) {
+                if (lane == g) mystart = __longlong_as_double((long long)u);
+                const int P = __builtin_amdgcn_readlane(info.P, g);
+                const bool ok = __builtin_amdgcn_readlane(info.ok, g) != 0;
+                const unsigned key = (unsigned)__builtin_amdgcn_readlane((int)info.key, g);
+                const long long base = rl_l(info.base, g);
+                const bool zero = __builtin_amdgcn_readlane(info.zero, g) != 0;
+                const bool nan = (u & (0x7ffull << 52)) == (0x7ffull << 52) &&
+                                 (u & ((1ull << 52) - 1)) != 0;
+                if (P <= 3) ++n_small;
+                else if (P <= SEQ_PMAX) ++n_big;
+                if (zero && !nan) {
+                    ++n_zero;
+                } else if (!(ok && stitch_step(u, key, P, base, E[g]))) {
+                    const long first = rl_l(mine.first, g);
+                    const long last = rl_l(mine.last, g);
+                    const long long tf = want ? (long long)clock64() : 0;
+                    u = rfl_u(ubits(run_wave(x, first, last - first + 1,
+                                             __longlong_as_double((long long)u))));
+                    if (want) cyc_fall += (long long)clock64() - tf;
+                    ++n_fall;
                 }
-                if (seq_apply(rldesc(my1, j), sb)) {
-                    cnt[1] += 1;
-                    inA = !inA;
-                    continue;
-                }
-                const long long t_open = want ? clock64() : 0;
-                cnt[5] += 1;
-                SeqDesc sgl;
-                if (inA) {
-                    seq_stage(bufA, blk);
-                    sgl = bufA.sg;
-                } else {
-                    seq_stage(bufB, blk);
-                    sgl = bufB.sg;
-                }
-                inA = !inA;
-                seq_walk_segments(sgl, blk, sb, cnt, want);
-                if (want) cnt[8] += clock64() - t_open;
             }
         }
+        if (lane < g_n) sq[c0 + lane] = mystart;
+#pragma unroll
+        for (int g = 0; g < STITCH_G; ++g) E[g] = En[g];
+        mine = next;
     }
+    const double s = __longlong_as_double((long long)u);
     if (lane == 0) {
-        sums[2 * q] = __longlong_as_double(sb);
+        sq[nc] = s;
+        sums[2 * q] = s;
         sums[2 * q + 1] = 0.0;
+        if (want) {
+            long long* o = stats + (long)SEQ_STATS * q;
+            o[0] = nc;
+            o[1] = n_small;
+            o[2] = n_big;
+            o[3] = n_fall;
+            o[4] = 0;
+            o[5] = (long long)clock64() - t0;
+            o[6] = cyc_fall;
+            o[7] = 0;
+            o[8] = n_zero;
+        }
     }
-    if (want && lane == 0) {
-        cnt[6] = clock64() - t_start;
-        for (int u = 0; u < 9; ++u) stats[9 * q + u] = cnt[u];
+}
+
+// One lane per chunk: run it from its exact start; it must reach the next
+// chunk's start.  A miss marks the sequence for the chain.
+__global__ void __launch_bounds__(256) pk_verify_kernel(const double* __restrict__ terms,
+                                                        long npad, const Peak* __restrict__ pk,
+                                                        const double* __restrict__ starts,
+                                                        unsigned long long* __restrict__ flags,
+                                                        long long* __restrict__ stats) {
+    const long nc = npad / SEQ_CBLK;
+    const long c = (long)blockIdx.x * 256 + threadIdx.x;
+    const int q = blockIdx.y;
+    const double* sq = starts + (long)q * (nc + 1);
+    Chunk k{0, 0, 0.0, 0, 0};
+    long len = 0;
+    if (c < nc) {
+        k = chunk_of(pk + (long)q * nc, nc, npad, c);
+        len = k.last - k.first + 1;
+    }
+    const long maxlen = wave_max_long(len);
+    if (maxlen == 0) return;  // uniform across the wavefront
+    const long minlen = wave_min_long(c < nc ? len : maxlen);
+    const double s = run_lane(terms + (long)q * npad, c < nc ? k.first : 0,
+                              c < nc ? len : maxlen, minlen, maxlen, c < nc ? sq[c] : 0.0);
+    if (c < nc && ubits(s) != ubits(sq[c + 1])) {
+        flags[q] = 1ull;
+        if (stats) atomicAdd((unsigned long long*)&stats[(long)SEQ_STATS * q + 4], 1ull);
+    }
+}
+
+// The chain, for the sequences whose verification failed.
+__global__ void __launch_bounds__(256) pk_redo_kernel(const double* __restrict__ terms, long npad,
+                                                      const unsigned long long* __restrict__ flags,
+                                                      double* __restrict__ sums,
+                                                      long long* __restrict__ stats) {
+    __shared__ double buf[2][SEQ_CHAIN_ROW];
+    const int q = blockIdx.x;
+    if (flags[q] == 0ull) return;  // uniform across the workgroup
+    const double s = seq_chain_group(terms + (long)q * npad, npad, buf);
+    if (threadIdx.x == 0) {
+        sums[2 * q] = s;
+        sums[2 * q + 1] = 0.0;
+        if (stats) stats[(long)SEQ_STATS * q + 7] = 1;
     }
 }
 
@@ -622,10 +673,11 @@ __global__ void __launch_bounds__(256) seq_walk_kernel(const double* __restrict_
 
 long seqsum_npad(long n) { return (n + SEQ_BLOCK2 - 1) / SEQ_BLOCK2 * SEQ_BLOCK2; }
 
+// bsum and bpre (2 nb each), peaks (2 nc), tables (64 nc), chunk starts
+// (nc + 1) and a flag per sequence
 long seqsum_workspace_len(long npad, int nq) {
-    const long n0 = npad / SEQ_CHUNK, n1 = npad / SEQ_BLOCK1, n2 = npad / SEQ_BLOCK2;
-    const long dsz = (long)(sizeof(SeqDesc) / sizeof(double));
-    return (long)nq * (2 * n1 + dsz * (n0 + n1 + n2));
+    const long nb = npad / SEQ_BLOCK1, nc = npad / SEQ_CBLK;
+    return (long)nq * (4 * nb + 2 * nc + (long)TAB * nc + nc + 2);
 }
 
 hipError_t launch_seqsum(int method, const double* terms, long npad, int nq, double* ws,
@@ -636,21 +688,34 @@ hipError_t launch_seqsum(int method, const double* terms, long npad, int nq, dou
                            sums);
         return hipGetLastError();
     }
-    if (method != SEQSUM_COMPOSITE) return hipErrorInvalidValue;
-    const long n0 = npad / SEQ_CHUNK, n1 = npad / SEQ_BLOCK1, n2 = npad / SEQ_BLOCK2;
-    double* bs = ws;
-    double* pre = bs + (long)nq * n1;
-    SeqDesc* dseg = reinterpret_cast<SeqDesc*>(pre + (long)nq * n1);
-    SeqDesc* d1 = dseg + (long)nq * n0;
-    SeqDesc* d2 = d1 + (long)nq * n1;
-    const dim3 g1((unsigned)((n1 + 3) / 4), (unsigned)nq);
-    hipLaunchKernelGGL(seq_blocksum_kernel, g1, dim3(256), 0, st, terms, npad, bs);
-    hipLaunchKernelGGL(seq_scan_kernel, dim3((unsigned)nq), dim3(1024), 0, st, bs, n1, pre);
-    hipLaunchKernelGGL(seq_prep_kernel, g1, dim3(256), 0, st, terms, npad, pre, dseg, d1);
-    hipLaunchKernelGGL(seq_level2_kernel, dim3((unsigned)((n2 * nq + 255) / 256)), dim3(256), 0,
-                       st, d1, n2, nq, d2);
-    hipLaunchKernelGGL(seq_walk_kernel, dim3((unsigned)nq), dim3(256), 0, st, terms, npad, dseg, d1,
-                       d2, sums, stats);
+    if (method != SEQSUM_PEAKS) return hipErrorInvalidValue;
+    // the chunk runs read up to 16 doubles past a sequence: the workspace
+    // must follow the terms
+    if (ws < terms + (long)nq * npad) return hipErrorInvalidValue;
+    const long nb = npad / SEQ_BLOCK1, nc = npad / SEQ_CBLK;
+    double* bsum = ws;
+    double* bpre = bsum + 2 * (long)nq * nb;
+    Peak* pk = reinterpret_cast<Peak*>(bpre + 2 * (long)nq * nb);
+    double* tab = reinterpret_cast<double*>(pk + (long)nq * nc);
+    double* starts = tab + (long)TAB * nq * nc;
+    unsigned long long* flags =
+        reinterpret_cast<unsigned long long*>(starts + (long)nq * (nc + 1));
+    const dim3 gb((unsigned)(nb / 4), (unsigned)nq);  // 4 wavefronts per 4096-block
+    hipLaunchKernelGGL(pk_blocksum_kernel, gb, dim3(256), 0, st, terms, npad, bsum);
+    hipLaunchKernelGGL(pk_scan_kernel, dim3((unsigned)nq), dim3(1024), 0, st, bsum, nb, bpre);
+    hipLaunchKernelGGL(pk_peak_kernel, gb, dim3(256), 0, st, terms, npad, bpre, pk);
+    hipLaunchKernelGGL((pk_tab_kernel<2, 1, 1>), dim3((unsigned)((nc + 127) / 128), (unsigned)nq),
+                       dim3(256), 0, st, terms, npad, pk, tab);
+    hipLaunchKernelGGL((pk_tab_kernel<8, 2, 3>), dim3((unsigned)((nc + 31) / 32), (unsigned)nq),
+                       dim3(256), 0, st, terms, npad, pk, tab);
+    hipLaunchKernelGGL((pk_tab_kernel<64, 4, SEQ_PMAX>), dim3((unsigned)((nc + 3) / 4), (unsigned)nq),
+                       dim3(256), 0, st, terms, npad, pk, tab);
+    hipLaunchKernelGGL(pk_stitch_kernel, dim3((unsigned)nq), dim3(64), STITCH_LDS_BYTES, st, terms,
+                       npad, pk, tab, starts, flags, sums, stats);
+    hipLaunchKernelGGL(pk_verify_kernel, dim3((unsigned)((nc + 255) / 256), (unsigned)nq),
+                       dim3(256), 0, st, terms, npad, pk, starts, flags, stats);
+    hipLaunchKernelGGL(pk_redo_kernel, dim3((unsigned)nq), dim3(256), 0, st, terms, npad, flags,
+                       sums, stats);
     return hipGetLastError();
 }
 

@@ -4,50 +4,44 @@
 // way (gravity_solver_3d.c:182-198, 226-258).
 //
 // Two methods with the same result bits:
-//   SEQSUM_CHAIN      one wavefront per sequence adds the terms in order.
-//   SEQSUM_COMPOSITE  while a running sum stays inside one binade
-//                     [2^k, 2^(k+1)) (one sign) every addition rounds to the
-//                     fixed grid u = 2^(k-52), so s + x rounds to
-//                     s + u * rint(x / u) whenever the exact s + x stays in
-//                     the binade, independently of s (rint: nearest integer;
-//                     exact halves are left to the chain).  A parallel pass
-//                     predicts each 16-term chunk's binade from approximate
-//                     prefix sums and stores, for runs of chunks of one
-//                     binade inside a 1024-term block (segments) and for
-//                     blocks of 1024 and 16384 terms of one binade, the
-//                     integer increment D and the range [lo, hi] of
-//                     S_j + floor(x_j/u) relative to the entry S = s/u.  A
-//                     walk then moves s block by block: a block applies when
-//                     s lies in its binade and S + lo >= 2^52, S + hi < 2^53
-//                     (every exact partial sum stayed in the binade);
-//                     otherwise it opens the block, down to segments, and
-//                     adds the terms of a segment that does not apply one by
-//                     one.  A sequence with at least half of its 1024-blocks
-//                     not of one binade (a sum that keeps crossing binades)
-//                     is added by the chain instead.
+//   SEQSUM_CHAIN  one wavefront per sequence adds the terms in order.
+//   SEQSUM_PEAKS  chunked with tabulated increments.  Write a double s of
+//                 binade e (sign and exponent field) as s = I u, I a signed
+//                 integer with 2^52 <= |I| < 2^53, u = ulp(s).  An addition
+//                 rounds s + x to the grid 2^m u of the result's binade;
+//                 while the result stays in that binade, adding an even
+//                 multiple of 2^m u to s shifts the result by exactly as
+//                 much (ties to even included), so RN(s + x) - s depends on s
+//                 only through I mod 2^(m+1).  Over additions that follow one
+//                 path of binades the increment depends only on I mod 2^P,
+//                 P = 1 + how far the path rises above the start binade.
+//                 Chunks therefore start at the running sum's predicted peak
+//                 in each 4096-term block (from double-double block sums)
+//                 and run to the next block's peak; each chunk is run from
+//                 2^P start values near its prediction, one per residue
+//                 (P <= 6), and a serial stitch carries the exact sum across
+//                 the chunks: end = E[I mod 2^P] + (start - variant start),
+//                 both exact when the binades are as predicted (checked; a
+//                 chunk that fails is added term by term; a chunk of zeros
+//                 leaves the sum as it is).  Every chunk is
+//                 then run again from its exact start, all in parallel, and
+//                 must land on the next chunk's start; if any does not, the
+//                 sequence is redone by the chain.  Predictions only decide
+//                 speed; the result is always the chain's.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 namespace hyd {
 
-enum SeqSumMethod { SEQSUM_CHAIN = 0, SEQSUM_COMPOSITE = 1 };
+enum SeqSumMethod { SEQSUM_CHAIN = 0, SEQSUM_PEAKS = 1 };
 
-constexpr int SEQ_CHUNK = 16;                    // terms per chunk (one lane)
+constexpr int SEQ_CHUNK = 16;                    // terms per lane in a block pass
 constexpr int SEQ_BLOCK1 = 64 * SEQ_CHUNK;        // 1024 terms (one wavefront)
-constexpr int SEQ_BLOCK2 = 16 * SEQ_BLOCK1;       // 16384 terms
-
-struct SeqDesc {
-    long long D;   // increment of S over the block, in units of 2^(k-52)
-    long long lo;  // min over terms of (increment so far + floor(x/u))
-    long long hi;  // max of the same
-    int k;         // binade of the running sum
-    int flags;     // SEQ_VALID, SEQ_NEG (running sum negative), SEQ_ZERO
-};
-constexpr int SEQ_VALID = 1;
-constexpr int SEQ_NEG = 2;
-constexpr int SEQ_ZERO = 4;  // every term is +-0: the block leaves any s != -0 as it is
-constexpr int SEQ_START = 8;  // a segment starts at this chunk (segment entries only)
+constexpr int SEQ_CBLK = 4 * SEQ_BLOCK1;          // 4096 terms: one chunk starts in each
+constexpr int SEQ_BLOCK2 = 16 * SEQ_BLOCK1;       // term arrays are padded to this
+constexpr int SEQ_PMAX = 6;                      // residue bits tabulated per chunk
+constexpr int SEQ_STATS = 9;                     // counters per sequence
 
 // Length of the term arrays: n rounded up to a multiple of SEQ_BLOCK2, the
 // tail padded with zeros by the caller.
@@ -55,10 +49,11 @@ long seqsum_npad(long n);
 // Workspace (doubles) of launch_seqsum for nq sequences of npad terms.
 long seqsum_workspace_len(long npad, int nq);
 // sums[2q] = the sequential sum of terms[q * npad .. + npad), sums[2q+1] = 0.
-// stats (device, 9 per sequence, or nullptr): what the composite walk did --
-// blocks of 16384 and 1024, segments and chunks applied, chunks added term by
-// term, 1024-blocks opened, clock cycles in all, in term-by-term adds and in
-// opened 1024-blocks.
+// stats (device, SEQ_STATS per sequence, or nullptr) for SEQSUM_PEAKS:
+// chunks; chunks with P = 1, 2 or 3; with P = 4..6; chunks added term by
+// term in the stitch; chunks whose verification failed; clock cycles of the
+// stitch; of its term-by-term additions; 1 when the chain redid the sum;
+// chunks of zeros passed through.
 hipError_t launch_seqsum(int method, const double* terms, long npad, int nq, double* ws,
                          double* sums, hipStream_t st, long long* stats = nullptr);
 

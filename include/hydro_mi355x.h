@@ -94,11 +94,14 @@ int hydro_fill_initial_state(int nx, int ny, int nz, double xmin, double xmax, d
 /* Diagnostic: the sum ((0 + x[0]) + x[1]) + ... of n doubles (host array)
  * with every addition rounded in turn, as the reference's moment loops add
  * (gravity_solver_3d.c:182-198), computed on the device by method 0 (one
- * chain of additions) or 1 (block increments with exact checks, falling back
- * to the chain where a block leaves its binade).  stats (9 values, or NULL)
- * receives method 1's walk counters: blocks of 16384 and 1024, segments and
- * chunks applied, chunks added term by term, 1024-blocks opened, and clock
- * cycles in all, in term-by-term adds and in opened 1024-blocks.
+ * chain of additions) or 1 (chunks that start at the running sum's predicted
+ * peaks, each run from one start value per residue of the start's low bits,
+ * a serial stitch of the exact sum through those tables and a parallel
+ * re-run of every chunk from its exact start; any mismatch redoes the sum by
+ * the chain).  stats (9 values, or NULL) receives method 1's counters:
+ * chunks, chunks with 1-3 and with 4-6 residue bits, chunks the stitch added
+ * term by term, chunks whose re-run missed, clock cycles of the stitch and of
+ * its term-by-term additions, 1 when the chain redid the sum.
  * *status = 0 on success. */
 double hydro_sequential_sum(const double *x, long n, int method, long long *stats, int *status);
 
