@@ -208,15 +208,16 @@ def initial_state(problem, shape, box=(0.0, 1.0, 0.0, 1.0, 0.0, 1.0), gamma=5.0 
 
 
 SEQSUM_STATS = ("chunks", "chunks_p1_3", "chunks_p4_6", "chunks_added", "verify_failed",
-                "cycles_stitch", "cycles_adding", "chain_redo", "chunks_zero")
+                "cycles_stitch", "groups_walked", "chain_redo", "chunks_zero")
 
 
 def sequential_sum(x, method=1, stats=False):
     """((0 + x[0]) + x[1]) + ... rounded after every addition, computed on
     the device: method 0 adds the terms in one chain, method 1 by chunks
-    with tabulated increments, a serial stitch and a parallel verification
-    (the moment sums of MOMENT_ORDER_SEQUENTIAL use method 1).  stats=True
-    also returns method 1's counters as a dict."""
+    with tabulated, certified increments carried group by group (the moment
+    sums of MOMENT_ORDER_SEQUENTIAL use it), method 2 as 1 and then re-runs
+    every chunk from its exact start (a check of the certificates).
+    stats=True also returns methods 1 and 2's counters as a dict."""
     x = np.ascontiguousarray(x, dtype=np.float64)
     st = C.c_int(-1)
     cnt = (C.c_longlong * 9)()

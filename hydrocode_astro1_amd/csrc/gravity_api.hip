@@ -97,7 +97,9 @@ int dev_create(GravitySolver3D* s) {
     if (!d->stream) return -2;
     if (const char* e = getenv("HYDRO_MI355X_MG_TAIL")) d->mg_use_tail = atoi(e) != 0;
     if (const char* e = getenv("HYDRO_MI355X_SEQSUM"))
-        d->seq_method = strcmp(e, "chain") == 0 ? hyd::SEQSUM_CHAIN : hyd::SEQSUM_PEAKS;
+        d->seq_method = strcmp(e, "chain") == 0     ? hyd::SEQSUM_CHAIN
+                        : strcmp(e, "checked") == 0 ? hyd::SEQSUM_PEAKS_CHECKED
+                                                    : hyd::SEQSUM_PEAKS;
     if (const char* e = getenv("HYDRO_MI355X_MG_SMOOTH")) {
         int a = 0, b = 0;
         float w = 0.0f;

@@ -46,7 +46,7 @@ struct GravDevice {
     long mom_scratch_len = 0;
     double *mom_seq = nullptr;  // stored terms and sums of the sequential order
     long mom_seq_len = 0;
-    int seq_method = hyd::SEQSUM_PEAKS;  // HYDRO_MI355X_SEQSUM=chain: one add chain
+    int seq_method = hyd::SEQSUM_PEAKS;  // HYDRO_MI355X_SEQSUM=chain or checked
     double *wlm = nullptr;      // multipole weights
     hyd::SorCtl *ctl = nullptr;
     hyd::SorCtl *ctl_host = nullptr;  // pinned readback

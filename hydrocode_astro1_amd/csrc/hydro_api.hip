@@ -539,7 +539,7 @@ double hydro_sequential_sum(const double* x, long n, int method, long long* stat
                 hipSuccess &&
             hipMemcpyAsync(&out, buf + len - 11, sizeof out, hipMemcpyDeviceToHost, st) ==
                 hipSuccess &&
-            (!stats || method != hyd::SEQSUM_PEAKS ||
+            (!stats || method == hyd::SEQSUM_CHAIN ||
              hipMemcpyAsync(stats, dstats, 9 * sizeof(long long), hipMemcpyDeviceToHost, st) ==
                  hipSuccess) &&
             hipStreamSynchronize(st) == hipSuccess)

@@ -1,17 +1,23 @@
 // seqsum.hip -- sequential-order sums (see seqsum.hpp for the method).
 //
 // SEQSUM_PEAKS, per sequence q of npad terms (nb = npad / 1024 blocks,
-// nc = npad / 4096 chunks):
-//   pk_blocksum_kernel  double-double sum of every 1024-term block
-//   pk_scan_kernel      exclusive prefix of the block sums (double-double)
-//   pk_peak_kernel      per 4096-term block: the predicted running sum after
-//                       each term, where its magnitude peaks (chunk starts),
-//                       and whether every term is zero
-//   pk_tab_kernel       one start value per residue of each chunk, run over
-//                       the chunk's terms: 2 lanes per chunk for P = 1, 8
-//                       for P = 2, 3, a wavefront for P = 4..6
+// nc = npad / 4096 chunks, ng = nc / 16 groups of chunks):
+//   pk_block_kernel     per 1024-term block: double-double sum, largest and
+//                       smallest in-block prefix, all-zero flag
+//   pk_scan_kernel      prefixes of the block sums and, per 4096-term block,
+//                       the predicted peak of the running sum's magnitude
+//                       (chunk starts)
+//   pk_class_kernel     the list of start values to tabulate, 2^P per chunk,
+//                       longest chunks first
+//   pk_tab_kernel       a lane per start value runs it through the chunk's
+//                       terms: the end value and its exactness certificate
+//   pk_group_kernel     per group of 16 chunks and start residue: where the
+//                       group ends and how far a start may move (the map)
 //   pk_stitch_kernel    one wavefront per sequence carries the exact sum
-//                       across the chunks (term by term where a chunk fails)
+//                       from group to group through the maps (chunk by
+//                       chunk, term by term where checks fail)
+// SEQSUM_PEAKS_CHECKED (and any call that asks for counters) also runs
+//   pk_fill_kernel      per group: the exact start of every chunk
 //   pk_verify_kernel    one lane per chunk runs it again from its exact start
 //   pk_redo_kernel      the chain, only for a sequence whose check failed
 #include "seqsum.hpp"
@@ -43,22 +49,6 @@ __device__ __forceinline__ long long imant(double d) {
     const unsigned long long u = ubits(d);
     const long long m = (long long)((u & ((1ull << 52) - 1)) | (1ull << 52));
     return (u >> 63) ? -m : m;
-}
-__device__ __forceinline__ double with_imant(long long i, double like) {
-    const unsigned long long a = (unsigned long long)(i < 0 ? -i : i);
-    return dfrom((ubits(like) & (0x7ffull << 52)) | (a & ((1ull << 52) - 1)) |
-                 (i < 0 ? (1ull << 63) : 0ull));
-}
-// The start value of residue r (I mod 2^P) nearest below V's integer, kept
-// inside V's binade.
-__device__ __forceinline__ double variant_start(double V, int P, int r) {
-    const long long mod = 1ll << P;
-    long long i = imant(V);
-    i = i - (i & (mod - 1)) + r;
-    const long long a = i < 0 ? -i : i;
-    if (a < (1ll << 52)) i += i > 0 ? mod : -mod;
-    else if (a >= (1ll << 53)) i -= i > 0 ? mod : -mod;
-    return with_imant(i, V);
 }
 // Residue bits of a chunk from its predicted start (peak) V and the next
 // chunk's predicted peak V1: 1 + how far V1's binade lies above V's, or BADP.
@@ -196,39 +186,104 @@ __device__ __forceinline__ void load_chunk(const double* __restrict__ p, double 
     }
 }
 
-// Double-double sum of each 1024-term block (a wavefront per block).
-__global__ void __launch_bounds__(256) pk_blocksum_kernel(const double* __restrict__ terms,
-                                                          long npad, double* __restrict__ bsum) {
+// Per 1024-term block (a wavefront): the double-double sum, the largest and
+// smallest in-block prefix (the running sum relative to the block's start)
+// with their first positions, and whether every term is zero.  The peak of
+// |base + prefix| over a block lies at one of the two extremes, so the
+// prefixes of the block sums then place every chunk start without reading
+// the terms again.
+struct BlockStat {
+    double hi, lo;      // block sum
+    double vmax, vmin;  // largest and smallest in-block prefix
+    int pmax, pmin;     // their first positions
+    int zero, pad;
+};
+
+__global__ void __launch_bounds__(256) pk_block_kernel(const double* __restrict__ terms, long npad,
+                                                       BlockStat* __restrict__ bs) {
     const long nb = npad / SEQ_BLOCK1;
-    const long j = (long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    const long j = (long)blockIdx.x * 4 + (threadIdx.x >> 6);  // < nb: nb is a multiple of 16
     const int lane = threadIdx.x & 63;
-    if (j >= nb) return;
     const int q = blockIdx.y;
     double v[SEQ_CHUNK];
     load_chunk(terms + (long)q * npad + j * SEQ_BLOCK1 + lane * SEQ_CHUNK, v);
+    double pre[SEQ_CHUNK];
+    double run = 0.0;
     DD acc{0.0, 0.0};
+    bool zero = true;
 #pragma unroll
-    for (int e = 0; e < SEQ_CHUNK; ++e) acc = dd_add(acc, DD{v[e], 0.0});
+    for (int e = 0; e < SEQ_CHUNK; ++e) {
+        run = run + v[e];
+        pre[e] = run;
+        acc = dd_add(acc, DD{v[e], 0.0});
+        zero = zero && v[e] == 0.0;
+    }
+    DD incl = acc;
 #pragma unroll
-    for (int o = 32; o > 0; o >>= 1) acc = dd_add(acc, dd_shfl_xor(acc, o));
+    for (int o = 1; o < 64; o <<= 1) {
+        const DD u = dd_shfl_up(incl, o);
+        if (lane >= o) incl = dd_add(incl, u);
+    }
+    DD excl = dd_shfl_up(incl, 1);
+    if (lane == 0) excl = DD{0.0, 0.0};
+    double vmax = -__builtin_huge_val(), vmin = __builtin_huge_val();
+    int pmax = 0, pmin = 0;
+#pragma unroll
+    for (int e = 0; e < SEQ_CHUNK; ++e) {
+        const double rel = excl.hi + (excl.lo + pre[e]);
+        if (rel > vmax) {
+            vmax = rel;
+            pmax = lane * SEQ_CHUNK + e;
+        }
+        if (rel < vmin) {
+            vmin = rel;
+            pmin = lane * SEQ_CHUNK + e;
+        }
+    }
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) {
+        const double a = __shfl_xor(vmax, o, 64), b = __shfl_xor(vmin, o, 64);
+        const int pa = __shfl_xor(pmax, o, 64), pb = __shfl_xor(pmin, o, 64);
+        if (a > vmax || (a == vmax && pa < pmax)) {
+            vmax = a;
+            pmax = pa;
+        }
+        if (b < vmin || (b == vmin && pb < pmin)) {
+            vmin = b;
+            pmin = pb;
+        }
+    }
+    const int wz = __all(zero);
+    const double thi = __shfl(incl.hi, 63, 64), tlo = __shfl(incl.lo, 63, 64);
     if (lane == 0) {
-        bsum[2 * ((long)q * nb + j)] = acc.hi;
-        bsum[2 * ((long)q * nb + j) + 1] = acc.lo;
+        BlockStat r;
+        r.hi = thi;
+        r.lo = tlo;
+        r.vmax = vmax;
+        r.vmin = vmin;
+        r.pmax = pmax;
+        r.pmin = pmin;
+        r.zero = wz;
+        r.pad = 0;
+        bs[(long)q * nb + j] = r;
     }
 }
 
-// Exclusive prefix of the nb block sums of sequence blockIdx.x.
-__global__ void __launch_bounds__(1024) pk_scan_kernel(const double* __restrict__ bsum, long nb,
-                                                       double* __restrict__ bpre) {
+// Per sequence (blockIdx.x): the exclusive prefix of the block sums
+// (double-double), and per 4096-term block the predicted peak of the running
+// sum's magnitude (the first on ties) and whether all its terms are zero.
+__global__ void __launch_bounds__(1024) pk_scan_kernel(const BlockStat* __restrict__ bs, long nb,
+                                                       Peak* __restrict__ pk) {
     __shared__ double sh[1024], sl[1024];
     const int tid = threadIdx.x;
-    const long per = (nb + 1023) / 1024;
+    // whole 4096-term blocks (four 1024-term blocks) per thread
+    const long per = ((nb + 1023) / 1024 + 3) / 4 * 4;
     const long b = (long)tid * per;
     const long e = b + per < nb ? b + per : nb;
-    const double* in = bsum + 2 * (long)blockIdx.x * nb;
-    double* out = bpre + 2 * (long)blockIdx.x * nb;
+    const BlockStat* in = bs + (long)blockIdx.x * nb;
+    Peak* out = pk + (long)blockIdx.x * (nb / 4);
     DD t{0.0, 0.0};
-    for (long i = b; i < e; ++i) t = dd_add(t, DD{in[2 * i], in[2 * i + 1]});
+    for (long i = b; i < e; ++i) t = dd_add(t, DD{in[i].hi, in[i].lo});
     sh[tid] = t.hi;
     sl[tid] = t.lo;
     __syncthreads();
@@ -245,94 +300,35 @@ __global__ void __launch_bounds__(1024) pk_scan_kernel(const double* __restrict_
     }
     DD run{0.0, 0.0};
     if (tid > 0) run = DD{sh[tid - 1], sl[tid - 1]};
-    for (long i = b; i < e; ++i) {
-        out[2 * i] = run.hi;
-        out[2 * i + 1] = run.lo;
-        run = dd_add(run, DD{in[2 * i], in[2 * i + 1]});
-    }
-}
-
-// Per 4096-term block (a workgroup, a wavefront per 1024 terms): the
-// predicted running sum after each term (block prefix plus the in-block
-// prefix), the first position of its largest magnitude, and whether all the
-// terms are zero.
-__global__ void __launch_bounds__(256) pk_peak_kernel(const double* __restrict__ terms, long npad,
-                                                      const double* __restrict__ bpre,
-                                                      Peak* __restrict__ pk) {
-    __shared__ double s_best[4], s_v[4];
-    __shared__ int s_k[4], s_zero[4];
-    const long nb = npad / SEQ_BLOCK1;
-    const int w = threadIdx.x >> 6;
-    const long j = (long)blockIdx.x * 4 + w;  // < nb: nb is a multiple of 16
-    const int lane = threadIdx.x & 63;
-    const int q = blockIdx.y;
-    double v[SEQ_CHUNK];
-    load_chunk(terms + (long)q * npad + j * SEQ_BLOCK1 + lane * SEQ_CHUNK, v);
-    double pre[SEQ_CHUNK];
-    double run = 0.0;
-    bool zero = true;
-#pragma unroll
-    for (int e = 0; e < SEQ_CHUNK; ++e) {
-        run = run + v[e];
-        pre[e] = run;
-        zero = zero && v[e] == 0.0;
-    }
-    DD incl{run, 0.0};
-#pragma unroll
-    for (int o = 1; o < 64; o <<= 1) {
-        const DD u = dd_shfl_up(incl, o);
-        if (lane >= o) incl = dd_add(incl, u);
-    }
-    DD excl = dd_shfl_up(incl, 1);
-    if (lane == 0) excl = DD{0.0, 0.0};
-    const long bi = (long)q * nb + j;
-    const DD base = dd_add(DD{bpre[2 * bi], bpre[2 * bi + 1]}, excl);
     double best = -1.0, bv = 0.0;
-    int bk = 0;
-#pragma unroll
-    for (int e = 0; e < SEQ_CHUNK; ++e) {
-        const double pv = base.hi + (base.lo + pre[e]);
-        const double a = fabs(pv);
+    int bp = 0, zero = 1;
+    for (long i = b; i < e; ++i) {
+        const BlockStat s = in[i];
+        const int w = (int)(i & 3);
+        if (w == 0) {
+            best = -1.0;
+            bv = 0.0;
+            bp = 0;
+            zero = 1;
+        }
+        const double cmax = run.hi + (run.lo + s.vmax), cmin = run.hi + (run.lo + s.vmin);
+        const double amax = fabs(cmax), amin = fabs(cmin);
+        const bool lo = amin > amax || (amin == amax && s.pmin < s.pmax);
+        const double a = lo ? amin : amax;
         if (a > best) {
             best = a;
-            bv = pv;
-            bk = w * SEQ_BLOCK1 + lane * SEQ_CHUNK + e;
+            bv = lo ? cmin : cmax;
+            bp = w * SEQ_BLOCK1 + (lo ? s.pmin : s.pmax);
         }
-    }
-#pragma unroll
-    for (int o = 32; o > 0; o >>= 1) {
-        const double ob = __shfl_xor(best, o, 64), ov = __shfl_xor(bv, o, 64);
-        const int ok = __shfl_xor(bk, o, 64);
-        if (ob > best || (ob == best && ok < bk)) {
-            best = ob;
-            bv = ov;
-            bk = ok;
+        zero = zero && s.zero;
+        if (w == 3) {
+            Peak r;
+            r.V = bv;
+            r.p = bp;
+            r.zero = zero;
+            out[i >> 2] = r;
         }
-    }
-    const int wzero = __all(zero);
-    if (lane == 0) {
-        s_best[w] = best;
-        s_v[w] = bv;
-        s_k[w] = bk;
-        s_zero[w] = wzero;
-    }
-    __syncthreads();
-    if (threadIdx.x == 0) {
-        // in term order: a later wavefront wins only with a larger magnitude
-        Peak r;
-        r.V = s_v[0];
-        r.p = s_k[0];
-        r.zero = s_zero[0];
-        double b = s_best[0];
-        for (int u = 1; u < 4; ++u) {
-            if (s_best[u] > b) {
-                b = s_best[u];
-                r.V = s_v[u];
-                r.p = s_k[u];
-            }
-            r.zero = r.zero && s_zero[u];
-        }
-        pk[(long)q * gridDim.x + blockIdx.x] = r;
+        run = dd_add(run, DD{s.hi, s.lo});
     }
 }
 
@@ -371,42 +367,6 @@ __device__ __forceinline__ long wave_max_long(long v) {
     }
     return v;
 }
-
-// s + x[first] + ... + x[first + len - 1], one lane's own run, the next 8
-// terms loaded while the current 8 are added.  Every lane of the wavefront
-// calls it: minlen and maxlen are the shortest and longest runs among the
-// lanes (a lane with nothing to add passes len = maxlen and any start).
-// Reads reach x[first + maxlen + 15]: the term arrays are followed by at
-// least that much readable memory (the workspace).
-__device__ __forceinline__ double run_lane(const double* __restrict__ x, long first, long len,
-                                           long minlen, long maxlen, double s) {
-    const double* p = x + first;
-    double cur[8], nxt[8];
-#pragma unroll
-    for (int i = 0; i < 8; ++i) cur[i] = p[i];
-    long base = 0;
-    for (; base + 8 <= minlen; base += 8) {  // every lane adds all 8
-#pragma unroll
-        for (int i = 0; i < 8; ++i) nxt[i] = p[base + 8 + i];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) s = s + cur[i];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) cur[i] = nxt[i];
-    }
-    for (; base < maxlen; base += 8) {  // the tails, predicated per lane
-#pragma unroll
-        for (int i = 0; i < 8; ++i) nxt[i] = p[base + 8 + i];
-#pragma unroll
-        for (int i = 0; i < 8; ++i) {
-            const double t = s + cur[i];
-            s = (base + i < len) ? t : s;
-        }
-#pragma unroll
-        for (int i = 0; i < 8; ++i) cur[i] = nxt[i];
-    }
-    return s;
-}
-
 __device__ __forceinline__ long wave_min_long(long v) {
 #pragma unroll
     for (int o = 32; o > 0; o >>= 1) {
@@ -416,63 +376,21 @@ __device__ __forceinline__ long wave_min_long(long v) {
     return v;
 }
 
-// End values of the chunks whose residue bits P lie in [PLO, PHI]: L lanes
-// per chunk (lane % L = residue), 64 / L chunks per wavefront.
-template <int L, int PLO, int PHI>
-__global__ void __launch_bounds__(256) pk_tab_kernel(const double* __restrict__ terms, long npad,
-                                                     const Peak* __restrict__ pk,
-                                                     double* __restrict__ tab) {
-    constexpr int CPW = 64 / L;
-    const long nc = npad / SEQ_CBLK;
-    const int lane = threadIdx.x & 63;
-    const long c = ((long)blockIdx.x * 4 + (threadIdx.x >> 6)) * CPW + lane / L;
-    const int r = lane % L;
-    const int q = blockIdx.y;
-    bool active = false;
-    Chunk k{0, 0, 0.0, BADP, 0};
-    if (c < nc) {
-        k = chunk_of(pk + (long)q * nc, nc, npad, c);
-        active = k.P >= PLO && k.P <= PHI && r < (1 << k.P);
-    }
-    const long len = active ? k.last - k.first + 1 : 0;
-    const long maxlen = wave_max_long(len);
-    if (maxlen == 0) return;  // uniform across the wavefront
-    const long minlen = wave_min_long(active ? len : maxlen);
-    double s = active ? variant_start(k.V, k.P, r) : 0.0;
-    s = run_lane(terms + (long)q * npad, active ? k.first : 0, active ? len : maxlen, minlen,
-                 maxlen, s);
-    if (active) tab[((long)q * nc + c) * TAB + r] = s;
-}
-
-// s + x[first] + ... in order: the wavefront loads 64 terms at a time (the
-// next 64 while the current ones are added) and the chain of additions runs
-// on the values read from lane i.  Returns the same value in every lane.
-__device__ __noinline__ double run_wave(const double* __restrict__ x, long first, long len,
-                                        double s) {
-    const int lane = threadIdx.x & 63;
-    double v = lane < len ? x[first + lane] : 0.0;
-    for (long base = 0; base < len; base += 64) {
-        const long cnt = len - base < 64 ? len - base : 64;
-        const double w = base + 64 + lane < len ? x[first + base + 64 + lane] : 0.0;
-        if (cnt == 64) {
-#pragma unroll
-            for (int i = 0; i < 64; ++i) s = s + rl_d(v, i);
-        } else {
-            for (int i = 0; i < (int)cnt; ++i) s = s + rl_d(v, i);
-        }
-        v = w;
-    }
-    return rfl_d(s);
-}
-
-constexpr int STITCH_G = 16;        // chunks per load round of the stitch
-// LDS bytes asked for each stitch workgroup (dynamic, unused): more than half
-// of a CU's 160 KB, so no two stitches share a CU
+constexpr int STITCH_G = 16;  // chunks per load round of a chunk-by-chunk stitch
+constexpr int GROUP = 16;     // chunks per group of the two-level stitch
+constexpr int MAP_BATCH = 8;  // groups whose maps pass through LDS together
+constexpr int PG = 6;         // residue bits of a group's start (one per lane)
+constexpr int NRES = 1 << PG; // maps per group
+// LDS bytes asked for each sequence-level stitch workgroup (the map ring,
+// 64 KB, and more): more than half of a CU's 160 KB, so no two share a CU
 constexpr int STITCH_LDS_BYTES = 96 * 1024;
+constexpr unsigned long long MANT = (1ull << 52) - 1;
+constexpr double ALLOW_MAX = 288230376151711744.0;  // 2^58, above any |delta|
 
-// What the stitch needs of a chunk: its start key (sign and exponent field),
-// residue bits, the start values' common base I_v - (I_v mod 2^P), and
-// whether base + r stays in the binade for every residue r.
+// What a stitch needs of a chunk: its start key (sign and exponent field),
+// residue bits, the start values' common base I_v - (I_v mod 2^P), whether
+// base + r stays in the binade for every residue r, and whether all its
+// terms are zero.
 struct StitchInfo {
     unsigned key;
     int P;
@@ -498,80 +416,333 @@ __device__ __forceinline__ StitchInfo stitch_info(const Chunk& k) {
     return i;
 }
 
-// One chunk of the stitch in integer arithmetic on the bits u of the exact
-// start: r = I mod 2^P, delta = I - (base + r) (a multiple of 2^P), and the
-// end E[r] + delta ulp(start) as the integer of E[r]'s binade.  False when
-// the start is not in the predicted binade or the end leaves E[r]'s.
-__device__ __forceinline__ bool stitch_step(unsigned long long& u, unsigned key, int P,
-                                            long long base, double Eg) {
-    if ((unsigned)(u >> 52) != key) return false;
-    const long long m = (long long)((u & ((1ull << 52) - 1)) | (1ull << 52));
-    const long long I = (u >> 63) ? -m : m;
-    const int r = (int)(I & ((1ll << P) - 1));
-    const long long delta = I - base - r;
-    const unsigned long long e = ubits(rl_d(Eg, r));
-    const int ee = (int)((e >> 52) & 0x7ff), es = (int)((u >> 52) & 0x7ff);
+constexpr int NBUCKET = 16;  // chunk length classes of the tabulation (512 terms each)
+
+// Lanes that tabulate chunk c, one per start value: 2^P (0 for a chunk of
+// zeros, a chunk whose start base leaves the binade, or P > 6), and its
+// length class.
+__device__ __forceinline__ int chunk_lanes(const Peak* __restrict__ pkq, long nc, long npad,
+                                           long c, int& bucket) {
+    const Chunk k = chunk_of(pkq, nc, npad, c);
+    const long len = k.last - k.first + 1;
+    bucket = len >> 9 < NBUCKET - 1 ? (int)(len >> 9) : NBUCKET - 1;
+    if (k.zero) return 0;
+    const StitchInfo i = stitch_info(k);
+    return i.ok ? 1 << i.P : 0;
+}
+
+// Per sequence (blockIdx.x): the list of tabulation lanes, one per start
+// value base + r of each tabulated chunk, entry c * 64 + r, the chunks
+// ordered from
+// the longest length class to the shortest (so that a wavefront's lanes run
+// about as long as each other, and the longest start first), at
+// lanes[q * 64 nc ...]; their number at counts[q].
+__global__ void __launch_bounds__(1024) pk_class_kernel(const Peak* __restrict__ pk, long npad,
+                                                        int* __restrict__ lanes,
+                                                        int* __restrict__ counts) {
+    __shared__ int sc[NBUCKET][1024];
+    __shared__ int base[NBUCKET + 1];
+    const long nc = npad / SEQ_CBLK;
+    const int q = blockIdx.x;
+    const int tid = threadIdx.x;
+    const Peak* pkq = pk + (long)q * nc;
+    const long per = (nc + 1023) / 1024;
+    const long b = (long)tid * per;
+    const long e = b + per < nc ? b + per : nc;
+    int cnt[NBUCKET];
+#pragma unroll
+    for (int k = 0; k < NBUCKET; ++k) cnt[k] = 0;
+    for (long c = b; c < e; ++c) {
+        int bk;
+        const int L = chunk_lanes(pkq, nc, npad, c, bk);
+#pragma unroll
+        for (int k = 0; k < NBUCKET; ++k) cnt[k] += k == bk ? L : 0;
+    }
+#pragma unroll
+    for (int k = 0; k < NBUCKET; ++k) sc[k][tid] = cnt[k];
+    __syncthreads();
+    for (int o = 1; o < 1024; o <<= 1) {
+        int add[NBUCKET];
+#pragma unroll
+        for (int k = 0; k < NBUCKET; ++k) add[k] = tid >= o ? sc[k][tid - o] : 0;
+        __syncthreads();
+#pragma unroll
+        for (int k = 0; k < NBUCKET; ++k) sc[k][tid] += add[k];
+        __syncthreads();
+    }
+    if (tid == 0) {  // the longest class first
+        int o = 0;
+        for (int k = NBUCKET - 1; k >= 0; --k) {
+            base[k] = o;
+            o += sc[k][1023];
+        }
+        base[NBUCKET] = o;
+        counts[q] = o;
+    }
+    __syncthreads();
+    int off[NBUCKET];
+#pragma unroll
+    for (int k = 0; k < NBUCKET; ++k) off[k] = base[k] + sc[k][tid] - cnt[k];
+    int* lq = lanes + (long)q * TAB * nc;
+    for (long c = b; c < e; ++c) {
+        int bk;
+        const int L = chunk_lanes(pkq, nc, npad, c, bk);
+        int o = 0;
+#pragma unroll
+        for (int k = 0; k < NBUCKET; ++k) {
+            o = k == bk ? off[k] : o;
+            off[k] += k == bk ? L : 0;
+        }
+        for (int r = 0; r < L; ++r) lq[o + r] = (int)(c * TAB + r);
+    }
+}
+
+// Adds x to each of the NCH sums s[r].  With CERT, also keeps for each the
+// highest exponent field met (em, the field in bits 20..30) and the
+// smallest distance mm of a sum to the edges of its binade [2^e, 2^(e+1))
+// (field 0: [0, 2^-1022)); both differences are exact.
+template <int NCH, bool CERT>
+__device__ __forceinline__ void add_all(double x, double (&s)[NCH], double (&mm)[NCH],
+                                        unsigned (&em)[NCH]) {
+#pragma unroll
+    for (int r = 0; r < NCH; ++r) {
+        s[r] = s[r] + x;
+        if (CERT) {
+            const unsigned e = (unsigned)(ubits(s[r]) >> 32) & 0x7ff00000u;
+            em[r] = e > em[r] ? e : em[r];
+            const double lo = dfrom((unsigned long long)e << 32);
+            const double up = dfrom((unsigned long long)(e + 0x00100000u) << 32);
+            const double a = fabs(s[r]);
+            // (a NaN here comes with em = 0x7ff, which voids the certificate)
+            mm[r] = fmin(mm[r], fmin(a - lo, up - a));
+        }
+    }
+}
+
+// Each lane adds its own len terms at p to its NCH sums, in order (len = 0:
+// nothing; a sum is never -0, so the +0 fed to a lane past its end leaves it
+// as it is).  32 terms are in flight while the previous 32 are added.
+// Reads reach p[max(len, 1) + 31]: the term arrays are followed by at least
+// that much readable memory (the workspace).
+template <int NCH, bool CERT>
+__device__ __forceinline__ void run_terms(const double* __restrict__ p, long len,
+                                          double (&s)[NCH], double (&mm)[NCH],
+                                          unsigned (&em)[NCH]) {
+    constexpr int BT = 32;
+    const long maxlen = wave_max_long(len);
+    const long minlen = wave_min_long(len > 0 ? len : maxlen);
+    double cur[BT], nxt[BT];
+#pragma unroll
+    for (int i = 0; i < BT; ++i) {
+        cur[i] = 0.0;
+        nxt[i] = 0.0;
+    }
+    if (len > 0) {
+#pragma unroll
+        for (int i = 0; i < BT; ++i) cur[i] = p[i];
+    }
+    long base = 0;
+    for (; base + BT <= minlen; base += BT) {  // every lane with terms adds all 16
+        if (len > 0) {
+#pragma unroll
+            for (int i = 0; i < BT; ++i) nxt[i] = p[base + BT + i];
+        }
+#pragma unroll
+        for (int i = 0; i < BT; ++i) add_all<NCH, CERT>(cur[i], s, mm, em);
+#pragma unroll
+        for (int i = 0; i < BT; ++i) cur[i] = nxt[i];
+    }
+    for (; base < maxlen; base += BT) {  // the tails: +0 past a lane's end
+        if (base + BT < len) {
+#pragma unroll
+            for (int i = 0; i < BT; ++i) nxt[i] = p[base + BT + i];
+        }
+#pragma unroll
+        for (int i = 0; i < BT; ++i) add_all<NCH, CERT>(base + i < len ? cur[i] : 0.0, s, mm, em);
+#pragma unroll
+        for (int i = 0; i < BT; ++i) cur[i] = nxt[i];
+    }
+}
+
+// The exactness certificate of a tabulated run from the start v0 (exponent
+// field of key, residue bits P) whose sums kept the distance mm from their
+// binades' edges and reached exponent field em at most: the largest |delta|
+// for which the run from v0 + delta ulp(v0) (delta a multiple of 2^P) ends
+// exactly delta ulp(v0) from the tabulated end, or -1 (only delta = 0).
+// Proof: let w be an exact sum of the run from v0 and v_i = RN(w), in the
+// binade of field e with spacing g, at distance m_i >= g from its edges; w
+// then lies in the same binade, at least m_i - g/2 inside.  If e <= e0 + P - 1,
+// D = delta ulp(v0) is an even multiple of g, so when |D| <= m_i - g, w + D
+// rounds to v_i + D (rounding within a binade commutes with shifts by even
+// multiples of the spacing, ties to even included); by induction the shifted
+// run stays D from the tabulated one.  With g <= 2^(P-1) ulp(v0) for every i,
+// |delta| <= floor(mm / ulp(v0)) - 2^(P-1) suffices.
+__device__ __forceinline__ double certificate(double mm, unsigned em, unsigned key, int P) {
+    const int e0 = (int)(key & 0x7ff);
+    if ((int)(em >> 20) > e0 + P - 1) return -1.0;
+    const double t = ldexp(mm, 1075 - e0);  // mm / ulp(v0), exact unless below 1
+    const double allow = (t >= 1.0 ? floor(t) : 0.0) - (double)(1 << (P - 1));
+    if (allow < 0.0) return -1.0;
+    return allow < ALLOW_MAX ? allow : ALLOW_MAX;
+}
+
+// Tabulation: lane i of sequence blockIdx.y takes entry i of the lane list,
+// start value base + r of chunk c, and runs it through the chunk's terms,
+// keeping its certificate (the grid strides over the list).
+__global__ void __launch_bounds__(256) pk_tab_kernel(const double* __restrict__ terms, long npad,
+                                                     const Peak* __restrict__ pk,
+                                                     const int* __restrict__ lanes,
+                                                     const int* __restrict__ counts,
+                                                     double* __restrict__ tab,
+                                                     double* __restrict__ cert) {
+    const long nc = npad / SEQ_CBLK;
+    const int q = blockIdx.y;
+    const int n = counts[q];
+    const int* lq = lanes + (long)q * TAB * nc;
+    for (long i0 = (long)blockIdx.x * 256; i0 < n; i0 += (long)gridDim.x * 256) {
+        const long i = i0 + threadIdx.x;
+        const bool active = i < n;
+        if (__ballot(active) == 0ull) continue;  // uniform across the wavefront
+        const int ent = active ? lq[i] : 0;
+        const long c = ent / TAB;
+        const int r = ent % TAB;
+        const Chunk k = chunk_of(pk + (long)q * nc, nc, npad, c);
+        const StitchInfo info = stitch_info(k);
+        const long long I = info.base + r;
+        double s[1] = {dfrom(((unsigned long long)info.key << 52) |
+                             ((unsigned long long)(I < 0 ? -I : I) & MANT))};
+        double mm[1] = {__builtin_huge_val()};
+        unsigned em[1] = {0u};
+        const long len = active ? k.last - k.first + 1 : 0;
+        run_terms<1, true>(terms + (long)q * npad + (active ? k.first : 0), len, s, mm, em);
+        if (active) {
+            tab[((long)q * nc + c) * TAB + r] = s[0];
+            cert[((long)q * nc + c) * TAB + r] = certificate(mm[0], em[0], info.key, info.P);
+        }
+    }
+}
+
+// s + x[first] + ... + x[first + len - 1] in order, first and len uniform
+// across the wavefront: every lane runs the same chain on terms read
+// through the scalar unit, the next 16 in flight while the current 16 are
+// added.  Returns the same value in every lane.
+__device__ __forceinline__ double run_wave(const double* __restrict__ x, long first, long len,
+                                           double s) {
+    const double* p = x + first;
+    long i = 0;
+    if (len >= 16) {
+        double t[16], tn[16];
+#pragma unroll
+        for (int k = 0; k < 16; ++k) t[k] = p[k];
+        for (; i + 32 <= len; i += 16) {
+#pragma unroll
+            for (int k = 0; k < 16; ++k) tn[k] = p[i + 16 + k];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) s = s + t[k];
+#pragma unroll
+            for (int k = 0; k < 16; ++k) t[k] = tn[k];
+        }
+#pragma unroll
+        for (int k = 0; k < 16; ++k) s = s + t[k];
+        i += 16;
+    }
+    for (; i < len; ++i) s = s + p[i];
+    return s;
+}
+
+__device__ __forceinline__ long long imant_u(unsigned long long u) {
+    const long long m = (long long)((u & MANT) | (1ull << 52));
+    return (u >> 63) ? -m : m;
+}
+__device__ __forceinline__ bool nan_u(unsigned long long u) {
+    return (u & (0x7ffull << 52)) == (0x7ffull << 52) && (u & MANT) != 0;
+}
+
+// The end of a chunk from a start of integer I and exponent field es in the
+// chunk's predicted binade, given e = the end tabulated from the start value
+// base + r (r = I mod 2^P) and that run's certificate allow: the integer of e
+// plus delta = I - base - r scaled to e's binade (k = es - e's exponent
+// field), as the bits of a value in e's binade.  False when delta is neither
+// 0 nor within the certificate, e is not a normal value, the scaling would
+// not be exact or could overflow, or the end leaves e's binade.
+__device__ __forceinline__ bool chunk_end(long long I, int es, int P, long long base,
+                                          unsigned long long e, long long allow,
+                                          unsigned long long& out, int& k) {
+    const long long r = I & ((1ll << P) - 1);
+    const long long delta = I - base - r;  // a multiple of 2^P
+    if (delta != 0 && (delta > allow || -delta > allow)) return false;
+    const int ee = (int)((e >> 52) & 0x7ff);
     if (ee == 0 || ee == 0x7ff) return false;
-    const long long me = (long long)((e & ((1ull << 52) - 1)) | (1ull << 52));
-    const long long IE = (e >> 63) ? -me : me;
-    const int k = es - ee;
+    const long long IE = imant_u(e);
+    k = es - ee;
     long long D;
     if (k >= 0) {
         if (k > 10 || delta >= (1ll << 50) || delta <= -(1ll << 50)) return false;
         D = delta * (1ll << k);
     } else {
-        D = delta >> (-k);  // exact: delta is a multiple of 2^P and -k < P
+        if (-k >= P) return false;
+        D = delta >> (-k);
     }
     const long long Ie = IE + D;
     if ((Ie < 0) != (IE < 0)) return false;
     const long long a = Ie < 0 ? -Ie : Ie;
     if (a < (1ll << 52) || a >= (1ll << 53)) return false;
-    u = rfl_u((e & (0xfffull << 52)) | ((unsigned long long)a & ((1ull << 52) - 1)));
+    out = (e & (0xfffull << 52)) | ((unsigned long long)a & MANT);
     return true;
 }
 
-// One wavefront per sequence carries the exact running sum from chunk to
-// chunk.  Lane r holds tab[c][r] of the STITCH_G chunks in hand, so E[r] is a
-// read of lane r; the next round's tables load meanwhile.  A chunk of zeros
-// leaves the sum as it is: s + 0 and s - 0 are s for every s but -0 and NaN,
-// and the sum, which starts at +0, is never -0 (the last bits are checked by
-// the verification anyway).  The launch asks
-// for STITCH_LDS_BYTES of LDS so that each stitch has a CU of its own (they
-// are serial chains of scalar work that would otherwise share a CU's scalar
-// unit).
-__global__ void __launch_bounds__(64) pk_stitch_kernel(const double* __restrict__ terms, long npad,
-                                                       const Peak* __restrict__ pk,
-                                                       const double* __restrict__ tab,
-                                                       double* __restrict__ starts,
-                                                       unsigned long long* __restrict__ flags,
-                                                       double* __restrict__ sums,
-                                                       long long* __restrict__ stats) {
-    const long nc = npad / SEQ_CBLK;
-    const int q = blockIdx.x;
-    const int lane = threadIdx.x;
-    const double* x = terms + (long)q * npad;
-    const Peak* pkq = pk + (long)q * nc;
-    const double* tq = tab + (long)q * nc * TAB;
-    double* sq = starts + (long)q * (nc + 1);
-    if (lane == 0) flags[q] = 0ull;
-    const bool want = stats != nullptr;
-    const long long t0 = want ? (long long)clock64() : 0;
-    long long n_small = 0, n_big = 0, n_fall = 0, cyc_fall = 0, n_zero = 0;
-    // the head: the terms up to the first block's peak, from 0
-    // (run_wave returns the same value in every lane; readfirstlane tells the
-    // compiler so, which keeps the stitch's integer work on the scalar unit)
-    unsigned long long u = rfl_u(ubits(run_wave(x, 0, pkq[0].p + 1, 0.0)));
-    double E[STITCH_G], En[STITCH_G];
+// One chunk of a chunk-by-chunk stitch on the exact start u (uniform across
+// the wavefront): Eg and Cg hold the chunk's tabulated ends and
+// certificates, those of residue r in lane r.
+__device__ __forceinline__ bool stitch_step(unsigned long long& u, unsigned key, int P,
+                                            long long base, double Eg, double Cg) {
+    if ((unsigned)(u >> 52) != key) return false;
+    const long long I = imant_u(u);
+    const int r = (int)(I & ((1ll << P) - 1));
+    unsigned long long o;
+    int k;
+    if (!chunk_end(I, (int)(key & 0x7ff), P, base, ubits(rl_d(Eg, r)), (long long)rl_d(Cg, r), o,
+                   k))
+        return false;
+    u = rfl_u(o);
+    return true;
+}
+
+struct StitchCount {
+    long long small, big, fall, zero;
+};
+
+// Chunks [cb, ce) of a sequence from the exact start u, one after another
+// (the whole wavefront, the sum uniform across it): tabulated where the
+// checks pass, term by term otherwise, a chunk of zeros passed through.  A
+// chunk of zeros leaves the sum as it is: s + 0 and s - 0 are s for every s
+// but -0 and NaN, and the sum, which starts at +0, is never -0.  Writes each
+// chunk's start to sq[c] when sq is given; returns the end.  Lane r holds
+// row r of the tables of the STITCH_G chunks in hand, so E[r] is a read of
+// lane r; the next round's rows load meanwhile.
+__device__ unsigned long long stitch_range(const double* __restrict__ x,
+                                           const Peak* __restrict__ pkq,
+                                           const double* __restrict__ tq,
+                                           const double* __restrict__ cq, long nc, long npad,
+                                           long cb, long ce, unsigned long long u,
+                                           double* __restrict__ sq, StitchCount& ct) {
+    const int lane = threadIdx.x & 63;
+    double E[STITCH_G], En[STITCH_G], C[STITCH_G], Cn[STITCH_G];
 #pragma unroll
-    for (int g = 0; g < STITCH_G; ++g) E[g] = g < nc ? tq[(long)g * TAB + lane] : 0.0;
-    Chunk mine = chunk_of(pkq, nc, npad, lane < nc ? lane : 0);
-    for (long c0 = 0; c0 < nc; c0 += STITCH_G) {
-        const int g_n = nc - c0 < STITCH_G ? (int)(nc - c0) : STITCH_G;
+    for (int g = 0; g < STITCH_G; ++g) {
+        E[g] = cb + g < ce ? tq[(cb + g) * TAB + lane] : 0.0;
+        C[g] = cb + g < ce ? cq[(cb + g) * TAB + lane] : 0.0;
+    }
+    Chunk mine = chunk_of(pkq, nc, npad, cb + lane < ce ? cb + lane : cb);
+    for (long c0 = cb; c0 < ce; c0 += STITCH_G) {
+        const int g_n = ce - c0 < STITCH_G ? (int)(ce - c0) : STITCH_G;
         const long c1 = c0 + STITCH_G;
-        // the next round's tables and chunk bounds, in flight during this round
 #pragma unroll
-        for (int g = 0; g < STITCH_G; ++g) En[g] = c1 + g < nc ? tq[(c1 + g) * TAB + lane] : 0.0;
-        const Chunk next = chunk_of(pkq, nc, npad, c1 + lane < nc ? c1 + lane : 0);
+        for (int g = 0; g < STITCH_G; ++g) {
+            En[g] = c1 + g < ce ? tq[(c1 + g) * TAB + lane] : 0.0;
+            Cn[g] = c1 + g < ce ? cq[(c1 + g) * TAB + lane] : 0.0;
+        }
+        const Chunk next = chunk_of(pkq, nc, npad, c1 + lane < ce ? c1 + lane : cb);
         const StitchInfo info = stitch_info(mine);
         double mystart = 0.0;
 #pragma unroll
@@ -583,45 +754,328 @@ __global__ void __launch_bounds__(64) pk_stitch_kernel(const double* __restrict_
                 const unsigned key = (unsigned)__builtin_amdgcn_readlane((int)info.key, g);
                 const long long base = rl_l(info.base, g);
                 const bool zero = __builtin_amdgcn_readlane(info.zero, g) != 0;
-                const bool nan = (u & (0x7ffull << 52)) == (0x7ffull << 52) &&
-                                 (u & ((1ull << 52) - 1)) != 0;
-                if (P <= 3) ++n_small;
-                else if (P <= SEQ_PMAX) ++n_big;
-                if (zero && !nan) {
-                    ++n_zero;
-                } else if (!(ok && stitch_step(u, key, P, base, E[g]))) {
+                if (P <= 3) ++ct.small;
+                else if (P <= SEQ_PMAX) ++ct.big;
+                if (zero && !nan_u(u)) {
+                    ++ct.zero;
+                } else if (!(ok && stitch_step(u, key, P, base, E[g], C[g]))) {
                     const long first = rl_l(mine.first, g);
                     const long last = rl_l(mine.last, g);
-                    const long long tf = want ? (long long)clock64() : 0;
                     u = rfl_u(ubits(run_wave(x, first, last - first + 1,
                                              __longlong_as_double((long long)u))));
-                    if (want) cyc_fall += (long long)clock64() - tf;
-                    ++n_fall;
+                    ++ct.fall;
                 }
             }
         }
-        if (lane < g_n) sq[c0 + lane] = mystart;
+        if (sq && lane < g_n) sq[c0 + lane] = mystart;
 #pragma unroll
-        for (int g = 0; g < STITCH_G; ++g) E[g] = En[g];
+        for (int g = 0; g < STITCH_G; ++g) {
+            E[g] = En[g];
+            C[g] = Cn[g];
+        }
         mine = next;
     }
+    return u;
+}
+
+// The map of a group of GROUP chunks, for start residue r mod 2^PG: from
+// the start gbase + r (gbase = the integer of the first chunk's predicted
+// start rounded down to a multiple of 2^PG, in its binade) the group ends at
+// end, and from gbase + r + delta (delta a multiple of 2^PG) at
+// end + delta 2^K exactly when lo <= delta <= hi.  valid = 0: no map for
+// this residue.
+struct GroupMap {
+    unsigned long long end;
+    long long lo, hi;
+    int K, valid;
+};
+// Per group: gbase and its binade's key (ok = 0: no maps), and whether all
+// its chunks are zeros.
+struct GroupHead {
+    long long base;
+    unsigned key;
+    short ok, zero;
+};
+
+// Level 1 of the stitch, one wavefront per group: lane r carries the start
+// gbase + r through the group's chunks with the chunk-by-chunk arithmetic.
+// A start that differs by delta (a multiple of 2^PG) follows the same
+// residues while PG + K >= P at every chunk (delta 2^K is then a multiple of
+// 2^P; K falls by one for each binade the sum rises), and its offset from
+// each chunk's tabulated run, the lane's offset d plus delta 2^K, must stay
+// within that run's certificate A: (-A - d) / 2^K <= delta <= (A - d) / 2^K
+// for every chunk.  It then ends delta 2^K from the lane's end.  Chunks of
+// zeros pass the value through.
+__global__ void __launch_bounds__(64) pk_group_kernel(const double* __restrict__ tab,
+                                                      const double* __restrict__ cert, long npad,
+                                                      const Peak* __restrict__ pk,
+                                                      GroupMap* __restrict__ gm,
+                                                      GroupHead* __restrict__ gh) {
+    const long nc = npad / SEQ_CBLK;
+    const long ng = (nc + GROUP - 1) / GROUP;
+    const long g = blockIdx.x;
+    const int q = blockIdx.y;
+    const int lane = threadIdx.x;
+    const long cb = g * GROUP;
+    const int cn = (int)(nc - cb < GROUP ? nc - cb : GROUP);
+    const Peak* pkq = pk + (long)q * nc;
+    const double* tq = tab + (long)q * nc * TAB;
+    const double* cq = cert + (long)q * nc * TAB;
+    const Chunk mine = chunk_of(pkq, nc, npad, cb + (lane < cn ? lane : 0));
+    const StitchInfo info = stitch_info(mine);
+    const bool allzero = __all(lane >= cn || mine.zero) != 0;
+    // the group's base: chunk cb's predicted start (lane 0's chunk)
+    const double V0 = rfl_d(mine.V);
+    GroupHead head;
+    head.key = key_of(V0);
+    head.zero = allzero ? 1 : 0;
+    head.ok = 0;
+    head.base = 0;
+    if (normal_finite(V0)) {
+        const long long iv = imant(V0);
+        const long long b = iv - (iv & ((1ll << PG) - 1));
+        const long long e = b + ((1ll << PG) - 1);
+        const long long ab = b < 0 ? -b : b, ae = e < 0 ? -e : e;
+        head.ok = ab >= (1ll << 52) && ab < (1ll << 53) && ae >= (1ll << 52) && ae < (1ll << 53);
+        head.base = b;
+    }
+    GroupMap* out = gm + ((long)q * ng + g) * NRES;
+    if (lane == 0) gh[(long)q * ng + g] = head;
+    GroupMap res;
+    res.end = 0;
+    res.lo = 0;
+    res.hi = -1;
+    res.K = 0;
+    res.valid = 0;
+    if (allzero || !head.ok) {  // uniform across the wavefront
+        out[lane] = res;
+        return;
+    }
+    bool valid = true;
+    unsigned long long u;
+    {
+        const long long I = head.base + lane;
+        const unsigned long long a = (unsigned long long)(I < 0 ? -I : I);
+        u = ((unsigned long long)head.key << 52) | (a & MANT);
+    }
+    long long lo = -(1ll << 62), hi = 1ll << 62;
+    int K = 0;
+    double E[STITCH_G], En[STITCH_G], C[STITCH_G], Cn[STITCH_G];
+#pragma unroll
+    for (int t = 0; t < STITCH_G; ++t) {
+        E[t] = t < cn ? tq[(cb + t) * TAB + lane] : 0.0;
+        C[t] = t < cn ? cq[(cb + t) * TAB + lane] : 0.0;
+    }
+    for (int t0 = 0; t0 < cn; t0 += STITCH_G) {
+        const int t1 = t0 + STITCH_G;
+#pragma unroll
+        for (int t = 0; t < STITCH_G; ++t) {
+            En[t] = t1 + t < cn ? tq[(cb + t1 + t) * TAB + lane] : 0.0;
+            Cn[t] = t1 + t < cn ? cq[(cb + t1 + t) * TAB + lane] : 0.0;
+        }
+#pragma unroll
+        for (int t = 0; t < STITCH_G; ++t) {
+            const int idx = t0 + t;
+            if (idx < cn && !__builtin_amdgcn_readlane(info.zero, idx)) {
+                const bool okc = __builtin_amdgcn_readlane(info.ok, idx) != 0;
+                const int P = okc ? __builtin_amdgcn_readlane(info.P, idx) : 0;
+                const unsigned key = (unsigned)__builtin_amdgcn_readlane((int)info.key, idx);
+                const long long base = rl_l(info.base, idx);
+                const long long I = imant_u(u);
+                const int r = valid ? (int)(I & ((1ll << P) - 1)) : 0;
+                const double Ev = __shfl(E[t], r, 64);
+                const long long allow = (long long)__shfl(C[t], r, 64);
+                if (valid) {
+                    unsigned long long o = 0;
+                    int k = 0;
+                    valid = okc && (unsigned)(u >> 52) == key && PG + K >= P &&
+                            chunk_end(I, (int)(key & 0x7ff), P, base, ubits(Ev), allow, o, k);
+                    if (valid) {
+                        const long long A = allow > 0 ? allow : 0;
+                        const long long d = I - base - (I & ((1ll << P) - 1));
+                        const long long a = -A - d, b = A - d;
+                        long long l, h;
+                        if (K >= 0) {
+                            const int sh = K < 62 ? K : 62;
+                            l = -((-a) >> sh);  // ceil(a / 2^K)
+                            h = b >> sh;        // floor(b / 2^K)
+                        } else {
+                            l = a * (1ll << (-K));
+                            h = b * (1ll << (-K));
+                        }
+                        lo = l > lo ? l : lo;
+                        hi = h < hi ? h : hi;
+                        u = o;
+                        K += k;
+                    }
+                }
+            }
+        }
+#pragma unroll
+        for (int t = 0; t < STITCH_G; ++t) {
+            E[t] = En[t];
+            C[t] = Cn[t];
+        }
+    }
+    if (valid && lo <= hi) {
+        res.end = u;
+        res.lo = lo;
+        res.hi = hi;
+        res.K = K;
+        res.valid = 1;
+    }
+    out[lane] = res;
+}
+
+// Level 2, one wavefront per sequence: the exact running sum from group to
+// group, through the group's map where the start lies in its first chunk's
+// binade and within the map's range, chunk by chunk otherwise.  Writes the
+// group starts and the sum.  The maps pass through LDS in batches of
+// MAP_BATCH groups (the next batch loads while one is walked); the launch
+// asks for STITCH_LDS_BYTES, which also keeps every sequence on a CU of its
+// own (serial chains of scalar work that would otherwise share a CU's
+// scalar unit).
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(1, 1)))
+pk_stitch_kernel(const double* __restrict__ terms, long npad,
+                                                       const Peak* __restrict__ pk,
+                                                       const double* __restrict__ tab,
+                                                       const double* __restrict__ cert,
+                                                       const GroupMap* __restrict__ gm,
+                                                       const GroupHead* __restrict__ gh,
+                                                       double* __restrict__ gstart,
+                                                       double* __restrict__ starts,
+                                                       unsigned long long* __restrict__ flags,
+                                                       double* __restrict__ sums,
+                                                       long long* __restrict__ stats) {
+    extern __shared__ GroupMap ring[];  // [2][MAP_BATCH][NRES]
+    const long nc = npad / SEQ_CBLK;
+    const long ng = (nc + GROUP - 1) / GROUP;
+    const int q = blockIdx.x;
+    const int lane = threadIdx.x;
+    const double* x = terms + (long)q * npad;
+    const Peak* pkq = pk + (long)q * nc;
+    const double* tq = tab + (long)q * nc * TAB;
+    const double* cq = cert + (long)q * nc * TAB;
+    const GroupMap* gmq = gm + (long)q * ng * NRES;
+    const GroupHead* ghq = gh + (long)q * ng;
+    double* gs = gstart + (long)q * (ng + 1);
+    if (lane == 0) flags[q] = 0ull;
+    const bool want = stats != nullptr;
+    const long long t0 = want ? (long long)clock64() : 0;
+    StitchCount ct{0, 0, 0, 0};
+    long long n_walk = 0;
+    // the head: the terms up to the first block's peak, from 0 (the same
+    // value in every lane; readfirstlane tells the compiler so, which keeps
+    // the walk's integer work on the scalar unit)
+    unsigned long long u = rfl_u(ubits(run_wave(x, 0, pkq[0].p + 1, 0.0)));
+    // the next batch's maps, staged in registers (lane r: residue r of each
+    // group; eight named registers, not an array, so that none lands in
+    // scratch memory)
+    static_assert(MAP_BATCH == 8, "the staging below names eight groups");
+    const double4* gm4 = reinterpret_cast<const double4*>(gmq);
+    double4 v0, v1, v2, v3, v4, v5, v6, v7;
+#define HYD_MAPS(X) X(0) X(1) X(2) X(3) X(4) X(5) X(6) X(7)
+#define HYD_LOAD(b) v##b = gm4[(G1 + b < ng ? G1 + b : ng - 1) * NRES + lane];
+#define HYD_STORE(b) buf4[b * NRES + lane] = v##b;
+    GroupHead hv;
+    {
+        const long G1 = 0;
+        HYD_MAPS(HYD_LOAD)
+    }
+    hv = ghq[lane < MAP_BATCH && lane < ng ? lane : 0];
+    for (long G0 = 0; G0 < ng; G0 += MAP_BATCH) {
+        GroupMap* buf = ring + ((G0 / MAP_BATCH) & 1) * MAP_BATCH * NRES;
+        double4* buf4 = reinterpret_cast<double4*>(buf);
+        HYD_MAPS(HYD_STORE)
+        const GroupHead hd = hv;  // lane b: the head of group G0 + b
+        __syncthreads();
+        const long G1 = G0 + MAP_BATCH;
+        HYD_MAPS(HYD_LOAD)
+        hv = ghq[lane < MAP_BATCH && G1 + lane < ng ? G1 + lane : 0];
+        const int bn = ng - G0 < MAP_BATCH ? (int)(ng - G0) : MAP_BATCH;
+        for (int b = 0; b < bn; ++b) {
+            const long g = G0 + b;
+            if (lane == 0) gs[g] = __longlong_as_double((long long)u);
+            if (__builtin_amdgcn_readlane((int)hd.zero, b) && !nan_u(u)) continue;
+            bool done = false;
+            const bool ok0 = __builtin_amdgcn_readlane((int)hd.ok, b) != 0;
+            const unsigned key0 = (unsigned)__builtin_amdgcn_readlane((int)hd.key, b);
+            if (ok0 && (unsigned)(u >> 52) == key0) {
+                const long long I = imant_u(u);
+                const int r = (int)(I & ((1ll << PG) - 1));
+                const long long delta = I - rl_l(hd.base, b) - r;
+                const GroupMap mp = buf[b * NRES + r];  // the same address in every lane
+                if (__builtin_amdgcn_readfirstlane(mp.valid) &&
+                    delta >= (long long)rfl_u((unsigned long long)mp.lo) &&
+                    delta <= (long long)rfl_u((unsigned long long)mp.hi)) {
+                    const unsigned long long A = rfl_u(mp.end);
+                    const int K = __builtin_amdgcn_readfirstlane(mp.K);
+                    const long long Ie =
+                        imant_u(A) + (K >= 0 ? delta * (1ll << K) : delta >> (-K));
+                    const long long aa = Ie < 0 ? -Ie : Ie;
+                    u = rfl_u((A & (0xfffull << 52)) | ((unsigned long long)aa & MANT));
+                    done = true;
+                }
+            }
+            if (!done) {
+                ++n_walk;
+                const long c0 = g * GROUP;
+                u = stitch_range(x, pkq, tq, cq, nc, npad, c0, c0 + GROUP < nc ? c0 + GROUP : nc, u,
+                                 nullptr, ct);
+            }
+        }
+        __syncthreads();
+    }
+#undef HYD_STORE
+#undef HYD_LOAD
+#undef HYD_MAPS
     const double s = __longlong_as_double((long long)u);
     if (lane == 0) {
-        sq[nc] = s;
+        gs[ng] = s;
+        starts[(long)q * (nc + 1) + nc] = s;
         sums[2 * q] = s;
         sums[2 * q + 1] = 0.0;
         if (want) {
             long long* o = stats + (long)SEQ_STATS * q;
             o[0] = nc;
-            o[1] = n_small;
-            o[2] = n_big;
-            o[3] = n_fall;
+            o[1] = 0;
+            o[2] = 0;
+            o[3] = 0;
             o[4] = 0;
             o[5] = (long long)clock64() - t0;
-            o[6] = cyc_fall;
+            o[6] = n_walk;
             o[7] = 0;
-            o[8] = n_zero;
+            o[8] = 0;
         }
+    }
+}
+
+// Level 3, one wavefront per group: the exact start of every chunk, chunk by
+// chunk from the group's exact start, and the chunk counters.
+__global__ void __launch_bounds__(64) pk_fill_kernel(const double* __restrict__ terms, long npad,
+                                                     const Peak* __restrict__ pk,
+                                                     const double* __restrict__ tab,
+                                                     const double* __restrict__ cert,
+                                                     const double* __restrict__ gstart,
+                                                     double* __restrict__ starts,
+                                                     long long* __restrict__ stats) {
+    const long nc = npad / SEQ_CBLK;
+    const long ng = (nc + GROUP - 1) / GROUP;
+    const long g = blockIdx.x;
+    const int q = blockIdx.y;
+    const long cb = g * GROUP;
+    const long ce = cb + GROUP < nc ? cb + GROUP : nc;
+    StitchCount ct{0, 0, 0, 0};
+    const unsigned long long u = rfl_u(ubits(gstart[(long)q * (ng + 1) + g]));
+    (void)stitch_range(terms + (long)q * npad, pk + (long)q * nc, tab + (long)q * nc * TAB,
+                       cert + (long)q * nc * TAB, nc, npad, cb, ce, u,
+                       starts + (long)q * (nc + 1), ct);
+    if (stats && threadIdx.x == 0) {
+        unsigned long long* o = (unsigned long long*)(stats + (long)SEQ_STATS * q);
+        atomicAdd(o + 1, (unsigned long long)ct.small);
+        atomicAdd(o + 2, (unsigned long long)ct.big);
+        atomicAdd(o + 3, (unsigned long long)ct.fall);
+        atomicAdd(o + 8, (unsigned long long)ct.zero);
     }
 }
 
@@ -642,12 +1096,11 @@ __global__ void __launch_bounds__(256) pk_verify_kernel(const double* __restrict
         k = chunk_of(pk + (long)q * nc, nc, npad, c);
         len = k.last - k.first + 1;
     }
-    const long maxlen = wave_max_long(len);
-    if (maxlen == 0) return;  // uniform across the wavefront
-    const long minlen = wave_min_long(c < nc ? len : maxlen);
-    const double s = run_lane(terms + (long)q * npad, c < nc ? k.first : 0,
-                              c < nc ? len : maxlen, minlen, maxlen, c < nc ? sq[c] : 0.0);
-    if (c < nc && ubits(s) != ubits(sq[c + 1])) {
+    if (wave_max_long(len) == 0) return;  // uniform across the wavefront
+    double s[1] = {c < nc ? sq[c] : 0.0}, mm[1] = {0.0};
+    unsigned em[1] = {0u};
+    run_terms<1, false>(terms + (long)q * npad + (c < nc ? k.first : 0), len, s, mm, em);
+    if (c < nc && ubits(s[0]) != ubits(sq[c + 1])) {
         flags[q] = 1ull;
         if (stats) atomicAdd((unsigned long long*)&stats[(long)SEQ_STATS * q + 4], 1ull);
     }
@@ -669,16 +1122,50 @@ __global__ void __launch_bounds__(256) pk_redo_kernel(const double* __restrict__
     }
 }
 
+// Workspace of the peak-chunk methods, offsets in doubles.
+struct PeaksLayout {
+    long bs, pk, lanes, counts, tab, cert, starts, gstart, gm, gh, flags, total;
+};
+PeaksLayout peaks_layout(long npad, int nq) {
+    const long nb = npad / SEQ_BLOCK1, nc = npad / SEQ_CBLK, ng = (nc + GROUP - 1) / GROUP;
+    static_assert(sizeof(BlockStat) == 6 * sizeof(double), "BlockStat layout");
+    static_assert(sizeof(Peak) == 2 * sizeof(double), "Peak layout");
+    static_assert(sizeof(GroupMap) == 4 * sizeof(double), "GroupMap layout");
+    static_assert(2 * MAP_BATCH * NRES * sizeof(GroupMap) <= STITCH_LDS_BYTES, "map ring");
+    static_assert(sizeof(GroupHead) == 2 * sizeof(double), "GroupHead layout");
+    PeaksLayout L;
+    long o = 0;
+    L.bs = o;
+    o += 6 * nb * nq;
+    L.pk = o;
+    o += 2 * nc * nq;
+    L.lanes = o;
+    o += (TAB * nc * nq + 1) / 2;
+    L.counts = o;
+    o += (nq + 1) / 2;
+    L.tab = o;
+    o += (long)TAB * nc * nq;
+    L.cert = o;
+    o += (long)TAB * nc * nq;
+    L.starts = o;
+    o += (nc + 1) * nq;
+    L.gstart = o;
+    o += (ng + 1) * nq;
+    L.gm = o;
+    o += 4L * NRES * ng * nq;
+    L.gh = o;
+    o += 2 * ng * nq;
+    L.flags = o;
+    o += nq;
+    L.total = o;
+    return L;
+}
+
 }  // namespace
 
 long seqsum_npad(long n) { return (n + SEQ_BLOCK2 - 1) / SEQ_BLOCK2 * SEQ_BLOCK2; }
 
-// bsum and bpre (2 nb each), peaks (2 nc), tables (64 nc), chunk starts
-// (nc + 1) and a flag per sequence
-long seqsum_workspace_len(long npad, int nq) {
-    const long nb = npad / SEQ_BLOCK1, nc = npad / SEQ_CBLK;
-    return (long)nq * (4 * nb + 2 * nc + (long)TAB * nc + nc + 2);
-}
+long seqsum_workspace_len(long npad, int nq) { return peaks_layout(npad, nq).total; }
 
 hipError_t launch_seqsum(int method, const double* terms, long npad, int nq, double* ws,
                          double* sums, hipStream_t st, long long* stats) {
@@ -688,34 +1175,45 @@ hipError_t launch_seqsum(int method, const double* terms, long npad, int nq, dou
                            sums);
         return hipGetLastError();
     }
-    if (method != SEQSUM_PEAKS) return hipErrorInvalidValue;
-    // the chunk runs read up to 16 doubles past a sequence: the workspace
+    if (method != SEQSUM_PEAKS && method != SEQSUM_PEAKS_CHECKED) return hipErrorInvalidValue;
+    // the chunk runs read up to 32 doubles past a sequence: the workspace
     // must follow the terms
     if (ws < terms + (long)nq * npad) return hipErrorInvalidValue;
-    const long nb = npad / SEQ_BLOCK1, nc = npad / SEQ_CBLK;
-    double* bsum = ws;
-    double* bpre = bsum + 2 * (long)nq * nb;
-    Peak* pk = reinterpret_cast<Peak*>(bpre + 2 * (long)nq * nb);
-    double* tab = reinterpret_cast<double*>(pk + (long)nq * nc);
-    double* starts = tab + (long)TAB * nq * nc;
-    unsigned long long* flags =
-        reinterpret_cast<unsigned long long*>(starts + (long)nq * (nc + 1));
-    const dim3 gb((unsigned)(nb / 4), (unsigned)nq);  // 4 wavefronts per 4096-block
-    hipLaunchKernelGGL(pk_blocksum_kernel, gb, dim3(256), 0, st, terms, npad, bsum);
-    hipLaunchKernelGGL(pk_scan_kernel, dim3((unsigned)nq), dim3(1024), 0, st, bsum, nb, bpre);
-    hipLaunchKernelGGL(pk_peak_kernel, gb, dim3(256), 0, st, terms, npad, bpre, pk);
-    hipLaunchKernelGGL((pk_tab_kernel<2, 1, 1>), dim3((unsigned)((nc + 127) / 128), (unsigned)nq),
-                       dim3(256), 0, st, terms, npad, pk, tab);
-    hipLaunchKernelGGL((pk_tab_kernel<8, 2, 3>), dim3((unsigned)((nc + 31) / 32), (unsigned)nq),
-                       dim3(256), 0, st, terms, npad, pk, tab);
-    hipLaunchKernelGGL((pk_tab_kernel<64, 4, SEQ_PMAX>), dim3((unsigned)((nc + 3) / 4), (unsigned)nq),
-                       dim3(256), 0, st, terms, npad, pk, tab);
-    hipLaunchKernelGGL(pk_stitch_kernel, dim3((unsigned)nq), dim3(64), STITCH_LDS_BYTES, st, terms,
-                       npad, pk, tab, starts, flags, sums, stats);
-    hipLaunchKernelGGL(pk_verify_kernel, dim3((unsigned)((nc + 255) / 256), (unsigned)nq),
-                       dim3(256), 0, st, terms, npad, pk, starts, flags, stats);
-    hipLaunchKernelGGL(pk_redo_kernel, dim3((unsigned)nq), dim3(256), 0, st, terms, npad, flags,
-                       sums, stats);
+    const long nb = npad / SEQ_BLOCK1, nc = npad / SEQ_CBLK, ng = (nc + GROUP - 1) / GROUP;
+    const PeaksLayout L = peaks_layout(npad, nq);
+    BlockStat* bs = reinterpret_cast<BlockStat*>(ws + L.bs);
+    Peak* pk = reinterpret_cast<Peak*>(ws + L.pk);
+    int* lanes = reinterpret_cast<int*>(ws + L.lanes);
+    int* counts = reinterpret_cast<int*>(ws + L.counts);
+    double* tab = ws + L.tab;
+    double* cert = ws + L.cert;
+    double* starts = ws + L.starts;
+    double* gstart = ws + L.gstart;
+    GroupMap* gm = reinterpret_cast<GroupMap*>(ws + L.gm);
+    GroupHead* gh = reinterpret_cast<GroupHead*>(ws + L.gh);
+    unsigned long long* flags = reinterpret_cast<unsigned long long*>(ws + L.flags);
+    const unsigned uq = (unsigned)nq;
+    hipLaunchKernelGGL(pk_block_kernel, dim3((unsigned)(nb / 4), uq), dim3(256), 0, st, terms,
+                       npad, bs);
+    hipLaunchKernelGGL(pk_scan_kernel, dim3(uq), dim3(1024), 0, st, bs, nb, pk);
+    hipLaunchKernelGGL(pk_class_kernel, dim3(uq), dim3(1024), 0, st, pk, npad, lanes, counts);
+    // about 2 lanes per chunk (most have P = 1); the grid strides over more
+    hipLaunchKernelGGL(pk_tab_kernel, dim3((unsigned)((4 * nc + 255) / 256), uq), dim3(256), 0,
+                       st, terms, npad, pk, lanes, counts, tab, cert);
+    hipLaunchKernelGGL(pk_group_kernel, dim3((unsigned)ng, uq), dim3(64), 0, st, tab, cert, npad,
+                       pk, gm, gh);
+    hipLaunchKernelGGL(pk_stitch_kernel, dim3(uq), dim3(64), STITCH_LDS_BYTES, st, terms, npad,
+                       pk, tab, cert, gm, gh, gstart, starts, flags, sums, stats);
+    if (method == SEQSUM_PEAKS_CHECKED || stats) {
+        hipLaunchKernelGGL(pk_fill_kernel, dim3((unsigned)ng, uq), dim3(64), 0, st, terms, npad,
+                           pk, tab, cert, gstart, starts, stats);
+    }
+    if (method == SEQSUM_PEAKS_CHECKED) {
+        hipLaunchKernelGGL(pk_verify_kernel, dim3((unsigned)((nc + 255) / 256), uq), dim3(256), 0,
+                           st, terms, npad, pk, starts, flags, stats);
+        hipLaunchKernelGGL(pk_redo_kernel, dim3(uq), dim3(256), 0, st, terms, npad, flags, sums,
+                           stats);
+    }
     return hipGetLastError();
 }
 

@@ -3,7 +3,7 @@
 // nq sequences at once.  The reference adds its multipole moment terms that
 // way (gravity_solver_3d.c:182-198, 226-258).
 //
-// Two methods with the same result bits:
+// Three methods with the same result bits:
 //   SEQSUM_CHAIN  one wavefront per sequence adds the terms in order.
 //   SEQSUM_PEAKS  chunked with tabulated increments.  Write a double s of
 //                 binade e (sign and exponent field) as s = I u, I a signed
@@ -19,22 +19,30 @@
 //                 in each 4096-term block (from double-double block sums)
 //                 and run to the next block's peak; each chunk is run from
 //                 2^P start values near its prediction, one per residue
-//                 (P <= 6), and a serial stitch carries the exact sum across
-//                 the chunks: end = E[I mod 2^P] + (start - variant start),
-//                 both exact when the binades are as predicted (checked; a
-//                 chunk that fails is added term by term; a chunk of zeros
-//                 leaves the sum as it is).  Every chunk is
-//                 then run again from its exact start, all in parallel, and
-//                 must land on the next chunk's start; if any does not, the
-//                 sequence is redone by the chain.  Predictions only decide
-//                 speed; the result is always the chain's.
+//                 (P <= 6), keeping how close each partial sum came to the
+//                 edges of its binade.  That distance certifies the run: any
+//                 start of the same residue within it follows the same
+//                 binades and ends shifted by exactly its offset.  The exact
+//                 sum is carried across a chunk in integers,
+//                 end = E[I mod 2^P] + (start - variant start), where the
+//                 offset is within the certificate (a chunk where it is not
+//                 is added term by term; a chunk of zeros leaves the sum as
+//                 it is).  The carrying is done 16 chunks at a time: each
+//                 group of chunks is carried from 64 start values (residues
+//                 mod 2^6) in parallel, which maps any start of the same
+//                 residue within the certificates to its end, and one
+//                 wavefront per sequence walks from group to group.
+//   SEQSUM_PEAKS_CHECKED  SEQSUM_PEAKS, then every chunk is run again from
+//                 its exact start, all in parallel, and must land on the next
+//                 chunk's start; if any does not, the sequence is redone by
+//                 the chain (a check of the certificates; tests use it).
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 namespace hyd {
 
-enum SeqSumMethod { SEQSUM_CHAIN = 0, SEQSUM_PEAKS = 1 };
+enum SeqSumMethod { SEQSUM_CHAIN = 0, SEQSUM_PEAKS = 1, SEQSUM_PEAKS_CHECKED = 2 };
 
 constexpr int SEQ_CHUNK = 16;                    // terms per lane in a block pass
 constexpr int SEQ_BLOCK1 = 64 * SEQ_CHUNK;        // 1024 terms (one wavefront)
@@ -49,11 +57,11 @@ long seqsum_npad(long n);
 // Workspace (doubles) of launch_seqsum for nq sequences of npad terms.
 long seqsum_workspace_len(long npad, int nq);
 // sums[2q] = the sequential sum of terms[q * npad .. + npad), sums[2q+1] = 0.
-// stats (device, SEQ_STATS per sequence, or nullptr) for SEQSUM_PEAKS:
-// chunks; chunks with P = 1, 2 or 3; with P = 4..6; chunks added term by
-// term in the stitch; chunks whose verification failed; clock cycles of the
-// stitch; of its term-by-term additions; 1 when the chain redid the sum;
-// chunks of zeros passed through.
+// stats (device, SEQ_STATS per sequence, or nullptr) for the peak-chunk
+// methods: chunks; chunks with P = 1, 2 or 3; with P = 4..6; chunks added
+// term by term; chunks whose check failed (SEQSUM_PEAKS_CHECKED); clock
+// cycles of the walk from group to group; groups it walked chunk by chunk;
+// 1 when the chain redid the sum; chunks of zeros passed through.
 hipError_t launch_seqsum(int method, const double* terms, long npad, int nq, double* ws,
                          double* sums, hipStream_t st, long long* stats = nullptr);
 

@@ -94,15 +94,16 @@ int hydro_fill_initial_state(int nx, int ny, int nz, double xmin, double xmax, d
 /* Diagnostic: the sum ((0 + x[0]) + x[1]) + ... of n doubles (host array)
  * with every addition rounded in turn, as the reference's moment loops add
  * (gravity_solver_3d.c:182-198), computed on the device by method 0 (one
- * chain of additions) or 1 (chunks that start at the running sum's predicted
- * peaks, each run from one start value per residue of the start's low bits,
- * a serial stitch of the exact sum through those tables and a parallel
- * re-run of every chunk from its exact start; any mismatch redoes the sum by
- * the chain).  stats (9 values, or NULL) receives method 1's counters:
- * chunks, chunks with 1-3 and with 4-6 residue bits, chunks the stitch added
- * term by term, chunks whose re-run missed, clock cycles of the stitch and of
- * its term-by-term additions, 1 when the chain redid the sum.
- * *status = 0 on success. */
+ * chain of additions), 1 (chunks that start at the running sum's predicted
+ * peaks, each run from one start value per residue of the start's low bits
+ * with a certificate of how far a start may move, and the exact sum carried
+ * through those tables group by group) or 2 (method 1, then every chunk run
+ * again from its exact start; any mismatch redoes the sum by the chain).
+ * stats (9 values, or NULL) receives methods 1 and 2's counters: chunks,
+ * chunks with 1-3 and with 4-6 residue bits, chunks added term by term,
+ * chunks whose re-run missed (method 2), clock cycles of the walk from group
+ * to group, groups it walked chunk by chunk, 1 when the chain redid the sum,
+ * chunks of zeros passed through.  *status = 0 on success. */
 double hydro_sequential_sum(const double *x, long n, int method, long long *stats, int *status);
 
 /* ---- gravity settings (no reference counterpart) ----------------------- */
