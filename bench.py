@@ -209,10 +209,11 @@ def selfgravity_block(H, ics, n, steps=10, arms=None):
     """The full step with self-gravity on the same 256^3 state, l = 6
     multipole boundary.  "reference_sor": the reference's Poisson settings
     (red-black SOR, omega 1.5, max |update| < 1e-6, at most 10000
-    iterations), moments summed by the pairwise tree.
-    "reference_sor_sequential_moments": the same with the moment terms added
-    one after another as the reference adds them (its bits; fewer steps,
-    each costs tens of milliseconds).  "nn_fp32_dc_multigrid": relative
+    iterations) with the default moment order, the reference's (terms added
+    one after another: its bits).  "reference_sor_tree_moments": the same
+    with the moments summed by the pairwise double-double tree (faster,
+    within 1e-11 relative L1 of the reference's run after 200 steps).
+    "nn_fp32_dc_multigrid": relative
     residual 1e-8, the learned guess (kept only when its residual beats the
     warm start) and FP32 defect correction with multigrid inner solves.  The
     first step solves cold, the next `steps` warm-start from the previous
@@ -240,17 +241,19 @@ def selfgravity_block(H, ics, n, steps=10, arms=None):
         gr.set_nn_model(None)
         s.close()
         info.update({"first_step": {"ms": cold_ms, "iterations": cold_it},
-                     "warm_ms_per_step": sum(ms) / len(ms), "warm_iterations": its})
+                     "warm_ms_per_step": sum(ms) / len(ms), "warm_ms": ms,
+                     "warm_iterations": its})
         return info
 
     def reference(gr):
         gr.set_params(1.5, 10000, 1e-6)
-        return {"sor": {"omega": 1.5, "tol_max_update": 1e-6, "max_iter": 10000}}
+        return {"sor": {"omega": 1.5, "tol_max_update": 1e-6, "max_iter": 10000},
+                "moment_order": "reference (sequential)"}
 
-    def reference_sequential(gr):
+    def reference_tree(gr):
         info = reference(gr)
-        gr.set_moment_order(H.MOMENT_ORDER_SEQUENTIAL)
-        info["moment_order"] = "sequential"
+        gr.set_moment_order(H.MOMENT_ORDER_TREE)
+        info["moment_order"] = "tree"
         return info
 
     def learned_multigrid(gr):
@@ -263,7 +266,7 @@ def selfgravity_block(H, ics, n, steps=10, arms=None):
         return {"tol_rel_residual": 1e-8, "iterations_are": "V-cycles"}
 
     table = {"reference_sor": (reference, steps),
-             "reference_sor_sequential_moments": (reference_sequential, min(steps, 3)),
+             "reference_sor_tree_moments": (reference_tree, steps),
              "nn_fp32_dc_multigrid": (learned_multigrid, steps)}
     out = {"grid": n, "multipole_lmax": 6, "warm_steps": steps}
     for name, (setup, k) in table.items():
@@ -380,7 +383,7 @@ def main():
                     help="only run the self-gravity step block with STEPS warm steps, then exit")
     ap.add_argument("--selfgravity-arms", default="", metavar="NAMES",
                     help="comma-separated subset of the self-gravity arms (reference_sor, "
-                         "reference_sor_sequential_moments, nn_fp32_dc_multigrid)")
+                         "reference_sor_tree_moments, nn_fp32_dc_multigrid)")
     ap.add_argument("--nn-probe", type=int, default=0, metavar="REPS",
                     help="only time REPS forward and REPS training passes of the guess network "
                          "on a 128^3 field, then exit")

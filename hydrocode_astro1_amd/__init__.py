@@ -28,7 +28,7 @@ GRAV_BC_PERIODIC, GRAV_BC_ISOLATED, GRAV_BC_DIRICHLET = 0, 1, 2
 POISSON_FP64_SOR, POISSON_FP32_DC, POISSON_NN_FP32_DC = 0, 1, 2
 SOR_CRIT_MAX_UPDATE, SOR_CRIT_REL_RESID = 0, 1
 DC_INNER_SOR, DC_INNER_MULTIGRID = 0, 1
-MOMENT_ORDER_TREE, MOMENT_ORDER_SEQUENTIAL = 0, 1
+MOMENT_ORDER_TREE, MOMENT_ORDER_SEQUENTIAL, MOMENT_ORDER_AUTO = 0, 1, 2
 # ProblemType (input_deck.h): the values the reference has generators for
 (KELVIN_HELMHOLTZ_3D, BLAST_WAVE_3D, STELLAR_PROFILE_3D, BINARY_STAR_3D,
  STELLAR_OCTANT_EXPLOSION_3D) = 11, 12, 15, 16, 17

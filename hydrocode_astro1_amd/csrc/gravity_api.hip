@@ -41,10 +41,18 @@ int dev_alloc(T** p, long count) {
     return 0;
 }
 
+// Whether the moments are summed in the reference's order: asked for, or
+// by default for plain FP64 SOR (the defect-correction modes keep the
+// compensated tree).
+static bool sequential_moments(const GravitySolver3D* s) {
+    return s->moment_order == MOMENT_ORDER_SEQUENTIAL ||
+           (s->moment_order == MOMENT_ORDER_AUTO && s->poisson_mode == POISSON_FP64_SOR);
+}
+
 // The multipole moments of rho (device) into d->mom, in the solver's order.
 int moments_dev(GravitySolver3D* s, const double* d_rho) {
     GravDevice* d = s->dev;
-    if (s->moment_order != MOMENT_ORDER_SEQUENTIAL) {
+    if (!sequential_moments(s)) {
         HYD_CHECK(hyd::launch_moments(d->g, d_rho, s->multipole_lmax, d->mom_scratch, d->mom,
                                       d->stream));
         return 0;
@@ -228,9 +236,11 @@ int sor_fp64(GravitySolver3D* s) {
     // iterations as the last solve, and iterations past convergence return
     // at once: one batch of that size ends most solves with a single host
     // round trip.  (The relative-residual rule stops only between batches.)
-    // (Capped: after a cold solve the next warm one is far shorter, and each
-    // launch past convergence still costs a few microseconds.)
-    int batch = (!rel && s->last_iterations > 1) ? std::min(s->last_iterations, 128) : 8;
+    // After a long solve (a cold start) the next, warm one is far shorter:
+    // it starts from a batch of 16 and grows, since every launch past
+    // convergence still costs some microseconds.
+    int batch = 8;
+    if (!rel && s->last_iterations > 1) batch = s->last_iterations <= 128 ? s->last_iterations : 16;
     double rel_prev = -1.0;
     int it_prev = 0;
     while (true) {
@@ -648,7 +658,7 @@ GravitySolver3D* gravity_solver_create(int nx, int ny, int nz, double xmin, doub
     s->poisson_mode = POISSON_FP64_SOR;
     s->criterion = SOR_CRIT_MAX_UPDATE;
     s->dc_inner = DC_INNER_SOR;
-    s->moment_order = MOMENT_ORDER_TREE;
+    s->moment_order = MOMENT_ORDER_AUTO;
     const long n = (long)nx * ny * nz;
     s->phi = (double*)calloc(n, sizeof(double));
     s->rho = NULL;
@@ -751,7 +761,9 @@ int gravity_set_poisson_mode(GravitySolver3D* s, int mode) {
 }
 
 int gravity_set_moment_order(GravitySolver3D* s, int order) {
-    if (order != MOMENT_ORDER_TREE && order != MOMENT_ORDER_SEQUENTIAL) return -1;
+    if (order != MOMENT_ORDER_TREE && order != MOMENT_ORDER_SEQUENTIAL &&
+        order != MOMENT_ORDER_AUTO)
+        return -1;
     s->moment_order = order;
     return 0;
 }

@@ -149,7 +149,7 @@ InputDeck* input_deck_create(const char* filename) {
     d->multipole_lmax = 2;
     d->poisson_mode = POISSON_FP64_SOR;
     d->dc_inner = DC_INNER_SOR;
-    d->moment_order = MOMENT_ORDER_TREE;
+    d->moment_order = MOMENT_ORDER_AUTO;
     d->output_enable = true;
     d->checkpoint = false;
 
@@ -200,7 +200,9 @@ InputDeck* input_deck_create(const char* filename) {
         if (strcmp(k, "moment_order") == 0) {
             d->moment_order = (strcmp(v, "sequential") == 0 || strcmp(v, "1") == 0)
                                   ? MOMENT_ORDER_SEQUENTIAL
-                                  : MOMENT_ORDER_TREE;
+                              : (strcmp(v, "tree") == 0 || strcmp(v, "0") == 0)
+                                  ? MOMENT_ORDER_TREE
+                                  : MOMENT_ORDER_AUTO;
             continue;
         }
         if (strcmp(k, "sor_criterion") == 0) {
@@ -252,8 +254,9 @@ void input_deck_print(const InputDeck* d) {
                                                    : "fp64");
     if (d->poisson_mode != POISSON_FP64_SOR)
         printf("  dc_inner = %s\n", d->dc_inner == DC_INNER_MULTIGRID ? "multigrid" : "sor");
-    printf("  moment_order = %s\n",
-           d->moment_order == MOMENT_ORDER_SEQUENTIAL ? "sequential" : "tree");
+    printf("  moment_order = %s\n", d->moment_order == MOMENT_ORDER_SEQUENTIAL ? "sequential"
+                                     : d->moment_order == MOMENT_ORDER_TREE     ? "tree"
+                                                                                : "auto");
     printf("\nOutput:\n  Directory: %s\n", d->output_dir);
     if (d->checkpoint) printf("  checkpoints: output/checkpoint_NNNNNN.bin\n");
     printf("==============================================\n\n");

@@ -116,11 +116,13 @@ int gravity_set_poisson_mode(GravitySolver3D *solver, int mode);
 /* DC_INNER_SOR or DC_INNER_MULTIGRID (input_deck.h) for the FP32 modes;
  * multigrid needs even nx, ny, nz >= 8 (returns -1 otherwise). */
 int gravity_set_dc_inner(GravitySolver3D *solver, int inner);
-/* MOMENT_ORDER_TREE or MOMENT_ORDER_SEQUENTIAL (input_deck.h): how the
- * isolated boundary's moment sums are added.  The sequential order stores
- * every term (max(4, nq) doubles per cell, nq = 9 + sum_{l=3..lmax} 2(l+1))
- * and adds them one after another, as compute_mass_moments and
- * compute_quadrupole_moment do (gravity_solver_3d.c:164-262). */
+/* MOMENT_ORDER_TREE, MOMENT_ORDER_SEQUENTIAL or MOMENT_ORDER_AUTO (the
+ * default; input_deck.h): how the isolated boundary's moment sums are
+ * added.  The sequential order stores every term (max(4, nq) doubles per
+ * cell, nq = 9 + sum_{l=3..lmax} 2(l+1)) and adds them one after another,
+ * as compute_mass_moments and compute_quadrupole_moment do
+ * (gravity_solver_3d.c:164-262), giving the reference's bits; AUTO takes it
+ * for plain FP64 SOR and the compensated tree for the FP32 modes. */
 int gravity_set_moment_order(GravitySolver3D *solver, int order);
 /* SOR_CRIT_MAX_UPDATE (the reference) or SOR_CRIT_REL_RESID. */
 int gravity_set_criterion(GravitySolver3D *solver, int criterion);

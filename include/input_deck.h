@@ -60,10 +60,13 @@ typedef enum {
 
 /* Order of the multipole moment sums, added by this build. */
 typedef enum {
-    MOMENT_ORDER_TREE = 0,       /* pairwise double-double tree (fast)      */
-    MOMENT_ORDER_SEQUENTIAL = 1  /* cell after cell as the reference
+    MOMENT_ORDER_TREE = 0,       /* pairwise double-double tree (compensated) */
+    MOMENT_ORDER_SEQUENTIAL = 1, /* cell after cell as the reference
                                     (gravity_solver_3d.c:182-198, 226-258):
                                     the same bits as its loops            */
+    MOMENT_ORDER_AUTO = 2        /* the default: SEQUENTIAL for plain FP64
+                                    SOR (the reference's results), TREE for
+                                    the FP32 defect-correction modes        */
 } MomentOrder;
 
 /* SOR stopping rules added by this build. */
@@ -157,7 +160,7 @@ typedef struct {
     bool output_enable;       /* false skips CSV/VTK writing              */
     char ic_file[256];        /* binary initial state written by this build */
     int dc_inner;             /* DcInner of the FP32 modes, default SOR   */
-    int moment_order;         /* MomentOrder, default TREE                */
+    int moment_order;         /* MomentOrder, default AUTO                */
     bool checkpoint;          /* also write HYDCK001 checkpoints at output
                                  times (U, potential, g), default false   */
 } InputDeck;

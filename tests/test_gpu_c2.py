@@ -3,15 +3,14 @@
 step against the oracle fixtures tests/golden/c2_polytrope_128.npz
 (multipole boundary to l = 2, the reference's) and c2_polytrope_128_l4.npz
 (to l = 4): dt and SOR iteration count of every step exactly and the final
-state bit for bit (SHA-256), in both moment orders.  With the default
-pairwise-tree moment sums the target is the oracle's tree-summed run; with
-MOMENT_ORDER_SEQUENTIAL (the reference's cell-after-cell sums) it is the
-oracle's sequential run.  The distance between the two runs is checked
-too: relative L1 <= 1e-12 for density and energy, and for the momenta
-relative to the density's L1 (a star near hydrostatic balance has momenta
-that are the small remainder of pressure against gravity, L1 about 0.2% of
-the density's, so their own relative L1 reads 6e-12 for the same last-bit
-moment difference)."""
+state bit for bit (SHA-256), in both moment orders.  The default order for
+plain FP64 SOR is the reference's (cell-after-cell sums): the target is the
+oracle's sequential run, which follows the reference's loops.  With
+MOMENT_ORDER_TREE (pairwise double-double sums) the target is the oracle's
+tree-summed run; its distance from the reference-order run is printed per
+field (relative to each field's own L1) and bounded: 6e-12 for the momenta
+of this star near hydrostatic balance, whose momenta are the small
+remainder of pressure against gravity (L1 about 0.2% of the density's)."""
 import hashlib
 import os
 
@@ -26,14 +25,14 @@ pytestmark = pytest.mark.gpu
 GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
 
 
-@pytest.mark.parametrize("order", ["tree", "sequential"])
+@pytest.mark.parametrize("order", ["tree", "default"])
 @pytest.mark.parametrize("name", ["c2_polytrope_128.npz", "c2_polytrope_128_l4.npz"])
 def test_polytrope_128_steps_bitwise(name, order):
     path = os.path.join(GOLDEN, name)
     if not os.path.exists(path):
         pytest.skip(f"fixture {name} not generated")
     f = np.load(path)
-    pre = "seq_" if order == "sequential" else ""
+    pre = "seq_" if order == "default" else ""
     if pre + "sha256" not in f:
         pytest.skip(f"fixture {name} holds no sequential-order digest")
     n, steps = int(f["n"]), int(f["steps"])
@@ -43,8 +42,8 @@ def test_polytrope_128_steps_bitwise(name, order):
     s = H.Solver(n, n, n, G=1.0, gamma=2.0, cfl=0.4, bc=(H.BC_OUTFLOW,) * 6)
     s.gravity.set_lmax(lmax)
     s.gravity.set_params(float(f["omega"]), 200000, float(f["tol"]))
-    if order == "sequential":
-        s.gravity.set_moment_order(H.MOMENT_ORDER_SEQUENTIAL)
+    if order == "tree":
+        s.gravity.set_moment_order(H.MOMENT_ORDER_TREE)
     s.set_state(U0)
     for k in range(steps):
         assert s.step() == 0
@@ -57,7 +56,8 @@ def test_polytrope_128_steps_bitwise(name, order):
     assert np.array_equal(digest, f[pre + "sha256"])
     assert np.array_equal(U[n // 2, :, :, 0], f[pre + "rho_slice"])
     d, l1 = f["seq_diff_l1"], f["seq_l1"]
-    assert d[0] <= 1e-12 * l1[0] and d[4] <= 1e-12 * l1[4]
-    assert np.all(d[1:4] <= 1e-12 * l1[0])
+    own = d / np.where(l1 > 0, l1, 1.0)
+    print("tree vs reference order, relative L1 per field:", own)
+    assert own[0] <= 1e-12 and own[4] <= 1e-12 and np.all(own[1:4] <= 1e-11)
     assert np.array_equal(f["iters"], f["seq_iters"])
     s.close()

@@ -3,10 +3,11 @@ CFL 0.4, outflow, the reference's SOR settings, multipole boundary to
 l = 6) on the device, 200 steps against the oracle fixture
 tests/golden/c3_explosion_256.npz: the initial state's digest, dt and SOR
 iteration count of every step exactly, and the final state bit for bit
-(SHA-256), in both moment orders.  With the default pairwise-tree moment
-sums the target is the oracle's tree-summed run; with
-MOMENT_ORDER_SEQUENTIAL (the reference's cell-after-cell sums) it is the
-oracle's sequential run.  The two runs' moment sums differ by about 1e-13
+(SHA-256), in both moment orders.  The default order for plain FP64 SOR
+is the reference's (cell-after-cell sums): the target is the oracle's
+sequential run, which follows the reference's loops.  With
+MOMENT_ORDER_TREE (pairwise double-double sums) the target is the oracle's
+tree-summed run.  The two orders' moment sums differ by about 1e-13
 relative, which over 200 steps of the explosion grows to 7e-12 relative L1
 in density and 1e-11 in energy and momenta (printed)."""
 import hashlib
@@ -29,19 +30,19 @@ def sha(a):
 
 
 @pytest.mark.skipif(not os.path.exists(FIX), reason="fixture not generated")
-@pytest.mark.parametrize("order", ["tree", "sequential"])
+@pytest.mark.parametrize("order", ["tree", "default"])
 def test_explosion_256_steps_bitwise(order):
     f = np.load(FIX)
     n, steps = int(f["n"]), int(f["steps"])
-    pre = "seq_" if order == "sequential" else ""
+    pre = "seq_" if order == "default" else ""
     print(f"explosion 256^3, {steps} steps, {order} moment order", flush=True)
     U0 = ics.explosion_n3(n)
     assert np.array_equal(sha(U0), f["ic_sha256"]), "initial state differs from the fixture's"
     s = H.Solver(n, n, n, G=1.0, gamma=5.0 / 3.0, cfl=0.4, bc=(H.BC_OUTFLOW,) * 6)
     s.gravity.set_lmax(int(f["lmax"]))
     s.gravity.set_params(float(f["omega"]), int(f["max_iter"]), float(f["tol"]))
-    if order == "sequential":
-        s.gravity.set_moment_order(H.MOMENT_ORDER_SEQUENTIAL)
+    if order == "tree":
+        s.gravity.set_moment_order(H.MOMENT_ORDER_TREE)
     s.set_state(U0)
     del U0
     for k in range(steps):

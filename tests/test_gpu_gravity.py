@@ -1,10 +1,10 @@
 """Device Poisson solve, multipole boundary and acceleration against the
-CPU oracle.  The oracle's moment mode OR_MOM_DDTREE sums moments with the
-same pairwise double-double tree as the device's default order, so results
-are compared bit for bit; against the reference's sequential sums the bar
-is relative L1.  The device's MOMENT_ORDER_SEQUENTIAL adds the terms one
-after another as the reference does and is compared bit for bit with the
-oracle's OR_MOM_SEQUENTIAL."""
+CPU oracle.  The device's default moment order for plain FP64 SOR adds the
+terms one after another as the reference does (MOMENT_ORDER_SEQUENTIAL)
+and is compared bit for bit with the oracle's OR_MOM_SEQUENTIAL.
+MOMENT_ORDER_TREE sums with the same pairwise double-double tree as the
+oracle's OR_MOM_DDTREE, also compared bit for bit; against the reference's
+sequential sums its bar is relative L1."""
 import numpy as np
 import pytest
 
@@ -26,7 +26,8 @@ def blob(shape, c=(0.45, 0.5, 0.55), R=0.35):
 
 
 ORDERS = {"tree": (H.MOMENT_ORDER_TREE, MOM_DDTREE),
-          "sequential": (H.MOMENT_ORDER_SEQUENTIAL, MOM_SEQUENTIAL)}
+          "sequential": (H.MOMENT_ORDER_SEQUENTIAL, MOM_SEQUENTIAL),
+          "default": (None, MOM_SEQUENTIAL)}
 
 
 @pytest.mark.parametrize("order", ["tree", "sequential"])
@@ -44,7 +45,8 @@ def test_moments_and_boundary_bitwise(lmax, shape, order):
     o = OracleSolver(g, lmax=lmax, moment_mode=or_order)
     dv = H.Gravity(*shape, G=1.0)
     dv.set_lmax(lmax)
-    dv.set_moment_order(dev_order)
+    if dev_order is not None:
+        dv.set_moment_order(dev_order)
     mo, hi_o = o.moments(rho)
     md, hi_d = dv.moments(rho)
     assert same_bits(md, mo)
@@ -57,14 +59,18 @@ def test_moments_and_boundary_bitwise(lmax, shape, order):
     assert same_bits(bd[shell], bo[shell])
 
 
+@pytest.mark.parametrize("order", ["tree", "default"])
 @pytest.mark.parametrize("tol", [1e-6, 1e-8])
-def test_gravity_solve_bitwise(tol):
+def test_gravity_solve_bitwise(tol, order):
+    dev_order, or_order = ORDERS[order]
     shape = (24, 20, 16)
     rho = blob(shape)
     g = Grid(*shape, G=1.0)
-    o = OracleSolver(g, tol=tol, moment_mode=MOM_DDTREE)
+    o = OracleSolver(g, tol=tol, moment_mode=or_order)
     dv = H.Gravity(*shape, G=1.0)
     dv.set_params(1.5, 10000, tol)
+    if dev_order is not None:
+        dv.set_moment_order(dev_order)
     for rep in range(2):  # the second solve warm-starts from the first
         r = rho * (1.0 + 0.01 * rep)
         st_o = o.gravity_solve(r)
@@ -89,8 +95,7 @@ def test_selfgravity_steps_sequential_order_bitwise():
     oseq.set_state(U0)
     d = H.Solver(*shape, G=1.0, gamma=2.0, cfl=0.4, bc=(2,) * 6)
     d.gravity.set_params(1.5, 10000, 1e-8)
-    d.gravity.set_lmax(4)
-    d.gravity.set_moment_order(H.MOMENT_ORDER_SEQUENTIAL)
+    d.gravity.set_lmax(4)  # the default moment order: the reference's
     d.set_state(U0)
     for s in range(4):
         oseq.step()
@@ -112,6 +117,7 @@ def test_selfgravity_steps_bitwise_and_l1_vs_sequential():
     oseq.set_state(U0)
     d = H.Solver(*shape, G=1.0, gamma=2.0, cfl=0.4, bc=(2,) * 6)
     d.gravity.set_params(1.5, 10000, 1e-8)
+    d.gravity.set_moment_order(H.MOMENT_ORDER_TREE)
     d.set_state(U0)
     for s in range(4):
         o.step()
@@ -135,7 +141,7 @@ def test_nonconvergence_keeps_stale_g():
     X, _, _ = ics.centres(shape)
     U0 = ics.prim_to_cons(rho, 0 * X, 0 * X, 0 * X, 0.1 * rho ** 2 + 1e-5, 2.0)
     g = Grid(*shape, G=1.0, gamma=2.0, cfl=0.4, bc=(2,) * 6)
-    o = OracleSolver(g, tol=1e-8, max_iter=5, moment_mode=MOM_DDTREE)
+    o = OracleSolver(g, tol=1e-8, max_iter=5, moment_mode=MOM_SEQUENTIAL)
     o.set_state(U0)
     d = H.Solver(*shape, G=1.0, gamma=2.0, cfl=0.4, bc=(2,) * 6)
     d.gravity.set_params(1.5, 5, 1e-8)
