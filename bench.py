@@ -49,9 +49,13 @@ HBM_PEAK_GBS = 8000.0
 BYTES_PER_ZONE = 80.0
 
 
-def cpu_baseline(U, n, budget_s=20.0):
+def cpu_baseline(U, n, budget_s=20.0, gpu=None):
     """Reference C solver (or the oracle restatement when the reference build
-    is absent) on one pinned core: hydro steps of the same 256^3 state."""
+    is absent) on one pinned core: hydro steps of the same 256^3 state, and,
+    with the reference build, its red-black SOR on the same density (two
+    capped solves: ms per iteration and the fixed cost of a solve).  `gpu`
+    (the GPU's iteration counts) turns those into CPU estimates of the
+    Poisson solve and of a warm self-gravity step."""
     import numpy as np
 
     sys.path.insert(0, os.path.join(ROOT, "tests"))
@@ -83,10 +87,42 @@ def cpu_baseline(U, n, budget_s=20.0):
     except OSError:
         pass
     del s
-    return {"value": n ** 3 * steps / el, "unit": "zone-cycles/s", "cores": 1, "kind": kind,
-            "sample": f"{steps} hydro steps (G=0) of the same {n}^3 state after 1 warm-up step, "
-                      f"single core of {cpu}",
-            "ms_per_step": 1e3 * el / steps}
+    out = {"value": n ** 3 * steps / el, "unit": "zone-cycles/s", "cores": 1, "kind": kind,
+           "sample": f"{steps} hydro steps (G=0) of the same {n}^3 state after 1 warm-up step, "
+                     f"single core of {cpu}",
+           "ms_per_step": 1e3 * el / steps}
+    lib = OL.ref_lib()
+    if lib is not None:
+        rho = np.ascontiguousarray(U[..., 0]).reshape(-1)
+        h = lib.refg_create(n, n, n, 0.0, 1.0, 0.0, 1.0, 0.0, 1.0, 1.0)
+        times = {}
+        for k in (2, 6):
+            lib.refg_set_phi(h, OL.dptr(np.zeros(n ** 3)))
+            lib.refg_set_params(h, 1.5, k, 1e-300)  # the cap ends the solve
+            t0 = time.perf_counter()
+            lib.refg_solve(h, OL.dptr(rho))
+            times[k] = time.perf_counter() - t0
+        lib.refg_destroy(h)
+        per_it = (times[6] - times[2]) / 4 * 1e3
+        fixed = max(0.0, times[2] * 1e3 - 2 * per_it)
+        sor = {"ms_per_iteration": per_it, "ms_fixed_per_solve": fixed,
+               "sample": f"reference gravity_solve (l = 2 moments, boundary, red-black SOR, "
+                         f"acceleration) of the {n}^3 explosion density capped at 2 and at 6 "
+                         f"iterations, same core"}
+        if gpu:
+            if gpu.get("plain_fp64_sor_iterations"):
+                k = gpu["plain_fp64_sor_iterations"]
+                sor["poisson_plain_fp64_warm_ms_estimate"] = fixed + k * per_it
+                sor["poisson_estimate_note"] = (f"{k} iterations, the GPU's count for the warm "
+                                                f"plain FP64 SOR solve to a 1e-8 relative "
+                                                f"residual (the same iterates)")
+            if gpu.get("selfgravity_warm_iterations"):
+                k = gpu["selfgravity_warm_iterations"]
+                sor["selfgravity_warm_step_ms_estimate"] = out["ms_per_step"] + fixed + k * per_it
+                sor["selfgravity_estimate_note"] = (f"hydro step + fixed solve cost + {k} SOR "
+                                                    f"iterations (the GPU's warm count)")
+        out["poisson_sor"] = sor
+    return out
 
 
 def network_timing(H, model, n=128, reps=5):
@@ -104,6 +140,36 @@ def network_timing(H, model, n=128, reps=5):
             "forward_tflops": flops / (f.value * 1e-3) / 1e12,
             "forward_mfma_frac": flops / (f.value * 1e-3) / peak,
             "training_tflops": 3 * flops / (fb.value * 1e-3) / 1e12}
+
+
+def sor_roofline(H, gr, rho, n, k1=100, k2=500):
+    """Device time per red-black iteration from two solves capped at k1 and
+    k2 iterations (the difference cancels the once-per-solve moments,
+    boundary and set-up), priced at the algorithmic bytes of an iteration:
+    FP64 SOR reads phi and rhs and writes phi (24 B per cell); the FP32
+    inner sweeps of defect correction read the correction and the residual
+    and write the correction (12 B per cell)."""
+    import numpy as np
+
+    out = {}
+    for name, mode, bpc in (("sor_fp64", H.POISSON_FP64_SOR, 24.0),
+                            ("sor_fp32_inner", H.POISSON_FP32_DC, 12.0)):
+        gr.set_mode(mode)
+        ms = {}
+        for k in (k1, k2):
+            gr.set_params(1.5, k, 1e-14)
+            gr.set_phi(np.zeros((n, n, n)))
+            gr.solve_potential(rho)
+            ms[k] = (gr.s.last_solve_ms, gr.diagnostics()[0])
+        per_it = (ms[k2][0] - ms[k1][0]) / (ms[k2][1] - ms[k1][1])
+        achieved = bpc * n ** 3 / (per_it * 1e-3) / 1e9
+        out[name] = {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                     "frac": achieved / HBM_PEAK_GBS, "us_per_iteration": 1e3 * per_it,
+                     "bytes_per_cell_per_iteration": bpc,
+                     "iterations_timed": [ms[k1][1], ms[k2][1]]}
+    gr.set_params(1.5, 200000, 1e-8)
+    gr.set_mode(H.POISSON_FP64_SOR)
+    return out
 
 
 def poisson_block(H, ics, n):
@@ -170,6 +236,8 @@ def poisson_block(H, ics, n):
     for name, mode, inner in arms:
         warm[name], phis[name] = arm(mode, phi0, rho1, inner)
     coldres, cphis = {"plain_fp64_sor": cold}, {}
+    # without the network, so that the network's share is isolated
+    coldres["fp32_dc"], cphis["fp32_dc"] = arm(H.POISSON_FP32_DC, np.zeros_like(phi0), rho0)
     coldres["fp32_dc_multigrid"], cphis["fp32_dc_multigrid"] = arm(
         H.POISSON_FP32_DC, np.zeros_like(phi0), rho0, MG)
     if model is not None:
@@ -177,6 +245,7 @@ def poisson_block(H, ics, n):
             coldres[name], cphis[name] = arm(H.POISSON_NN_FP32_DC, np.zeros_like(phi0), rho0,
                                              inner)
     gr.set_dc_inner(H.DC_INNER_SOR)
+    out["roofline"] = sor_roofline(H, gr, rho0, n)
     # tight FP64 solutions as the accuracy yardstick
     gr.set_mode(H.POISSON_FP64_SOR)
     gr.set_params(1.5, 400000, 1e-11)
@@ -521,7 +590,18 @@ def run(args):
         except Exception as e:
             res["selfgravity_step"] = {"error": repr(e)}
     if not args.no_cpu:
-        res["cpu_baseline"] = cpu_baseline(U, n)
+        gpu = {}
+        try:
+            gpu["plain_fp64_sor_iterations"] = res["poisson"]["warm_start"]["plain_fp64_sor"][
+                "iterations"]
+        except (KeyError, TypeError):
+            pass
+        try:
+            its = res["selfgravity_step"]["reference_sor"]["warm_iterations"]
+            gpu["selfgravity_warm_iterations"] = round(sum(its) / len(its))
+        except (KeyError, TypeError, ZeroDivisionError):
+            pass
+        res["cpu_baseline"] = cpu_baseline(U, n, gpu=gpu)
     return res
 
 
