@@ -84,8 +84,7 @@ struct HydroDevice {
     double *U[2][5] = {{nullptr}};
     int cur = 0;
     double *face = nullptr;
-    // [0..2] signal-speed slots, [3..5] their NaN flags, [6] the boundary
-    // kernel's block ticket
+    // [0..2] signal-speed slots, [3..5] their NaN flags, [6] unused
     unsigned long long *ms = nullptr;
     double *dt_log = nullptr;          // [DT_LOG] dt per step (ring)
     double *g_zero = nullptr;          // zero g planes when gravity is off

@@ -152,7 +152,6 @@ int step_device(GodunovHLLC3D* s) {
     bc.face = d->face;
     bc.ms_out = a.ms_out;
     bc.nan_out = a.nan_out;
-    bc.ticket = reinterpret_cast<unsigned int*>(d->ms + 6);
     bc.nx = s->nx; bc.ny = s->ny; bc.nz = s->nz;
     bc.bc[0] = s->bc_xmin; bc.bc[1] = s->bc_xmax;
     bc.bc[2] = s->bc_ymin; bc.bc[3] = s->bc_ymax;

@@ -213,13 +213,17 @@ __global__ void __launch_bounds__(NT) hydro_step_kernel(HydroStepArgs a) {
             load_raw(a, cidx - sx, valid, pr);
             hllc<0>(to_cell(pr, gamma), lds_get(s_cell[ib & 1], row + 1, lane + 1), gamma, fxl);
         }
+        // plane i+1 is loaded one plane ahead of its use, so its latency has a
+        // whole plane of work to hide behind
+        Raw rn;
+        if (ib + 1 < nx) load_raw(a, cidx + sx, valid, rn);
         for (int i = ib; i < ie; ++i) {
             const long off = (long)(i - ib) * sx;
             const bool has_next = (i + 1 < nx);
             const int pb = i & 1;
             __syncthreads();  // B1: plane i complete in LDS
-            Raw rn;
-            if (has_next) load_raw(a, cidx + off + sx, valid, rn);  // prefetch plane i+1
+            Raw rn2;  // plane i+2, used at the next plane
+            if (i + 2 < nx && i + 1 < ie) load_raw(a, cidx + off + 2 * sx, valid, rn2);
 
             const Cell cur = lds_get(s_cell[pb], row + 1, lane + 1);
             const double* me = &s_cell[pb][0][row + 1][lane + 1];
@@ -320,6 +324,7 @@ __global__ void __launch_bounds__(NT) hydro_step_kernel(HydroStepArgs a) {
                     for (int f = 0; f < 5; ++f) a.face[face_index(nx, ny, nz, i, j, k, f)] = q[f];
                 }
             }
+            rn = rn2;
         }
         s_smax[wave] = wave_max_u64(smax);
     } else if (wave == 0) {
@@ -332,15 +337,20 @@ __global__ void __launch_bounds__(NT) hydro_step_kernel(HydroStepArgs a) {
 #pragma unroll
             for (int f = 0; f < 5; ++f) s_fzl[ib & 1][f][lane] = fh[f];
         }
+        Raw rn, rh;  // plane i+1, loaded one plane ahead
+        if (ib + 1 < nx) {
+            load_raw(a, cidx + sx, valid, rn);
+            if (has_halo) load_raw(a, hidx + sx, hvalid, rh);
+        }
         for (int i = ib; i < ie; ++i) {
             const long off = (long)(i - ib) * sx;
             const bool has_next = (i + 1 < nx);
             const int pb = i & 1;
             __syncthreads();  // B1
-            Raw rn, rh;
-            if (has_next) {
-                load_raw(a, cidx + off + sx, valid, rn);
-                if (has_halo) load_raw(a, hidx + off + sx, hvalid, rh);
+            Raw rn2, rh2;
+            if (i + 2 < nx && i + 1 < ie) {
+                load_raw(a, cidx + off + 2 * sx, valid, rn2);
+                if (has_halo) load_raw(a, hidx + off + 2 * sx, hvalid, rh2);
             }
             {
                 double fy[5];
@@ -359,28 +369,39 @@ __global__ void __launch_bounds__(NT) hydro_step_kernel(HydroStepArgs a) {
 #pragma unroll
                 for (int f = 0; f < 5; ++f) s_fzl[pb ^ 1][f][lane] = fh[f];
             }
+            rn = rn2;
+            rh = rh2;
         }
     } else {
         // ------------------------------------------ high halo row (j0+TJ)
         // loads row j0+TJ and the right z halo of the next plane.
+        Raw rn, rh;  // plane i+1, loaded one plane ahead
+        if (ib + 1 < nx) {
+            load_raw(a, cidx + sx, valid, rn);
+            if (has_halo) load_raw(a, hidx + sx, hvalid, rh);
+        }
         for (int i = ib; i < ie; ++i) {
             const long off = (long)(i - ib) * sx;
             const bool has_next = (i + 1 < nx);
             const int pb = i & 1;
             __syncthreads();  // B1
+            Raw rn2, rh2;
+            if (i + 2 < nx && i + 1 < ie) {
+                load_raw(a, cidx + off + 2 * sx, valid, rn2);
+                if (has_halo) load_raw(a, hidx + off + 2 * sx, hvalid, rh2);
+            }
             if (has_next) {
-                Raw rn, rh;
-                load_raw(a, cidx + off + sx, valid, rn);
-                if (has_halo) load_raw(a, hidx + off + sx, hvalid, rh);
                 lds_put(s_cell[pb ^ 1], LR - 1, lane + 1, to_cell(rn, gamma));
                 if (has_halo) lds_put(s_cell[pb ^ 1], lane + 1, LC - 1, to_cell(rh, gamma));
             }
             __syncthreads();  // B2
+            rn = rn2;
+            rh = rh2;
         }
     }
     // one atomic per block for the signal speed of the updated interior; a
-    // NaN speed only raises the flag (the boundary kernel's last block then
-    // reduces the whole state in the reference's order)
+    // NaN speed only raises the flag (hydro_nan_fold_kernel, after the
+    // boundary kernel, then reduces the whole state in the reference's order)
     if (nan_speed) *a.nan_out = 1ull;
     __syncthreads();
     if (threadIdx.x == 0) {
@@ -519,25 +540,6 @@ __global__ void __launch_bounds__(256) hydro_bc_kernel(HydroBCArgs a) {
     smax = block_max_u64(smax);
     if (threadIdx.x == 0 && smax != 0ull && smax > __atomic_load_n(a.ms_out, __ATOMIC_RELAXED))
         atomicMax(a.ms_out, smax);
-    // The last block to finish sees every cell's final value (each block
-    // makes its writes visible before it takes a ticket) and, when a NaN
-    // speed was flagged, redoes the reduction in the reference's order.
-    __shared__ unsigned int s_ticket;
-    __shared__ unsigned long long s_flag;
-    __threadfence();
-    __syncthreads();
-    if (threadIdx.x == 0) s_ticket = atomicAdd(a.ticket, 1u);
-    __syncthreads();
-    if (s_ticket != gridDim.x * gridDim.y * gridDim.z - 1) return;
-    __threadfence();
-    if (threadIdx.x == 0) {
-        *a.ticket = 0u;  // for the next launch
-        s_flag = __atomic_load_n(a.nan_out, __ATOMIC_RELAXED);
-    }
-    __syncthreads();
-    if (s_flag == 0ull) return;
-    const double* u[5] = {a.u[0], a.u[1], a.u[2], a.u[3], a.u[4]};
-    nan_fold_block(u, (long)nx * ny * nz, a.gamma, a.ms_out, a.nan_out);
 }
 
 // Signal speed of a whole state (after an upload).
@@ -562,8 +564,9 @@ __global__ void __launch_bounds__(256) hydro_maxspeed_kernel(const double* __res
         atomicMax(out, smax);
 }
 
-// One block: the reference-order reduction when hydro_maxspeed_kernel
-// flagged a NaN speed.
+// One block: the reference-order reduction when a NaN speed was flagged
+// (by hydro_maxspeed_kernel, or by the step and boundary kernels of a step,
+// launched after them so that it sees every cell's final value).
 __global__ void __launch_bounds__(256) hydro_nan_fold_kernel(const double* d, const double* mx,
                                                              const double* my, const double* mz,
                                                              const double* e, long n, double gamma,
@@ -643,6 +646,11 @@ hipError_t launch_hydro_bc(const HydroBCArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(hydro_bc_kernel,
                        dim3((unsigned)((inner + 255) / 256), (unsigned)outer, a.zcopy ? 4 : 6),
                        dim3(256), 0, st, a);
+    // returns at once unless a NaN speed was flagged (a block-count ticket
+    // in the boundary kernel for this cost 0.1 ms per step: 1024 blocks
+    // queueing on one atomic)
+    hipLaunchKernelGGL(hydro_nan_fold_kernel, dim3(1), dim3(256), 0, st, a.u[0], a.u[1], a.u[2],
+                       a.u[3], a.u[4], (long)a.nx * a.ny * a.nz, a.gamma, a.ms_out, a.nan_out);
     return hipGetLastError();
 }
 

@@ -33,7 +33,6 @@ struct HydroBCArgs {
     const double* face;     // U_pre of the boundary cells
     unsigned long long* ms_out;
     unsigned long long* nan_out;  // NaN flag of ms_out
-    unsigned int* ticket;   // blocks finished (0 between launches)
     int nx, ny, nz;
     int bc[6];              // xmin xmax ymin ymax zmin zmax (BCType values)
     double gamma;
