@@ -347,8 +347,9 @@ class Gravity:
 
     def rel_residual(self, rho):
         r = C.c_double(0.0)
-        _check(self.L.gravity_rel_residual(self.h, _d(self._rho(rho)), C.byref(r)),
-               "gravity_rel_residual")
+        if _check(self.L.gravity_rel_residual(self.h, _d(self._rho(rho)), C.byref(r)),
+                  "gravity_rel_residual") == -1:
+            raise ValueError("gravity_rel_residual is not provided for nz == 1")
         return r.value
 
 

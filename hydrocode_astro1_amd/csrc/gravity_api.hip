@@ -101,6 +101,7 @@ int dev_create(GravitySolver3D* s) {
     g.dx = s->dx; g.dy = s->dy; g.dz = s->dz;
     d->n = (long)s->nx * s->ny * s->nz;
     d->nsplit = (long)s->nx * s->ny * g.nzh;
+    d->two_d = (s->nz == 1);
     d->stream = hyd_stream();
     if (!d->stream) return -2;
     if (const char* e = getenv("HYDRO_MI355X_MG_TAIL")) d->mg_use_tail = atoi(e) != 0;
@@ -154,7 +155,8 @@ SorCoef coefficients(const GravitySolver3D* s) {
     c.dx2i = 1.0 / dx2;
     c.dy2i = 1.0 / dy2;
     c.dz2i = 1.0 / dz2;
-    c.factor = 2.0 * (c.dx2i + c.dy2i + c.dz2i);
+    c.factor = s->nz == 1 ? 2.0 * (c.dx2i + c.dy2i)  // gravity_solver_3d.c:539-545
+                          : 2.0 * (c.dx2i + c.dy2i + c.dz2i);
     c.omega = s->omega;
     return c;
 }
@@ -194,6 +196,11 @@ int rel_residual_dev(GravitySolver3D* s, float* rr, float* rb, double* rel) {
 int apply_boundary(GravitySolver3D* s) {
     GravDevice* d = s->dev;
     const int bc = (s->bc_type == GRAV_BC_PERIODIC) ? 0 : (s->bc_type == GRAV_BC_ISOLATED ? 1 : 2);
+    if (d->two_d) {
+        HYD_CHECK(hyd::launch_sor2d_boundary(d->g, bc, s->G, s->multipole_lmax, d->mom, d->wlm,
+                                             d->phi_r, d->stream));
+        return 0;
+    }
     HYD_CHECK(hyd::launch_sor_boundary(d->g, bc, s->G, s->multipole_lmax, d->mom, d->wlm, d->phi_r,
                                        d->phi_b, d->stream));
     return 0;
@@ -219,9 +226,16 @@ int sor_fp64(GravitySolver3D* s) {
         s->last_residual = 0.0;
         return -1;
     }
+    const double four_pi_G = 4.0 * kPi * s->G;
     auto iteration = [&](int it) -> int {
         // the stop rule of iteration it, evaluated once for both sweeps
         if (!rel) HYD_CHECK(hyd::launch_sor_check(d->ctl, it, tol, d->stream));
+        if (d->two_d) {  // gravity_solver_3d.c:569-634, rho from the caller's array
+            for (int colour = 0; colour < 2; ++colour)
+                HYD_CHECK(hyd::launch_sor2d_sweep(d->g, c, colour, it, stol, four_pi_G, d->phi_r,
+                                                  d->rho_2d, d->ctl, d->stream));
+            return 0;
+        }
         HYD_CHECK(hyd::launch_sor_sweep(d->g, c, 0, it, stol, d->phi_r, d->phi_b, d->rhs_r,
                                         d->rhs_b, d->ctl, d->stream));
         HYD_CHECK(hyd::launch_sor_sweep(d->g, c, 1, it, stol, d->phi_r, d->phi_b, d->rhs_r,
@@ -565,7 +579,16 @@ int potential_device(GravitySolver3D* s, const double* d_rho) {
     GravDevice* d = s->dev;
     HYD_CHECK(hipEventRecord(d->ev0, d->stream));
     const double four_pi_G = 4.0 * kPi * s->G;  // gravity_solver_3d.c:527
-    HYD_CHECK(hyd::launch_natural_to_split(d->g, d_rho, four_pi_G, d->rhs_r, d->rhs_b, d->stream));
+    if (d->two_d) {
+        if (s->poisson_mode != POISSON_FP64_SOR || s->criterion != SOR_CRIT_MAX_UPDATE) {
+            fprintf(stderr, "Error: nz == 1 is solved by FP64 SOR with the max-update rule only.\n");
+            return -2;
+        }
+        d->rho_2d = d_rho;
+    } else {
+        HYD_CHECK(hyd::launch_natural_to_split(d->g, d_rho, four_pi_G, d->rhs_r, d->rhs_b,
+                                               d->stream));
+    }
     if (s->bc_type == GRAV_BC_ISOLATED && moments_dev(s, d_rho)) return -2;
     int st;
     if (s->poisson_mode == POISSON_FP64_SOR) {
@@ -591,16 +614,50 @@ int potential_device(GravitySolver3D* s, const double* d_rho) {
 
 int accel_device(GravitySolver3D* s) {
     GravDevice* d = s->dev;
-    HYD_CHECK(hyd::launch_acceleration(d->g, d->phi_r, d->phi_b, d->gx, d->gy, d->gz, d->stream));
+    if (d->two_d)
+        HYD_CHECK(hyd::launch_accel2d(d->g, d->phi_r, d->gx, d->gy, d->gz, d->stream));
+    else
+        HYD_CHECK(hyd::launch_acceleration(d->g, d->phi_r, d->phi_b, d->gx, d->gy, d->gz, d->stream));
     d->g_valid = true;
+    return 0;
+}
+
+// The potential in the natural layout (device): phi_r itself for nz == 1,
+// else the colours merged into nat_tmp.
+const double* natural_phi(GravDevice* d, int* err) {
+    *err = 0;
+    if (d->two_d) return d->phi_r;
+    if (!d->nat_tmp && dev_alloc(&d->nat_tmp, d->n)) {
+        *err = -2;
+        return nullptr;
+    }
+    if (hyd::launch_split_to_natural(d->g, d->phi_r, d->phi_b, d->nat_tmp, d->stream) !=
+        hipSuccess) {
+        *err = -2;
+        return nullptr;
+    }
+    return d->nat_tmp;
+}
+
+// A natural-layout potential (host) into the device potential.
+int upload_phi(GravDevice* d, const double* phi) {
+    const size_t b = sizeof(double) * d->n;
+    if (d->two_d) {
+        HYD_CHECK(hipMemcpyAsync(d->phi_r, phi, b, hipMemcpyHostToDevice, d->stream));
+        return 0;
+    }
+    if (!d->nat_tmp && dev_alloc(&d->nat_tmp, d->n)) return -2;
+    HYD_CHECK(hipMemcpyAsync(d->nat_tmp, phi, b, hipMemcpyHostToDevice, d->stream));
+    HYD_CHECK(hyd::launch_natural_to_split(d->g, d->nat_tmp, 1.0, d->phi_r, d->phi_b, d->stream));
     return 0;
 }
 
 int download_phi(GravitySolver3D* s) {
     GravDevice* d = s->dev;
-    if (!d->nat_tmp && dev_alloc(&d->nat_tmp, d->n)) return -2;
-    HYD_CHECK(hyd::launch_split_to_natural(d->g, d->phi_r, d->phi_b, d->nat_tmp, d->stream));
-    HYD_CHECK(hipMemcpyAsync(s->phi, d->nat_tmp, sizeof(double) * d->n, hipMemcpyDeviceToHost,
+    int err;
+    const double* nat = natural_phi(d, &err);
+    if (err) return err;
+    HYD_CHECK(hipMemcpyAsync(s->phi, nat, sizeof(double) * d->n, hipMemcpyDeviceToHost,
                              d->stream));
     HYD_CHECK(hipStreamSynchronize(d->stream));
     return 0;
@@ -632,9 +689,10 @@ extern "C" {
 GravitySolver3D* gravity_solver_create(int nx, int ny, int nz, double xmin, double xmax,
                                        double ymin, double ymax, double zmin, double zmax,
                                        double G) {
-    if (nx < 3 || ny < 3 || nz < 3) {
+    if (nx < 3 || ny < 3 || nz < 1 || nz == 2) {
         fprintf(stderr,
-                "Error: gravity solver needs nx, ny, nz >= 3 on the device (got %d x %d x %d).\n",
+                "Error: gravity solver needs nx, ny >= 3 and nz == 1 or >= 3 on the device "
+                "(got %d x %d x %d).\n",
                 nx, ny, nz);
         return NULL;
     }
@@ -756,6 +814,10 @@ int gravity_set_multipole_lmax(GravitySolver3D* s, int lmax) {
 int gravity_set_poisson_mode(GravitySolver3D* s, int mode) {
     if (mode != POISSON_FP64_SOR && mode != POISSON_FP32_DC && mode != POISSON_NN_FP32_DC)
         return -1;
+    if (s->dev->two_d && mode != POISSON_FP64_SOR) {
+        fprintf(stderr, "gravity_set_poisson_mode: nz == 1 is solved by FP64 SOR only\n");
+        return -1;
+    }
     s->poisson_mode = mode;
     return 0;
 }
@@ -790,16 +852,17 @@ int gravity_set_nn_model(GravitySolver3D* s, void* model) {
 
 int gravity_set_criterion(GravitySolver3D* s, int criterion) {
     if (criterion != SOR_CRIT_MAX_UPDATE && criterion != SOR_CRIT_REL_RESID) return -1;
+    if (s->dev->two_d && criterion != SOR_CRIT_MAX_UPDATE) {
+        fprintf(stderr, "gravity_set_criterion: nz == 1 stops by the max-update rule only\n");
+        return -1;
+    }
     s->criterion = criterion;
     return 0;
 }
 
 int gravity_sync_to_device(GravitySolver3D* s) {
     GravDevice* d = s->dev;
-    if (!d->nat_tmp && dev_alloc(&d->nat_tmp, d->n)) return -2;
-    HYD_CHECK(hipMemcpyAsync(d->nat_tmp, s->phi, sizeof(double) * d->n, hipMemcpyHostToDevice,
-                             d->stream));
-    HYD_CHECK(hyd::launch_natural_to_split(d->g, d->nat_tmp, 1.0, d->phi_r, d->phi_b, d->stream));
+    if (upload_phi(d, s->phi)) return -2;
     HYD_CHECK(hipStreamSynchronize(d->stream));
     return 0;
 }
@@ -828,9 +891,10 @@ int gravity_boundary_potential(GravitySolver3D* s, const double* rho, double* ph
     if (upload_rho(s, rho)) return -2;
     if (s->bc_type == GRAV_BC_ISOLATED && moments_dev(s, d->rho_tmp)) return -2;
     if (apply_boundary(s)) return -2;
-    if (!d->nat_tmp && dev_alloc(&d->nat_tmp, d->n)) return -2;
-    HYD_CHECK(hyd::launch_split_to_natural(d->g, d->phi_r, d->phi_b, d->nat_tmp, d->stream));
-    HYD_CHECK(hipMemcpyAsync(phi_out, d->nat_tmp, sizeof(double) * d->n, hipMemcpyDeviceToHost,
+    int err;
+    const double* nat = natural_phi(d, &err);
+    if (err) return err;
+    HYD_CHECK(hipMemcpyAsync(phi_out, nat, sizeof(double) * d->n, hipMemcpyDeviceToHost,
                              d->stream));
     HYD_CHECK(hipStreamSynchronize(d->stream));
     return 0;
@@ -838,6 +902,10 @@ int gravity_boundary_potential(GravitySolver3D* s, const double* rho, double* ph
 
 int gravity_rel_residual(GravitySolver3D* s, const double* rho, double* rel) {
     GravDevice* d = s->dev;
+    if (d->two_d) {
+        fprintf(stderr, "gravity_rel_residual: not provided for nz == 1\n");
+        return -1;
+    }
     if (upload_rho(s, rho)) return -2;
     HYD_CHECK(hyd::launch_natural_to_split(d->g, d->rho_tmp, 4.0 * kPi * s->G, d->rhs_r, d->rhs_b,
                                            d->stream));
@@ -863,12 +931,7 @@ int grav_internal_restore(GravitySolver3D* s, const double* phi, const double* g
                           const double* gy, const double* gz) {
     GravDevice* d = s->dev;
     const size_t b = sizeof(double) * d->n;
-    if (phi) {
-        if (!d->nat_tmp && dev_alloc(&d->nat_tmp, d->n)) return -2;
-        HYD_CHECK(hipMemcpyAsync(d->nat_tmp, phi, b, hipMemcpyHostToDevice, d->stream));
-        HYD_CHECK(hyd::launch_natural_to_split(d->g, d->nat_tmp, 1.0, d->phi_r, d->phi_b,
-                                               d->stream));
-    }
+    if (phi && upload_phi(d, phi)) return -2;
     if (gx && gy && gz) {
         HYD_CHECK(hipMemcpyAsync(d->gx, gx, b, hipMemcpyHostToDevice, d->stream));
         HYD_CHECK(hipMemcpyAsync(d->gy, gy, b, hipMemcpyHostToDevice, d->stream));

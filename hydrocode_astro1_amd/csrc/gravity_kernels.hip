@@ -254,6 +254,96 @@ __global__ void __launch_bounds__(256) sor_boundary_kernel(GridDims g, int bc, d
     (((i + j + k) & 1) ? phi_b : phi_r)[sidx(g, i, j, k)] = v;
 }
 
+// ------------------------------------------------- the 2D branch (nz = 1) --
+// gravity_solver_3d.c:569-634: one z level, no z derivatives, the potential
+// in the natural layout phi[i ny + j]; the stop rule, slots and check kernel
+// are the 3D ones.  One thread per cell of the colour: row i = 1 + blockIdx.y,
+// j = j0 + 2 m with (i + j) % 2 == colour.
+__global__ void __launch_bounds__(256) sor2d_sweep_kernel(GridDims g, SorCoef c, int colour,
+                                                          int it, double tol, double four_pi_G,
+                                                          double* __restrict__ phi,
+                                                          const double* __restrict__ rho,
+                                                          SorCtl* ctl) {
+    if (tol >= 0.0 && ctl->converged) return;
+    const unsigned blin = blockIdx.y * gridDim.x + blockIdx.x;
+    const int i = 1 + blockIdx.y;
+    const int j = 1 + ((i + 1 + colour) & 1) + 2 * (int)(blockIdx.x * 256 + threadIdx.x);
+    unsigned long long rb = 0ull;
+    if (j <= g.ny - 2) {
+        const long idx = (long)i * g.ny + j;
+        const double old = phi[idx];
+        const double lap = (phi[idx - g.ny] + phi[idx + g.ny]) * c.dx2i +
+                           (phi[idx - 1] + phi[idx + 1]) * c.dy2i;
+        const double rhs = four_pi_G * rho[idx];
+        const double gs = (lap - rhs) / c.factor;
+        phi[idx] = old + c.omega * (gs - old);
+        rb = dbits(fabs(gs - old));
+    }
+    if (tol < 0.0) return;
+    rb = block_max_u64(rb);
+    if (threadIdx.x == 0 && rb != 0ull) sor_slot_add(ctl, it & 3, blin, rb);
+}
+
+// Boundary of the 2D branch: the x faces (all j) and the y faces (all i),
+// no z faces (gravity_solver_3d.c:318-400, :450).  The y face writes the
+// corners last, with its coordinate formula.
+__global__ void __launch_bounds__(256) sor2d_boundary_kernel(GridDims g, int bc, double G,
+                                                             int lmax,
+                                                             const double* __restrict__ mom,
+                                                             const double* __restrict__ wlm,
+                                                             double* __restrict__ phi) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= 2L * g.ny + 2L * (g.nx - 2)) return;
+    int i, j;
+    if (t < 2L * g.ny) {
+        i = t < g.ny ? 0 : g.nx - 1;
+        j = (int)(t % g.ny);
+    } else {
+        const long u = t - 2L * g.ny;
+        i = 1 + (int)(u % (g.nx - 2));
+        j = u < g.nx - 2 ? 0 : g.ny - 1;
+    }
+    double v;
+    if (bc == 1) {
+        const double z = g.zmin + (0 + 0.5) * g.dz;
+        double x, y;
+        if (j == 0 || j == g.ny - 1) {
+            x = g.xmin + (i + 0.5) * g.dx;
+            y = (j == 0) ? g.ymin + 0.5 * g.dy : g.ymax - 0.5 * g.dy;
+        } else {
+            x = (i == 0) ? g.xmin + 0.5 * g.dx : g.xmax - 0.5 * g.dx;
+            y = g.ymin + (j + 0.5) * g.dy;
+        }
+        v = mp_potential(G, lmax, mom, wlm, x, y, z);
+    } else if (bc == 0) {  // periodic copies: x then y, gravity_solver_3d.c:273-289
+        const int si = (i == 0) ? g.nx - 1 : i;
+        const int sj = (j == 0) ? g.ny - 1 : j;
+        v = phi[(long)si * g.ny + sj];
+    } else {
+        return;
+    }
+    phi[(long)i * g.ny + j] = v;
+}
+
+// g = -grad phi of the 2D branch (gravity_solver_3d.c:764-809): interior
+// (i, j) from centred differences, gz = 0, the boundary ring all zero.
+__global__ void __launch_bounds__(256) accel2d_kernel(GridDims g, const double* __restrict__ phi,
+                                                      double* __restrict__ gx,
+                                                      double* __restrict__ gy,
+                                                      double* __restrict__ gz) {
+    const long t = (long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= (long)g.nx * g.ny) return;
+    const int i = (int)(t / g.ny), j = (int)(t % g.ny);
+    double ax = 0.0, ay = 0.0;
+    if (i >= 1 && i <= g.nx - 2 && j >= 1 && j <= g.ny - 2) {
+        ax = -(phi[t + g.ny] - phi[t - g.ny]) / (2.0 * g.dx);
+        ay = -(phi[t + 1] - phi[t - 1]) / (2.0 * g.dy);
+    }
+    gx[t] = ax;
+    gy[t] = ay;
+    gz[t] = 0.0;
+}
+
 // --------------------------------------------------------- acceleration --
 // gravity_solver_3d.c:820-897.  The sequential x, y, z boundary copies leave
 // every boundary cell with the interior value at the clamped index.
@@ -893,6 +983,33 @@ hipError_t launch_sor_boundary(const GridDims& g, int bc, double G, int lmax, co
     const long tot = 2L * g.ny * g.nz + 2L * (g.nx - 2) * g.nz + 2L * (g.nx - 2) * (g.ny - 2);
     hipLaunchKernelGGL(sor_boundary_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0, st,
                        g, bc, G, lmax, mom, wlm, phi_r, phi_b);
+    return hipGetLastError();
+}
+
+hipError_t launch_sor2d_sweep(const GridDims& g, const SorCoef& c, int colour, int it, double tol,
+                              double four_pi_G, double* phi, const double* rho, SorCtl* ctl,
+                              hipStream_t st) {
+    const int per_row = (g.ny - 1) / 2;  // cells of one colour in a row, at most
+    const dim3 grid((unsigned)((per_row + 255) / 256), (unsigned)(g.nx - 2));
+    hipLaunchKernelGGL(sor2d_sweep_kernel, grid, dim3(256), 0, st, g, c, colour, it, tol,
+                       four_pi_G, phi, rho, ctl);
+    return hipGetLastError();
+}
+
+hipError_t launch_sor2d_boundary(const GridDims& g, int bc, double G, int lmax, const double* mom,
+                                 const double* wlm, double* phi, hipStream_t st) {
+    if (bc == 2) return hipSuccess;
+    const long tot = 2L * g.ny + 2L * (g.nx - 2);
+    hipLaunchKernelGGL(sor2d_boundary_kernel, dim3((unsigned)((tot + 255) / 256)), dim3(256), 0,
+                       st, g, bc, G, lmax, mom, wlm, phi);
+    return hipGetLastError();
+}
+
+hipError_t launch_accel2d(const GridDims& g, const double* phi, double* gx, double* gy, double* gz,
+                          hipStream_t st) {
+    const long n = (long)g.nx * g.ny;
+    hipLaunchKernelGGL(accel2d_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, st, g, phi,
+                       gx, gy, gz);
     return hipGetLastError();
 }
 

@@ -118,6 +118,15 @@ hipError_t launch_sor_check(SorCtl* ctl, int it, double tol, hipStream_t st);
 // g = -grad phi with the boundary copies of gravity_solver_3d.c:820-897.
 hipError_t launch_acceleration(const GridDims& g, const double* phi_r, const double* phi_b,
                                double* gx, double* gy, double* gz, hipStream_t st);
+// The 2D branch (nz = 1, gravity_solver_3d.c:569-634, :764-809): the
+// potential in the natural layout, rhs = four_pi_G rho formed in the sweep.
+hipError_t launch_sor2d_sweep(const GridDims& g, const SorCoef& c, int colour, int it, double tol,
+                              double four_pi_G, double* phi, const double* rho, SorCtl* ctl,
+                              hipStream_t st);
+hipError_t launch_sor2d_boundary(const GridDims& g, int bc, double G, int lmax, const double* mom,
+                                 const double* wlm, double* phi, hipStream_t st);
+hipError_t launch_accel2d(const GridDims& g, const double* phi, double* gx, double* gy, double* gz,
+                          hipStream_t st);
 
 // ---- defect correction ---------------------------------------------------
 // r = f - L phi (FP64 arithmetic, FP32 store, r_r/r_b may be null) on

@@ -73,6 +73,11 @@ struct GravDevice {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
     bool g_valid = false;  // device g has been computed at least once
+    // nz == 1 (gravity_solver_3d.c:569-634): phi_r holds the potential in
+    // the natural layout; the split arrays and the defect-correction modes
+    // are not used
+    bool two_d = false;
+    const double* rho_2d = nullptr;  // density of the solve under way (nz == 1)
 };
 
 struct OutputPipe;  // hydro_io.hip: snapshot, copy stream and writer thread

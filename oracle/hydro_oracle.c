@@ -737,6 +737,13 @@ void or_get_g(const OrSolver *s, double *gx, double *gy, double *gz) {
     memcpy(gy, s->hgy, ncell(s) * sizeof(double));
     memcpy(gz, s->hgz, ncell(s) * sizeof(double));
 }
+/* the gravity solver's own g (gravity_solver_3d.h: gx, gy, gz), which the
+ * hydro side copies only after a converged solve */
+void or_get_grav_g(const OrSolver *s, double *gx, double *gy, double *gz) {
+    memcpy(gx, s->gx, ncell(s) * sizeof(double));
+    memcpy(gy, s->gy, ncell(s) * sizeof(double));
+    memcpy(gz, s->gz, ncell(s) * sizeof(double));
+}
 double or_get_dt(const OrSolver *s) { return s->dt; }
 void or_get_diag(const OrSolver *s, int *iters, double *res, int *status) {
     if (iters) *iters = s->last_iter;

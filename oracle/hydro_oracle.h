@@ -52,6 +52,7 @@ void or_get_state(const OrSolver *s, double *U);
 void or_set_phi(OrSolver *s, const double *phi);
 void or_get_phi(const OrSolver *s, double *phi);
 void or_get_g(const OrSolver *s, double *gx, double *gy, double *gz);
+void or_get_grav_g(const OrSolver *s, double *gx, double *gy, double *gz);
 double or_get_dt(const OrSolver *s);
 void or_get_diag(const OrSolver *s, int *iters, double *res, int *status);
 

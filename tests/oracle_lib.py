@@ -55,6 +55,7 @@ def oracle_lib():
         for f in ("or_set_state", "or_get_state", "or_set_phi", "or_get_phi"):
             getattr(lib, f).argtypes = [C.c_void_p, _dp]
         lib.or_get_g.argtypes = [C.c_void_p, _dp, _dp, _dp]
+        lib.or_get_grav_g.argtypes = [C.c_void_p, _dp, _dp, _dp]
         lib.or_get_dt.restype = C.c_double
         lib.or_get_dt.argtypes = [C.c_void_p]
         lib.or_get_diag.argtypes = [C.c_void_p, _ip, _dp, _ip]
@@ -184,6 +185,12 @@ class OracleSolver:
         self.lib.or_get_g(self.h, *[dptr(x) for x in a])
         return [x.reshape(self.g.shape) for x in a]
 
+    def grav_g(self):
+        """The gravity solver's own g (written by a converged solve)."""
+        a = [np.empty(self.g.ncells) for _ in range(3)]
+        self.lib.or_get_grav_g(self.h, *[dptr(x) for x in a])
+        return [x.reshape(self.g.shape) for x in a]
+
     @property
     def dt(self):
         return self.lib.or_get_dt(self.h)
@@ -226,6 +233,44 @@ class OracleSolver:
         rho = np.ascontiguousarray(rho, dtype=np.float64).reshape(-1)
         phi = np.ascontiguousarray(phi, dtype=np.float64).reshape(-1)
         return self.lib.or_rel_residual(self.h, dptr(rho), dptr(phi))
+
+
+class RefGravity:
+    """The compiled reference's gravity solver on its own (gravity_solver_3d.c)."""
+
+    def __init__(self, grid, omega=1.5, max_iter=10000, tol=1e-6, grav_bc=GRAV_ISOLATED):
+        self.lib = ref_lib()
+        assert self.lib is not None
+        self.g = grid
+        self.h = self.lib.refg_create(grid.nx, grid.ny, grid.nz, *grid.box, grid.G)
+        self.lib.refg_set_params(self.h, omega, max_iter, tol)
+        self.lib.refg_set_bc(self.h, grav_bc)
+
+    def __del__(self):
+        if getattr(self, "h", None):
+            self.lib.refg_destroy(self.h)
+            self.h = None
+
+    def set_phi(self, phi):
+        self._phi = np.ascontiguousarray(phi, dtype=np.float64).reshape(-1).copy()
+        self.lib.refg_set_phi(self.h, dptr(self._phi))
+
+    def solve(self, rho):
+        """gravity_solve: the potential, then g when it converged.  The
+        solver keeps a pointer to rho, so it is held here."""
+        self._rho = np.ascontiguousarray(rho, dtype=np.float64).reshape(-1).copy()
+        return self.lib.refg_solve(self.h, dptr(self._rho))
+
+    def fields(self):
+        a = [np.empty(self.g.ncells) for _ in range(4)]
+        self.lib.refg_get(self.h, *[dptr(x) for x in a])
+        return [x.reshape(self.g.shape) for x in a]
+
+    def diag(self):
+        it = np.zeros(1, dtype=np.int32)
+        res = np.zeros(1)
+        self.lib.refg_diag(self.h, iptr(it), dptr(res))
+        return int(it[0]), float(res[0])
 
 
 class RefSolver:

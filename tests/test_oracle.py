@@ -420,3 +420,70 @@ def test_dt_nan_order_closed_forms():
     tail = max(speeds(U[-1, -1, -1]))
     assert tail < 1e-10
     assert L.or_compute_dt(OL.dptr(U), *args) == dt_of(tail)
+
+
+# ------------------------------------------------ the 2D branch (nz == 1) --
+def _gravity2d_cases():
+    f = gold("gravity2d.npz")
+    for c in range(int(f["ncase"])):
+        yield c, f, tuple(int(v) for v in f[f"shape_{c}"])
+
+
+def test_gravity2d_golden():
+    """The oracle's nz == 1 branch against the reference's solves: status,
+    iterations, residual, phi and g bit for bit, warm starts included."""
+    for c, f, shape in _gravity2d_cases():
+        g = OL.Grid(*shape, box=tuple(f[f"box_{c}"]), G=1.0)
+        s = OL.OracleSolver(g, tol=float(f[f"tol_{c}"]), max_iter=int(f[f"max_iter_{c}"]),
+                            grav_bc=int(f[f"bc_{c}"]))
+        s.set_phi(f[f"phi0_{c}"])
+        for q, sc in enumerate(f[f"scales_{c}"]):
+            st = s.gravity_solve(f[f"rho_{c}"] * sc)
+            it, res, _ = s.diag()
+            assert (st, it, res) == (int(f[f"status_{c}_{q}"]), int(f[f"iters_{c}_{q}"]),
+                                     float(f[f"res_{c}_{q}"])), (c, q)
+            assert same_bits(s.phi(), f[f"phi_{c}_{q}"]), (c, q)
+            for a, name in zip(s.grav_g(), ("gx", "gy", "gz")):
+                assert same_bits(a, f[f"{name}_{c}_{q}"]), (c, q, name)
+
+
+@needs_ref
+def test_gravity2d_matches_reference_live():
+    """A ragged 2D density with empty cells against the live reference."""
+    rng = np.random.default_rng(11)
+    shape = (27, 41, 1)
+    box = (0.0, 0.9, -0.2, 1.1, 0.0, 0.03)
+    rho = np.ascontiguousarray(rng.lognormal(0.0, 0.5, shape) * (rng.random(shape) > 0.2))
+    g = OL.Grid(*shape, box=box, G=0.7)
+    a = OL.OracleSolver(g, tol=1e-9)
+    b = OL.RefGravity(g, tol=1e-9)
+    for sc in (1.0, 0.98):
+        assert a.gravity_solve(rho * sc) == b.solve(rho * sc) == 0
+        assert a.diag()[:2] == b.diag()
+        for x, y in zip([a.phi()] + a.grav_g(), b.fields()):
+            assert same_bits(x, y)
+
+
+def test_gravity2d_manufactured_solution():
+    """nz == 1: phi* given, rho := (five-point Laplacian of phi*) / (4 pi G)
+    inside, the boundary fixed to phi* (Dirichlet); SOR returns phi* to the
+    tolerance.  A z term in the stencil or the 3D factor would not."""
+    nx, ny = 30, 22
+    g = OL.Grid(nx, ny, 1, box=(0.0, 1.0, 0.0, 1.1, 0.0, 0.05), G=1.0)
+    X, Y, _ = ics.centres((nx, ny, 1), g.box)
+    phi = np.sin(np.pi * X) * np.cos(2 * np.pi * Y) + (X - 0.3 * Y) ** 2 + 0 * Y
+    dx, dy = 1.0 / nx, 1.1 / ny
+    lap = np.zeros_like(phi)
+    lap[1:-1, 1:-1] = ((phi[2:, 1:-1] + phi[:-2, 1:-1] - 2 * phi[1:-1, 1:-1]) / dx ** 2 +
+                       (phi[1:-1, 2:] + phi[1:-1, :-2] - 2 * phi[1:-1, 1:-1]) / dy ** 2)
+    s = OL.OracleSolver(g, tol=1e-13, max_iter=200000, grav_bc=OL.GRAV_DIRICHLET)
+    start = phi.copy()
+    start[1:-1, 1:-1] = 0.0
+    s.set_phi(start)
+    assert s.gravity_solve(lap / (4 * np.pi)) == 0
+    assert np.max(np.abs(s.phi() - phi)) < 1e-9
+    gx, gy, gz = s.grav_g()
+    assert np.array_equal(gz, np.zeros_like(gz))
+    ex = -(phi[2:, 1:-1] - phi[:-2, 1:-1]) / (2 * dx)
+    assert np.max(np.abs(gx[1:-1, 1:-1] - ex)) < 1e-6
+    assert not np.any(gx[0]) and not np.any(gy[:, 0])
