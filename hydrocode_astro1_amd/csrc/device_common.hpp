@@ -330,6 +330,25 @@ __device__ __forceinline__ double shfl_xor_d(double v, int m) {
     return __shfl_xor(v, m, 64);
 }
 
+// Whole-wavefront lane shifts by one lane as DPP moves (wave_shr:1 = 0x138,
+// wave_shl:1 = 0x130; two 32-bit moves per double, no LDS round trip).  The
+// lane with no source (lane 0 for shift_up, lane 63 for shift_down) keeps
+// `edge`.  Every lane of the wavefront must be active.
+__device__ __forceinline__ double lane_shift_up(double v, double edge) {
+    const int lo = __builtin_amdgcn_update_dpp(__double2loint(edge), __double2loint(v), 0x138,
+                                               0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(edge), __double2hiint(v), 0x138,
+                                               0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+__device__ __forceinline__ double lane_shift_down(double v, double edge) {
+    const int lo = __builtin_amdgcn_update_dpp(__double2loint(edge), __double2loint(v), 0x130,
+                                               0xf, 0xf, false);
+    const int hi = __builtin_amdgcn_update_dpp(__double2hiint(edge), __double2hiint(v), 0x130,
+                                               0xf, 0xf, false);
+    return __hiloint2double(hi, lo);
+}
+
 // Sum of a double-double over the 64 lanes of a wavefront as a pairwise tree
 // over lane order (offsets 1, 2, 4, ... 32); every lane ends with the sum.
 __device__ __forceinline__ DD wave_dd_tree(DD v) {

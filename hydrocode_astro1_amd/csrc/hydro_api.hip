@@ -138,7 +138,7 @@ int step_device(GodunovHLLC3D* s) {
     a.dt_out = d->dt_log + (d->step % HydroDevice::DT_LOG);
     a.nx = s->nx; a.ny = s->ny; a.nz = s->nz;
     a.cx = d->cx;
-    a.ntj = a.ntk = a.nxc = 0;
+    a.ntj = a.ntk = a.wpb = 0;
     a.dx = s->dx; a.dy = s->dy; a.dz = s->dz;
     a.cfl = s->cfl;
     a.gamma = s->gamma;

@@ -24,7 +24,7 @@
 //            r forms F_y(r, r+1) against row r+1 and passes it through LDS to
 //            row r+1.
 //   z faces: lane l forms F_z(l, l+1) against LDS; lane l+1 receives it by a
-//            lane shuffle.  The z halo cells k0-1 / k0+64 of rows 0..7 are
+//            DPP lane shift.  The z halo cells k0-1 / k0+64 of rows 0..7 are
 //            loaded by lanes 0..7 of the halo waves.
 // Each face is formed once per block (plus the halo faces: 9 y rows for 8
 // cell rows, 65 z faces for 64 cells), each cell's primitives once.
@@ -135,29 +135,20 @@ __device__ __forceinline__ long face_index(int nx, int ny, int nz, int i, int j,
 // The three roles (cell rows, the low halo row, the high halo row) run
 // separate loops with the same barrier sequence, so each role's registers
 // are allocated for that role alone.
-__global__ void __launch_bounds__(NT) hydro_step_kernel(HydroStepArgs a) {
-    __shared__ double s_cell[2][NF][LR][LC];
-    __shared__ double s_fy[5][TJ + 1][TK];
-    __shared__ double s_fzl[2][5][TJ];
-    __shared__ unsigned long long s_smax[NW];
+typedef double PlaneBufs[2][NF][LR][LC];
+typedef double FyRows[5][TJ + 1][TK];
+typedef double FzHalo[2][5][TJ];
 
+// One segment of a block's work: tile (kt, jt), planes ib .. ie-1.  smax and
+// nan_speed accumulate the signal speed of the cell rows' updated interior
+// over the segments of the block.
+__device__ __forceinline__ void march_segment(const HydroStepArgs& a, PlaneBufs& s_cell,
+                                              FyRows& s_fy, FzHalo& s_fzl, int kt, int jt,
+                                              int ib, int ie, int wave, int lane, double dt,
+                                              const EosConst& kc, unsigned long long& smax,
+                                              bool& nan_speed) {
     const int nx = a.nx, ny = a.ny, nz = a.nz;
-    // wavefront index made scalar so role branches are uniform jumps
-    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
-    const int lane = threadIdx.x & 63;
-
-    // Block -> tile.  Consecutive blocks land on different XCDs; regroup so
-    // that each XCD walks a contiguous range of tiles (shared halos stay in
-    // that XCD's L2).
-    int b = blockIdx.x;
-    const int nb = gridDim.x;
-    if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
-    const int kt = b % a.ntk;
-    const int jt = (b / a.ntk) % a.ntj;
-    const int xc = b / (a.ntk * a.ntj);
     const int j0 = jt * TJ, k0 = kt * TK;
-    const int ib = xc * a.cx;
-    const int ie = min(nx, ib + a.cx);
 
     const int row = wave - 1;  // -1 .. TJ
     const int j = j0 + row;
@@ -175,16 +166,7 @@ __global__ void __launch_bounds__(NT) hydro_step_kernel(HydroStepArgs a) {
     const long sx = (long)ny * nz;
     const long cidx = ((long)ib * ny + j) * nz + k;  // own cell at plane ib
     const long hidx = ((long)ib * ny + hj) * nz + hk;
-
-    const double dt = dt_from_speed(*a.ms_in, *a.nan_in, a.dx, a.dy, a.dz, a.cfl);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        *a.dt_out = dt;
-        *a.ms_clear = 0ull;
-        *a.nan_clear = 0ull;
-    }
-    bool nan_speed = false;  // a NaN signal speed among this lane's new cells
-    const double gamma = a.gamma;
-    const EosConst kc = eos_const(gamma);
+    const double gamma = kc.gamma;
 
     // prologue: plane ib into buffer ib & 1
     {
@@ -207,7 +189,6 @@ __global__ void __launch_bounds__(NT) hydro_step_kernel(HydroStepArgs a) {
         const bool zlo = a.zcopy && k0 == 0;
         const bool zhi = a.zcopy && k0 + TK >= nz;
         double fxl[5] = {0, 0, 0, 0, 0};
-        unsigned long long smax = 0ull;
         if (ib > 0) {  // F_x(ib-1) for the update of plane ib
             Raw pr;
             load_raw(a, cidx - sx, valid, pr);
@@ -241,12 +222,10 @@ __global__ void __launch_bounds__(NT) hydro_step_kernel(HydroStepArgs a) {
                 const double* nb = &s_cell[pb][0][row + 1][lane + 2];
                 hllc_pick<2, FS>(cur.d, cur.w, cur.p, cur.c, nb[0], nb[7 * FS], nb[8 * FS],
                                  nb[9 * FS], me, nb, gamma, fz);
+                // F_z(k-1, k) from the lane below; lane 0 takes the halo
+                // face formed by the low halo wave
 #pragma unroll
-                for (int f = 0; f < 5; ++f) {
-                    double l = __shfl_up(fz[f], 1, 64);
-                    if (lane == 0) l = s_fzl[pb][f][row];
-                    dz[f] = fz[f] - l;
-                }
+                for (int f = 0; f < 5; ++f) dz[f] = fz[f] - lane_shift_up(fz[f], s_fzl[pb][f][row]);
             }
             double tx[5] = {0, 0, 0, 0, 0};
             if (has_next) {
@@ -299,12 +278,12 @@ __global__ void __launch_bounds__(NT) hydro_step_kernel(HydroStepArgs a) {
                 double qz[5];
                 if (zlo) {
 #pragma unroll
-                    for (int f = 0; f < 5; ++f) qz[f] = __shfl_down(q[f], 1, 64);
+                    for (int f = 0; f < 5; ++f) qz[f] = lane_shift_down(q[f], q[f]);
                 }
                 if (zhi) {
 #pragma unroll
                     for (int f = 0; f < 5; ++f) {
-                        const double t = __shfl_up(q[f], 1, 64);
+                        const double t = lane_shift_up(q[f], q[f]);
                         qz[f] = (k == nz - 1) ? t : qz[f];
                     }
                 }
@@ -326,7 +305,6 @@ __global__ void __launch_bounds__(NT) hydro_step_kernel(HydroStepArgs a) {
             }
             rn = rn2;
         }
-        s_smax[wave] = wave_max_u64(smax);
     } else if (wave == 0) {
         // ------------------------------------------- low halo row (j0-1)
         // F_y(j0-1, j0) each plane; loads the left z halo and forms
@@ -399,6 +377,53 @@ __global__ void __launch_bounds__(NT) hydro_step_kernel(HydroStepArgs a) {
             rh = rh2;
         }
     }
+}
+
+// Block -> work.  The (tile, plane) pairs are taken in one flat order, tile
+// after tile (k tiles fastest, then j tiles) and x fastest within a tile;
+// each block takes the next a.wpb of them, which may end one tile and start
+// the next (two segments, each with its own prologue).  With as many blocks
+// as CUs every CU gets the same number of planes in a single round.
+// Consecutive blocks land on different XCDs; they are regrouped so that each
+// XCD walks a contiguous run of tiles (shared halos stay in its L2).
+__global__ void __launch_bounds__(NT) hydro_step_kernel(HydroStepArgs a) {
+    __shared__ PlaneBufs s_cell;
+    __shared__ FyRows s_fy;
+    __shared__ FzHalo s_fzl;
+    __shared__ unsigned long long s_smax[NW];
+
+    const int nx = a.nx;
+    // wavefront index made scalar so role branches are uniform jumps
+    const int wave = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+    const int lane = threadIdx.x & 63;
+
+    int b = blockIdx.x;
+    const int nb = gridDim.x;
+    if ((nb & 7) == 0) b = (b & 7) * (nb >> 3) + (b >> 3);
+    const long total = (long)a.ntk * a.ntj * nx;
+    long f = (long)b * a.wpb;
+    const long fend = min(total, f + (long)a.wpb);
+
+    const double dt = dt_from_speed(*a.ms_in, *a.nan_in, a.dx, a.dy, a.dz, a.cfl);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        *a.dt_out = dt;
+        *a.ms_clear = 0ull;
+        *a.nan_clear = 0ull;
+    }
+    const EosConst kc = eos_const(a.gamma);
+    unsigned long long smax = 0ull;
+    bool nan_speed = false;  // a NaN signal speed among this lane's new cells
+    while (f < fend) {
+        const int t = (int)(f / nx);
+        const int ib = (int)(f - (long)t * nx);
+        const int ie = (int)min((long)nx, ib + (fend - f));
+        march_segment(a, s_cell, s_fy, s_fzl, t % a.ntk, t / a.ntk, ib, ie, wave, lane, dt, kc,
+                      smax, nan_speed);
+        f += ie - ib;
+        // the low halo wave may still read the last plane buffer
+        __syncthreads();
+    }
+    if (wave >= 1 && wave <= TJ) s_smax[wave] = wave_max_u64(smax);
     // one atomic per block for the signal speed of the updated interior; a
     // NaN speed only raises the flag (hydro_nan_fold_kernel, after the
     // boundary kernel, then reduces the whole state in the reference's order)
@@ -596,10 +621,13 @@ __global__ void soa_to_aos_kernel(double* __restrict__ aos, const double* d, con
 }
 
 // ---------------------------------------------------------------- launchers
-// Planes per block along x when the caller leaves it open.  A block fills
-// a CU's LDS, so the grid runs in rounds of one block per CU: pick the chunk
-// count that minimises rounds x (planes per chunk + 1 recomputed plane).
-static int choose_cx(int nx, long tiles) {
+// (tile, plane) pairs per block when the caller leaves it open.  A block
+// fills a CU's LDS, so the grid runs in rounds of one block per CU; a block
+// pays about one plane more per segment it starts (the plane before its
+// first one, for F_x).  Choose the number of rounds R (blocks = R x CUs, the
+// pairs split evenly) that minimises R x (pairs per block + 2); at least 8
+// pairs per block.
+static int choose_wpb(long total) {
     static int ncu = 0;
     if (ncu == 0) {
         int dev = 0, v = 0;
@@ -610,27 +638,26 @@ static int choose_cx(int nx, long tiles) {
         else
             ncu = 256;
     }
-    int best_cx = nx;
-    long best = -1;
-    for (int cx = 8; cx <= 96; ++cx) {
-        const long nxc = (nx + cx - 1) / cx;
-        const long rounds = (tiles * nxc + ncu - 1) / ncu;
-        const long cost = rounds * (cx + 1);
+    long best_w = total, best = -1;
+    for (int r = 1; r <= 8; ++r) {
+        const long w = (total + (long)r * ncu - 1) / ((long)r * ncu);
+        if (w < 8 && r > 1) break;
+        const long cost = (long)r * (w + 2);
         if (best < 0 || cost < best) {
             best = cost;
-            best_cx = cx;
+            best_w = w;
         }
     }
-    return best_cx < nx ? best_cx : nx;
+    return (int)(best_w < 8 ? (total < 8 ? total : 8) : best_w);
 }
 
 hipError_t launch_hydro_step(const HydroStepArgs& a, hipStream_t st) {
     const int ntj = (a.ny + TJ - 1) / TJ, ntk = (a.nz + TK - 1) / TK;
     HydroStepArgs b = a;
-    if (b.cx <= 0) b.cx = choose_cx(a.nx, (long)ntj * ntk);
-    const int nxc = (a.nx + b.cx - 1) / b.cx;
-    b.ntj = ntj; b.ntk = ntk; b.nxc = nxc;
-    const long nblk = (long)ntj * ntk * nxc;
+    const long total = (long)ntj * ntk * a.nx;
+    b.ntj = ntj; b.ntk = ntk;
+    b.wpb = a.cx > 0 ? a.cx : choose_wpb(total);
+    const long nblk = (total + b.wpb - 1) / b.wpb;
     hipLaunchKernelGGL(hydro_step_kernel, dim3((unsigned)nblk), dim3(NT), 0, st, b);
     return hipGetLastError();
 }

@@ -21,8 +21,8 @@ struct HydroStepArgs {
     unsigned long long* nan_clear;
     double* dt_out;         // dt used by this step
     int nx, ny, nz;
-    int cx;                 // planes per block along x
-    int ntj, ntk, nxc;      // filled by the launcher
+    int cx;                 // (tile, plane) pairs per block; 0: the launcher chooses
+    int ntj, ntk, wpb;      // filled by the launcher (wpb: pairs per block)
     double dx, dy, dz, cfl, gamma;
     int gravity;
     int zcopy;              // z faces outflow on both sides: the step writes them (hydro_zcopy)

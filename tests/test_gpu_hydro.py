@@ -64,9 +64,10 @@ def test_random_state_each_bc_five_steps_bitwise(bc):
 
 @pytest.mark.parametrize("cx", ["0", "5", "64"])
 def test_tile_edges_bitwise(cx, monkeypatch):
-    """A grid that is not a multiple of the 10 x 64 tile in j and k, with x
-    chunks of the automatic size, 5 planes and one chunk (cx 64 > nx / 2),
-    with strong shocks and floors."""
+    """A grid that is not a multiple of the 10 x 64 tile in j and k, with
+    strong shocks and floors, and blocks of the automatic size, of 5 and of
+    64 (tile, plane) pairs: runs that start and end inside a tile and runs
+    that end one tile (nx = 70) and start the next."""
     monkeypatch.setenv("HYDRO_MI355X_CX", cx)
     shape = (70, 19, 131)
     rng = np.random.default_rng(3)
