@@ -172,6 +172,65 @@ def sor_roofline(H, gr, rho, n, k1=100, k2=500):
     return out
 
 
+def rank_share(H, ics, n, ranks=(2, 4, 8), steps=30, k1=100, k2=300):
+    """One rank's share of the n^3 explosion run under an x-slab
+    decomposition over N GPUs, timed on this GPU at its real shape: the n/N
+    planes around the centre (the hot core included) plus one ghost plane on
+    each side, which is what a rank holds after its halo exchange.  The hydro
+    step (G = 0) counts zone-cycles of the slab's own planes only; FP64 SOR
+    and the FP32 inner sweeps are timed per red-black iteration from two
+    capped solves on the slab's density.  The exchanges themselves are not
+    run (one GPU): a hydro step sends one plane of 40 B cells to each
+    neighbour, an SOR half-iteration one colour plane of 8 B cells, priced
+    here at one xGMI link of 153 GB/s each way."""
+    import numpy as np
+
+    link = 153e9
+    U = ics.explosion_n3(n)
+    out = {"grid": n, "link_GB_per_s": link / 1e9, "hydro_steps": steps,
+           "sor_iterations_timed": [k1, k2], "slabs": {}}
+    for N in ranks:
+        m = n // N
+        lo = n // 2 - m // 2 - 1
+        nx = m + 2
+        slab = np.ascontiguousarray(U[lo:lo + nx])
+        box = (lo / n, (lo + nx) / n, 0.0, 1.0, 0.0, 1.0)
+        s = H.Solver(nx, n, n, box=box, G=0.0, gamma=5.0 / 3.0, cfl=0.4,
+                     bc=(H.BC_OUTFLOW,) * 6)
+        s.set_state(slab)
+        s.benchmark(5)
+        wall, step_ms, bc_ms = s.benchmark(steps)
+        s.close()
+        own = float(m) * n * n
+        per_rank = own * steps / (wall * 1e-3)
+        halo_us = 1e6 * n * n * 40.0 / link
+        r = {"slab": [nx, n, n], "own_planes": m, "hydro_ms_per_step": wall / steps,
+             "hydro_step_kernel_ms": step_ms, "hydro_boundary_ms": bc_ms,
+             "zone_cycles_per_s_per_rank": per_rank,
+             "zone_cycles_per_s_if_halo_hidden": N * per_rank,
+             "hydro_halo_us_per_step_per_link": halo_us}
+        rho = np.ascontiguousarray(slab[..., 0])
+        g = H.Gravity(nx, n, n, box=box, G=1.0)
+        g.set_lmax(6)
+        g.set_criterion(H.SOR_CRIT_REL_RESID)
+        for name, mode in (("sor_fp64", H.POISSON_FP64_SOR), ("sor_fp32_inner", H.POISSON_FP32_DC)):
+            g.set_mode(mode)
+            t = {}
+            for k in (k1, k2):
+                g.set_params(1.5, k, 1e-14)
+                g.set_phi(np.zeros((nx, n, n)))
+                g.solve_potential(rho)
+                t[k] = (g.s.last_solve_ms, g.diagnostics()[0])
+            r[name + "_us_per_iteration"] = 1e3 * (t[k2][0] - t[k1][0]) / (t[k2][1] - t[k1][1])
+        g.close()
+        r["sor_fp64_halo_us_per_iteration_per_link"] = 2 * 1e6 * (n * n / 2) * 8.0 / link
+        out["slabs"][str(N)] = r
+        print(f"[bench] rank share of {N}: {nx}x{n}x{n}, hydro {wall / steps:.3f} ms/step, "
+              f"SOR {r['sor_fp64_us_per_iteration']:.1f} us/iteration", file=sys.stderr,
+              flush=True)
+    return out
+
+
 def poisson_block(H, ics, n):
     """256^3 Poisson solves to ||f - L phi|| / ||f|| <= 1e-8 (interior).
 
@@ -456,6 +515,9 @@ def main():
     ap.add_argument("--nn-probe", type=int, default=0, metavar="REPS",
                     help="only time REPS forward and REPS training passes of the guess network "
                          "on a 128^3 field, then exit")
+    ap.add_argument("--rank-share", action="store_true",
+                    help="only time one rank's share of the 256^3 run for 2, 4 and 8 x slabs "
+                         "(hydro step and SOR iterations at the slab's shape), then exit")
     ap.add_argument("--calibrate", action="store_true",
                     help="only run the device layout-conversion copy (40 B read + 40 B written "
                          "per cell) used to scale HBM byte counters, then exit")
@@ -533,6 +595,8 @@ def run(args):
             out.append({"mode": int(mode), "iterations": it, "ms": g.s.last_solve_ms,
                         "us_per_iteration": 1e3 * g.s.last_solve_ms / max(it, 1)})
         return {"poisson_probe": out, "grid": n}
+    if args.rank_share:
+        return {"rank_share": rank_share(H, ics, n)}
     if args.calibrate:
         import numpy as np
         s = H.Solver(n, n, n, G=0.0)
