@@ -207,18 +207,23 @@ __global__ void __launch_bounds__(256) pk_block_kernel(const double* __restrict_
     const int q = blockIdx.y;
     double v[SEQ_CHUNK];
     load_chunk(terms + (long)q * npad + j * SEQ_BLOCK1 + lane * SEQ_CHUNK, v);
+    // the lane's running sum (its prefixes) with the exact error of each
+    // addition accumulated beside it: a compensated sum (Ogita, Rump and
+    // Oishi's Sum2), as accurate as a double-double for 16 terms and a third
+    // of its cost; the statistics only predict the running sum, whose exact
+    // value the certificates establish later
     double pre[SEQ_CHUNK];
-    double run = 0.0;
-    DD acc{0.0, 0.0};
+    double run = 0.0, err = 0.0;
     bool zero = true;
 #pragma unroll
     for (int e = 0; e < SEQ_CHUNK; ++e) {
-        run = run + v[e];
+        const DD t = two_sum(run, v[e]);
+        run = t.hi;
+        err = err + t.lo;
         pre[e] = run;
-        acc = dd_add(acc, DD{v[e], 0.0});
         zero = zero && v[e] == 0.0;
     }
-    DD incl = acc;
+    DD incl = two_sum(run, err);
 #pragma unroll
     for (int o = 1; o < 64; o <<= 1) {
         const DD u = dd_shfl_up(incl, o);
@@ -586,27 +591,43 @@ __device__ __forceinline__ double certificate(double mm, unsigned em, unsigned k
     return allow < ALLOW_MAX ? allow : ALLOW_MAX;
 }
 
-// Tabulation: lane i of sequence blockIdx.y takes entry i of the lane list,
+// Tabulation stages: each chunk of a wavefront is read 32 terms at a time
+// into a row of the wavefront's LDS block by coalesced loads (two rows per
+// load instruction), and every lane adds its chunk's row in order.  A
+// wavefront spans at most 33 chunks: pk_class_kernel gives each chunk 2^P >=
+// 2 consecutive lanes, so only the first and the last can be cut.
+constexpr int TAB_STAGE = 32;                   // terms per row and stage
+constexpr int TAB_ROWS = 33;                    // chunks per wavefront
+constexpr int TAB_PAIRS = (TAB_ROWS + 1) / 2;   // load instructions per stage
+constexpr int TAB_STRIDE = TAB_STAGE + 1;       // row stride (doubles): rows on distinct banks
+
+// Tabulation: lane i of sequence blockIdx.x takes entry i of the lane list,
 // start value base + r of chunk c, and runs it through the chunk's terms,
-// keeping its certificate (the grid strides over the list).
-__global__ void __launch_bounds__(256) pk_tab_kernel(const double* __restrict__ terms, long npad,
-                                                     const Peak* __restrict__ pk,
-                                                     const int* __restrict__ lanes,
-                                                     const int* __restrict__ counts,
-                                                     double* __restrict__ tab,
-                                                     double* __restrict__ cert) {
+// keeping its certificate (one wavefront per workgroup, blockIdx.x the
+// sequence; blockIdx.y strides over the list).  The additions are those of run_terms, in the same order:
+// a lane past its chunk's end adds +0, which leaves its sum as it is.
+__global__ void __launch_bounds__(64) pk_tab_kernel(const double* __restrict__ terms, long npad,
+                                                    const Peak* __restrict__ pk,
+                                                    const int* __restrict__ lanes,
+                                                    const int* __restrict__ counts,
+                                                    double* __restrict__ tab,
+                                                    double* __restrict__ cert) {
+    __shared__ double s_rows[TAB_ROWS * TAB_STRIDE];
+    __shared__ long s_first[TAB_ROWS], s_len[TAB_ROWS];
     const long nc = npad / SEQ_CBLK;
-    const int q = blockIdx.y;
+    const int q = blockIdx.x;
     const int n = counts[q];
     const int* lq = lanes + (long)q * TAB * nc;
-    for (long i0 = (long)blockIdx.x * 256; i0 < n; i0 += (long)gridDim.x * 256) {
-        const long i = i0 + threadIdx.x;
-        const bool active = i < n;
-        if (__ballot(active) == 0ull) continue;  // uniform across the wavefront
+    const double* tq = terms + (long)q * npad;
+    const int lane = threadIdx.x;
+    const int half = lane >> 5, col = lane & 31;
+    for (long i0 = (long)blockIdx.y * 64; i0 < n; i0 += (long)gridDim.y * 64) {
+        const long i = i0 + lane;
+        const bool active = i < n;  // lane 0 always is
         const int ent = active ? lq[i] : 0;
-        const long c = ent / TAB;
+        const long c = active ? ent / TAB : -1;
         const int r = ent % TAB;
-        const Chunk k = chunk_of(pk + (long)q * nc, nc, npad, c);
+        const Chunk k = chunk_of(pk + (long)q * nc, nc, npad, active ? c : 0);
         const StitchInfo info = stitch_info(k);
         const long long I = info.base + r;
         double s[1] = {dfrom(((unsigned long long)info.key << 52) |
@@ -614,7 +635,51 @@ __global__ void __launch_bounds__(256) pk_tab_kernel(const double* __restrict__ 
         double mm[1] = {__builtin_huge_val()};
         unsigned em[1] = {0u};
         const long len = active ? k.last - k.first + 1 : 0;
-        run_terms<1, true>(terms + (long)q * npad + (active ? k.first : 0), len, s, mm, em);
+        // the wavefront's chunks, in list order: row d is the d-th chunk
+        const long cprev = __shfl_up(c, 1, 64);
+        const bool leader = active && (lane == 0 || cprev != c);
+        const unsigned long long lmask = __ballot(leader);
+        const int D = __popcll(lmask);
+        if (D > TAB_ROWS) {  // not produced by pk_class_kernel; kept correct, term by term
+            const double* pc = tq + (active ? k.first : 0);
+            for (long t = 0; t < len; ++t) add_all<1, true>(pc[t], s, mm, em);
+        } else {
+            const unsigned long long upto = lane == 63 ? ~0ull : (2ull << lane) - 1;
+            const int row = __popcll(lmask & upto) - 1;  // inactive lanes: the last row
+            if (leader) {
+                s_first[row] = k.first;
+                s_len[row] = len;
+            }
+            __syncthreads();
+            const long maxlen = wave_max_long(len);
+            double pre[TAB_PAIRS];  // the next stage's terms, two rows per load
+            auto load = [&](long base) {
+#pragma unroll
+                for (int p = 0; p < TAB_PAIRS; ++p) {
+                    const int d = 2 * p + half;
+                    double v = 0.0;
+                    if (d < D) {
+                        const long idx = base + col;
+                        if (idx < s_len[d]) v = tq[s_first[d] + idx];
+                    }
+                    pre[p] = v;
+                }
+            };
+            load(0);
+            for (long base = 0; base < maxlen; base += TAB_STAGE) {
+#pragma unroll
+                for (int p = 0; p < TAB_PAIRS; ++p) {
+                    const int d = 2 * p + half;
+                    if (d < D) s_rows[d * TAB_STRIDE + col] = pre[p];
+                }
+                __syncthreads();
+                if (base + TAB_STAGE < maxlen) load(base + TAB_STAGE);
+                const double* rowp = s_rows + row * TAB_STRIDE;
+#pragma unroll
+                for (int t = 0; t < TAB_STAGE; ++t) add_all<1, true>(rowp[t], s, mm, em);
+                __syncthreads();  // the row block is written again next stage
+            }
+        }
         if (active) {
             tab[((long)q * nc + c) * TAB + r] = s[0];
             cert[((long)q * nc + c) * TAB + r] = certificate(mm[0], em[0], info.key, info.P);
@@ -1198,8 +1263,10 @@ hipError_t launch_seqsum(int method, const double* terms, long npad, int nq, dou
     hipLaunchKernelGGL(pk_scan_kernel, dim3(uq), dim3(1024), 0, st, bs, nb, pk);
     hipLaunchKernelGGL(pk_class_kernel, dim3(uq), dim3(1024), 0, st, pk, npad, lanes, counts);
     // about 2 lanes per chunk (most have P = 1); the grid strides over more
-    hipLaunchKernelGGL(pk_tab_kernel, dim3((unsigned)((4 * nc + 255) / 256), uq), dim3(256), 0,
-                       st, terms, npad, pk, lanes, counts, tab, cert);
+    // the sequence is the fastest launch index, so the wavefronts of every
+    // sequence's longest chunks start first and the short ones fill the tail
+    hipLaunchKernelGGL(pk_tab_kernel, dim3(uq, (unsigned)((4 * nc + 63) / 64)), dim3(64), 0, st,
+                       terms, npad, pk, lanes, counts, tab, cert);
     hipLaunchKernelGGL(pk_group_kernel, dim3((unsigned)ng, uq), dim3(64), 0, st, tab, cert, npad,
                        pk, gm, gh);
     hipLaunchKernelGGL(pk_stitch_kernel, dim3(uq), dim3(64), STITCH_LDS_BYTES, st, terms, npad,
