@@ -216,8 +216,10 @@ def sequential_sum(x, method=1, stats=False):
     the device: method 0 adds the terms in one chain, method 1 by chunks
     with tabulated, certified increments carried group by group (the moment
     sums of MOMENT_ORDER_SEQUENTIAL use it), method 2 as 1 and then re-runs
-    every chunk from its exact start (a check of the certificates).
-    stats=True also returns methods 1 and 2's counters as a dict."""
+    every chunk from its exact start (a check of the certificates), method 3
+    as 1 with the tables built by the LDS-staged kernel that the moment sums
+    use (a single sum otherwise takes the per-lane kernel).
+    stats=True also returns methods 1, 2 and 3's counters as a dict."""
     x = np.ascontiguousarray(x, dtype=np.float64)
     st = C.c_int(-1)
     cnt = (C.c_longlong * 9)()

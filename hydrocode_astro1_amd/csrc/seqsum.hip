@@ -591,6 +591,45 @@ __device__ __forceinline__ double certificate(double mm, unsigned em, unsigned k
     return allow < ALLOW_MAX ? allow : ALLOW_MAX;
 }
 
+// Tabulation for launches with few sums: lane i of sequence blockIdx.y takes
+// entry i of the lane list and runs its start value through the chunk's
+// terms from global memory, 32 in flight per lane.  With a handful of sums
+// there are only a few hundred wavefronts, too few for the staged kernel
+// below, whose stages put each wavefront's loads, LDS round trip and
+// additions one after another.
+__global__ void __launch_bounds__(256) pk_tab_lanes_kernel(const double* __restrict__ terms,
+                                                           long npad, const Peak* __restrict__ pk,
+                                                           const int* __restrict__ lanes,
+                                                           const int* __restrict__ counts,
+                                                           double* __restrict__ tab,
+                                                           double* __restrict__ cert) {
+    const long nc = npad / SEQ_CBLK;
+    const int q = blockIdx.y;
+    const int n = counts[q];
+    const int* lq = lanes + (long)q * TAB * nc;
+    for (long i0 = (long)blockIdx.x * 256; i0 < n; i0 += (long)gridDim.x * 256) {
+        const long i = i0 + threadIdx.x;
+        const bool active = i < n;
+        if (__ballot(active) == 0ull) continue;  // uniform across the wavefront
+        const int ent = active ? lq[i] : 0;
+        const long c = ent / TAB;
+        const int r = ent % TAB;
+        const Chunk k = chunk_of(pk + (long)q * nc, nc, npad, c);
+        const StitchInfo info = stitch_info(k);
+        const long long I = info.base + r;
+        double s[1] = {dfrom(((unsigned long long)info.key << 52) |
+                             ((unsigned long long)(I < 0 ? -I : I) & MANT))};
+        double mm[1] = {__builtin_huge_val()};
+        unsigned em[1] = {0u};
+        const long len = active ? k.last - k.first + 1 : 0;
+        run_terms<1, true>(terms + (long)q * npad + (active ? k.first : 0), len, s, mm, em);
+        if (active) {
+            tab[((long)q * nc + c) * TAB + r] = s[0];
+            cert[((long)q * nc + c) * TAB + r] = certificate(mm[0], em[0], info.key, info.P);
+        }
+    }
+}
+
 // Tabulation stages: each chunk of a wavefront is read 32 terms at a time
 // into a row of the wavefront's LDS block by coalesced loads (two rows per
 // load instruction), and every lane adds its chunk's row in order.  A
@@ -600,6 +639,7 @@ constexpr int TAB_STAGE = 32;                   // terms per row and stage
 constexpr int TAB_ROWS = 33;                    // chunks per wavefront
 constexpr int TAB_PAIRS = (TAB_ROWS + 1) / 2;   // load instructions per stage
 constexpr int TAB_STRIDE = TAB_STAGE + 1;       // row stride (doubles): rows on distinct banks
+constexpr int TAB_FEW_SUMS = 8;  // up to this many sums per launch: pk_tab_lanes_kernel
 
 // Tabulation: lane i of sequence blockIdx.x takes entry i of the lane list,
 // start value base + r of chunk c, and runs it through the chunk's terms,
@@ -1240,7 +1280,8 @@ hipError_t launch_seqsum(int method, const double* terms, long npad, int nq, dou
                            sums);
         return hipGetLastError();
     }
-    if (method != SEQSUM_PEAKS && method != SEQSUM_PEAKS_CHECKED) return hipErrorInvalidValue;
+    if (method != SEQSUM_PEAKS && method != SEQSUM_PEAKS_CHECKED && method != SEQSUM_PEAKS_STAGED)
+        return hipErrorInvalidValue;
     // the chunk runs read up to 32 doubles past a sequence: the workspace
     // must follow the terms
     if (ws < terms + (long)nq * npad) return hipErrorInvalidValue;
@@ -1263,10 +1304,15 @@ hipError_t launch_seqsum(int method, const double* terms, long npad, int nq, dou
     hipLaunchKernelGGL(pk_scan_kernel, dim3(uq), dim3(1024), 0, st, bs, nb, pk);
     hipLaunchKernelGGL(pk_class_kernel, dim3(uq), dim3(1024), 0, st, pk, npad, lanes, counts);
     // about 2 lanes per chunk (most have P = 1); the grid strides over more
-    // the sequence is the fastest launch index, so the wavefronts of every
-    // sequence's longest chunks start first and the short ones fill the tail
-    hipLaunchKernelGGL(pk_tab_kernel, dim3(uq, (unsigned)((4 * nc + 63) / 64)), dim3(64), 0, st,
-                       terms, npad, pk, lanes, counts, tab, cert);
+    if (nq <= TAB_FEW_SUMS && method != SEQSUM_PEAKS_STAGED) {
+        hipLaunchKernelGGL(pk_tab_lanes_kernel, dim3((unsigned)((4 * nc + 255) / 256), uq),
+                           dim3(256), 0, st, terms, npad, pk, lanes, counts, tab, cert);
+    } else {
+        // the sequence is the fastest launch index, so the wavefronts of every
+        // sequence's longest chunks start first and the short ones fill the tail
+        hipLaunchKernelGGL(pk_tab_kernel, dim3(uq, (unsigned)((4 * nc + 63) / 64)), dim3(64), 0,
+                           st, terms, npad, pk, lanes, counts, tab, cert);
+    }
     hipLaunchKernelGGL(pk_group_kernel, dim3((unsigned)ng, uq), dim3(64), 0, st, tab, cert, npad,
                        pk, gm, gh);
     hipLaunchKernelGGL(pk_stitch_kernel, dim3(uq), dim3(64), STITCH_LDS_BYTES, st, terms, npad,

@@ -36,13 +36,21 @@
 //                 its exact start, all in parallel, and must land on the next
 //                 chunk's start; if any does not, the sequence is redone by
 //                 the chain (a check of the certificates; tests use it).
+//   SEQSUM_PEAKS_STAGED  SEQSUM_PEAKS with the LDS-staged tabulation whatever
+//                 the number of sums (SEQSUM_PEAKS takes it above 8 sums per
+//                 launch), so that tests reach it with single sums.
 #pragma once
 
 #include <hip/hip_runtime.h>
 
 namespace hyd {
 
-enum SeqSumMethod { SEQSUM_CHAIN = 0, SEQSUM_PEAKS = 1, SEQSUM_PEAKS_CHECKED = 2 };
+enum SeqSumMethod {
+    SEQSUM_CHAIN = 0,
+    SEQSUM_PEAKS = 1,
+    SEQSUM_PEAKS_CHECKED = 2,
+    SEQSUM_PEAKS_STAGED = 3
+};
 
 constexpr int SEQ_CHUNK = 16;                    // terms per lane in a block pass
 constexpr int SEQ_BLOCK1 = 64 * SEQ_CHUNK;        // 1024 terms (one wavefront)

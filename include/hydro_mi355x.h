@@ -97,8 +97,10 @@ int hydro_fill_initial_state(int nx, int ny, int nz, double xmin, double xmax, d
  * chain of additions), 1 (chunks that start at the running sum's predicted
  * peaks, each run from one start value per residue of the start's low bits
  * with a certificate of how far a start may move, and the exact sum carried
- * through those tables group by group) or 2 (method 1, then every chunk run
- * again from its exact start; any mismatch redoes the sum by the chain).
+ * through those tables group by group), 2 (method 1, then every chunk run
+ * again from its exact start; any mismatch redoes the sum by the chain) or 3
+ * (method 1 with the tables built by the LDS-staged kernel that the moment
+ * sums use, which a single sum would otherwise not reach).
  * stats (9 values, or NULL) receives methods 1 and 2's counters: chunks,
  * chunks with 1-3 and with 4-6 residue bits, chunks added term by term,
  * chunks whose re-run missed (method 2), clock cycles of the walk from group

@@ -3,7 +3,9 @@ the plain add chain, the peak-chunk method and its checked form all equal
 an in-order loop on the CPU bit for bit, on the sequences of
 tests/seqsum_model.py that stress the method (ties, walks across zero,
 cancellation, wide dynamic range, subnormals, zeros, ragged lengths, column
-swings).  The checked form re-runs every chunk from its exact start: no
+swings); method 3 is the peak-chunk method with the LDS-staged tabulation
+that the 53 moment sums of a solve take.  The checked form re-runs every
+chunk from its exact start: no
 re-run misses (the certificates hold), and the counters show the tables
 carrying the sum (few chunks added term by term, few groups walked chunk by
 chunk)."""
@@ -16,7 +18,7 @@ import seqsum_model as M
 pytestmark = pytest.mark.gpu
 
 
-@pytest.mark.parametrize("method", [0, 1, 2])
+@pytest.mark.parametrize("method", [0, 1, 2, 3])
 @pytest.mark.parametrize("name", list(M.hard_cases().keys()))
 def test_sequential_sum_bitwise(name, method):
     print(f"sequential sum {name} method {method}", flush=True)
@@ -37,13 +39,15 @@ def test_peak_chunks_carry_the_sum(name):
     assert st["chunks_added"] <= 0.02 * st["chunks"]
 
 
-def test_peak_chunks_on_many_blocks():
-    """Four million terms of column swings (1024 chunks) against the CPU loop."""
+@pytest.mark.parametrize("method", [2, 3])
+def test_peak_chunks_on_many_blocks(method):
+    """Four million terms of column swings (1024 chunks) against the CPU loop,
+    checked (method 2) and with the LDS-staged tabulation (method 3)."""
     rng = np.random.default_rng(3)
     n = 1 << 22
     col = np.tile(np.linspace(-1.0, 1.0, 256), n // 256)
     x = col * (1.0 + 0.01 * rng.standard_normal(n)) * 1e-13 + 1e-19
-    got, st = H.sequential_sum(x, 2, stats=True)
+    got, st = H.sequential_sum(x, method, stats=True)
     want = 0.0
     for v in np.split(x, 64):
         want = M.plain_sum(np.concatenate([[want], v]))
