@@ -347,22 +347,24 @@ __global__ void __launch_bounds__(256) accel2d_kernel(GridDims g, const double* 
 // --------------------------------------------------------- acceleration --
 // gravity_solver_3d.c:820-897.  The sequential x, y, z boundary copies leave
 // every boundary cell with the interior value at the clamped index.
-__global__ void acceleration_kernel(GridDims g, const double* __restrict__ pr,
-                                    const double* __restrict__ pb, double* __restrict__ gx,
-                                    double* __restrict__ gy, double* __restrict__ gz) {
-    const long n = (long)g.nx * g.ny * g.nz;
-    for (long t = (long)blockIdx.x * blockDim.x + threadIdx.x; t < n;
-         t += (long)gridDim.x * blockDim.x) {
-        const int k = (int)(t % g.nz);
-        const long ij = t / g.nz;
-        const int j = (int)(ij % g.ny), i = (int)(ij / g.ny);
-        const int a = min(max(i, 1), g.nx - 2);
-        const int b = min(max(j, 1), g.ny - 2);
-        const int c = min(max(k, 1), g.nz - 2);
-        gx[t] = -(phi_at(g, pr, pb, a + 1, b, c) - phi_at(g, pr, pb, a - 1, b, c)) / (2.0 * g.dx);
-        gy[t] = -(phi_at(g, pr, pb, a, b + 1, c) - phi_at(g, pr, pb, a, b - 1, c)) / (2.0 * g.dy);
-        gz[t] = -(phi_at(g, pr, pb, a, b, c + 1) - phi_at(g, pr, pb, a, b, c - 1)) / (2.0 * g.dz);
-    }
+// One thread per cell: k along the 64 lanes of a row, 4 rows (j) per block,
+// one plane (i) per blockIdx.z, so no index needs an integer division.
+__global__ void __launch_bounds__(256) acceleration_kernel(GridDims g, const double* __restrict__ pr,
+                                                           const double* __restrict__ pb,
+                                                           double* __restrict__ gx,
+                                                           double* __restrict__ gy,
+                                                           double* __restrict__ gz) {
+    const int k = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int i = blockIdx.z;
+    if (k >= g.nz || j >= g.ny) return;
+    const long t = ((long)i * g.ny + j) * g.nz + k;
+    const int a = min(max(i, 1), g.nx - 2);
+    const int b = min(max(j, 1), g.ny - 2);
+    const int c = min(max(k, 1), g.nz - 2);
+    gx[t] = -(phi_at(g, pr, pb, a + 1, b, c) - phi_at(g, pr, pb, a - 1, b, c)) / (2.0 * g.dx);
+    gy[t] = -(phi_at(g, pr, pb, a, b + 1, c) - phi_at(g, pr, pb, a, b - 1, c)) / (2.0 * g.dy);
+    gz[t] = -(phi_at(g, pr, pb, a, b, c + 1) - phi_at(g, pr, pb, a, b, c - 1)) / (2.0 * g.dz);
 }
 
 // -------------------------------------------------------------- moments --
@@ -1020,9 +1022,9 @@ hipError_t launch_sor_reset(SorCtl* ctl, hipStream_t st) {
 
 hipError_t launch_acceleration(const GridDims& g, const double* phi_r, const double* phi_b,
                                double* gx, double* gy, double* gz, hipStream_t st) {
-    const long n = (long)g.nx * g.ny * g.nz;
-    hipLaunchKernelGGL(acceleration_kernel, dim3(grid_for(n)), dim3(256), 0, st, g, phi_r, phi_b,
-                       gx, gy, gz);
+    hipLaunchKernelGGL(acceleration_kernel,
+                       dim3((unsigned)((g.nz + 63) / 64), (unsigned)((g.ny + 3) / 4), (unsigned)g.nx),
+                       dim3(256), 0, st, g, phi_r, phi_b, gx, gy, gz);
     return hipGetLastError();
 }
 

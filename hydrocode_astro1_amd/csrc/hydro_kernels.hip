@@ -142,6 +142,7 @@ typedef double FzHalo[2][5][TJ];
 // One segment of a block's work: tile (kt, jt), planes ib .. ie-1.  smax and
 // nan_speed accumulate the signal speed of the cell rows' updated interior
 // over the segments of the block.
+template <bool GRAV>
 __device__ __forceinline__ void march_segment(const HydroStepArgs& a, PlaneBufs& s_cell,
                                               FyRows& s_fy, FzHalo& s_fzl, int kt, int jt,
                                               int ib, int ie, int wave, int lane, double dt,
@@ -257,7 +258,7 @@ __device__ __forceinline__ void march_segment(const HydroStepArgs& a, PlaneBufs&
                     floor_roundtrip(q[0], q[1], q[2], q[3], q[4], kc);
                 }
                 const long idx = cidx + off;
-                if (a.gravity) {
+                if (GRAV) {
                     // apply_gravity_source_terms, hydro_solver_3d.c:606-627
                     const double gx = a.g[0][idx], gy = a.g[1][idx], gz = a.g[2][idx];
                     const double rho = q[0];
@@ -386,6 +387,7 @@ __device__ __forceinline__ void march_segment(const HydroStepArgs& a, PlaneBufs&
 // as CUs every CU gets the same number of planes in a single round.
 // Consecutive blocks land on different XCDs; they are regrouped so that each
 // XCD walks a contiguous run of tiles (shared halos stay in its L2).
+template <bool GRAV>
 __global__ void __launch_bounds__(NT) hydro_step_kernel(HydroStepArgs a) {
     __shared__ PlaneBufs s_cell;
     __shared__ FyRows s_fy;
@@ -417,7 +419,7 @@ __global__ void __launch_bounds__(NT) hydro_step_kernel(HydroStepArgs a) {
         const int t = (int)(f / nx);
         const int ib = (int)(f - (long)t * nx);
         const int ie = (int)min((long)nx, ib + (fend - f));
-        march_segment(a, s_cell, s_fy, s_fzl, t % a.ntk, t / a.ntk, ib, ie, wave, lane, dt, kc,
+        march_segment<GRAV>(a, s_cell, s_fy, s_fzl, t % a.ntk, t / a.ntk, ib, ie, wave, lane, dt, kc,
                       smax, nan_speed);
         f += ie - ib;
         // the low halo wave may still read the last plane buffer
@@ -658,7 +660,12 @@ hipError_t launch_hydro_step(const HydroStepArgs& a, hipStream_t st) {
     b.ntj = ntj; b.ntk = ntk;
     b.wpb = a.cx > 0 ? a.cx : choose_wpb(total);
     const long nblk = (total + b.wpb - 1) / b.wpb;
-    hipLaunchKernelGGL(hydro_step_kernel, dim3((unsigned)nblk), dim3(NT), 0, st, b);
+    // the instantiation without gravity carries no source-term code: 166
+    // VGPRs and no spill, against 168 and three spilled for the other
+    if (b.gravity)
+        hipLaunchKernelGGL(hydro_step_kernel<true>, dim3((unsigned)nblk), dim3(NT), 0, st, b);
+    else
+        hipLaunchKernelGGL(hydro_step_kernel<false>, dim3((unsigned)nblk), dim3(NT), 0, st, b);
     return hipGetLastError();
 }
 
