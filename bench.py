@@ -343,14 +343,18 @@ def selfgravity_block(H, ics, n, steps=10, arms=None):
     within 1e-11 relative L1 of the reference's run after 200 steps).
     "nn_fp32_dc_multigrid": relative
     residual 1e-8, the learned guess (kept only when its residual beats the
-    warm start) and FP32 defect correction with multigrid inner solves.  The
-    first step solves cold, the next `steps` warm-start from the previous
-    potential.  `arms` picks a subset by name."""
+    warm start) and FP32 defect correction with multigrid inner solves.
+    "fp32_dc_multigrid": the same without a network.
+    "nn_correction_fp32_dc_multigrid": the same with the start correction
+    only (the warm start plus the network's correction of it, kept only when
+    its residual beats the warm start's).  The first step solves cold, the
+    next `steps` warm-start from the previous potential.  `arms` picks a
+    subset by name."""
     from hydrocode_astro1_amd import nn
 
     U = ics.explosion_n3(n)
 
-    def run(setup, steps=steps):
+    def run(name, setup, steps=steps):
         s = H.Solver(n, n, n, G=1.0, gamma=5.0 / 3.0, cfl=0.4, bc=(H.BC_OUTFLOW,) * 6)
         gr = s.gravity
         gr.set_lmax(6)
@@ -360,17 +364,25 @@ def selfgravity_block(H, ics, n, steps=10, arms=None):
         s.step()
         cold_ms = 1e3 * (time.perf_counter() - t0)
         cold_it = gr.diagnostics()[0]
-        its, ms = [], []
+        its, ms, starts = [], [], []
         for _ in range(steps):
             t0 = time.perf_counter()
             s.step()  # returns after reading dt back: the step is complete
             ms.append(1e3 * (time.perf_counter() - t0))
             its.append(gr.diagnostics()[0])
+            starts.append(gr.s.last_guess)
+            if len(its) % 20 == 0:
+                print(f"[selfgravity] {name}: {len(its)}/{steps} warm steps", file=sys.stderr,
+                      flush=True)
         gr.set_nn_model(None)
+        gr.set_nn_correction(None)
         s.close()
         info.update({"first_step": {"ms": cold_ms, "iterations": cold_it},
                      "warm_ms_per_step": sum(ms) / len(ms), "warm_ms": ms,
                      "warm_iterations": its})
+        if any(v >= 0 for v in starts):
+            # 0 warm start, 1 potential guess, 2 corrected warm start
+            info["warm_start_used"] = starts
         return info
 
     def reference(gr):
@@ -393,13 +405,44 @@ def selfgravity_block(H, ics, n, steps=10, arms=None):
             gr.set_nn_model(nn.Model(path=nn.DEFAULT_WEIGHTS))
         return {"tol_rel_residual": 1e-8, "iterations_are": "V-cycles"}
 
+    def multigrid(gr):
+        gr.set_params(1.5, 1000, 1e-8)
+        gr.set_criterion(H.SOR_CRIT_REL_RESID)
+        gr.set_mode(H.POISSON_FP32_DC)
+        gr.set_dc_inner(H.DC_INNER_MULTIGRID)
+        return {"tol_rel_residual": 1e-8, "iterations_are": "V-cycles"}
+
+    def corrected_multigrid(gr):
+        info = multigrid(gr)
+        gr.set_mode(H.POISSON_NN_FP32_DC)
+        if os.path.exists(nn.CORRECTION_WEIGHTS):
+            gr.set_nn_correction(nn.Model(path=nn.CORRECTION_WEIGHTS))
+            info["correction_weights"] = os.path.relpath(nn.CORRECTION_WEIGHTS, ROOT)
+        return info
+
     table = {"reference_sor": (reference, steps),
              "reference_sor_tree_moments": (reference_tree, steps),
-             "nn_fp32_dc_multigrid": (learned_multigrid, steps)}
+             "nn_fp32_dc_multigrid": (learned_multigrid, steps),
+             "fp32_dc_multigrid": (multigrid, steps),
+             "nn_correction_fp32_dc_multigrid": (corrected_multigrid, steps)}
+    # SOR inner sweeps (about a second per solve at 256^3): only when named
+    def sor_inner(setup):
+        def f(gr):
+            info = setup(gr)
+            gr.set_dc_inner(H.DC_INNER_SOR)
+            info["iterations_are"] = "SOR sweeps"
+            return info
+        return f
+
+    named_only = {"fp32_dc": (sor_inner(multigrid), steps),
+                  "nn_correction_fp32_dc": (sor_inner(corrected_multigrid), steps)}
     out = {"grid": n, "multipole_lmax": 6, "warm_steps": steps}
     for name, (setup, k) in table.items():
         if arms is None or name in arms:
-            out[name] = run(setup, k)
+            out[name] = run(name, setup, k)
+    for name, (setup, k) in named_only.items():
+        if arms is not None and name in arms:
+            out[name] = run(name, setup, k)
     return out
 
 

@@ -58,6 +58,7 @@ class GravitySolver3D(C.Structure):
         ("nn_model", C.c_void_p), ("last_guess", C.c_int),
         ("last_guess_rel_residual", C.c_double), ("last_warm_rel_residual", C.c_double),
         ("dc_inner", C.c_int), ("moment_order", C.c_int),
+        ("nn_correction", C.c_void_p), ("last_corr_rel_residual", C.c_double),
     ]
 
 
@@ -139,6 +140,7 @@ EXPORTS = [
     ("gravity_moments_noalloc", C.c_int,
      [C.c_int] * 3 + [C.c_double] * 6 + [C.c_int, C.c_void_p, C.c_void_p]),
     ("gravity_set_nn_model", C.c_int, [_GP, C.c_void_p]),
+    ("gravity_set_nn_correction", C.c_int, [_GP, C.c_void_p]),
     ("nn_model_create", C.c_void_p, [C.c_ulonglong]),
     ("nn_model_destroy", None, [C.c_void_p]),
     ("nn_model_num_params", C.c_long, [C.c_void_p]),
@@ -148,6 +150,8 @@ EXPORTS = [
     ("nn_model_load", C.c_int, [C.c_void_p, C.c_char_p]),
     ("nn_model_get_params", C.c_int, [C.c_void_p, _fp]),
     ("nn_model_set_params", C.c_int, [C.c_void_p, _fp]),
+    ("nn_model_set_kind", C.c_int, [C.c_void_p, C.c_int, C.c_double]),
+    ("nn_model_kind", C.c_int, [C.c_void_p, _dp]),
     ("nn_model_forward_backward", C.c_int,
      [C.c_void_p, _fp, _fp, C.c_int, _dp, C.c_double, _fp, _fp, _fp, _fp]),
     ("nn_dataset_create", C.c_void_p, [C.c_int, C.c_int]),
@@ -155,6 +159,7 @@ EXPORTS = [
     ("nn_dataset_fill", C.c_int,
      [C.c_void_p, C.c_int, _dp, C.c_int, _dp, C.c_int, C.c_double, C.c_double, _GP]),
     ("nn_dataset_get", C.c_int, [C.c_void_p, C.c_int, _fp, _fp, _dp]),
+    ("nn_dataset_set", C.c_int, [C.c_void_p, C.c_int, _fp, _fp]),
     ("nn_train", C.c_int, [C.c_void_p, C.c_void_p, _ip, C.c_int, C.c_int, C.c_float, _fp]),
     ("nn_evaluate", C.c_int, [C.c_void_p, C.c_void_p, C.c_int, C.c_int, _fp]),
     ("nn_benchmark", C.c_int, [C.c_void_p, C.c_int, C.c_int, _dp, _dp]),
@@ -283,16 +288,25 @@ class Gravity:
             raise ValueError(f"bad inner solver {inner} for this grid")
 
     def set_moment_order(self, order):
-        """MOMENT_ORDER_TREE (default) or MOMENT_ORDER_SEQUENTIAL, the
-        reference's cell-after-cell sums."""
+        """MOMENT_ORDER_AUTO (default: the reference's cell-after-cell sums
+        under FP64 SOR, the tree order in the FP32 modes), MOMENT_ORDER_TREE
+        or MOMENT_ORDER_SEQUENTIAL."""
         if self.L.gravity_set_moment_order(self.h, order) != 0:
             raise ValueError(f"bad moment order {order}")
 
     def set_nn_model(self, model):
         """Attach an nn.Model for POISSON_NN_FP32_DC (None detaches)."""
+        if self.L.gravity_set_nn_model(self.h, model.h if model is not None else None) != 0:
+            raise ValueError("the network is not a potential guess")
         self._nn = model
-        _check(self.L.gravity_set_nn_model(self.h, model.h if model is not None else None),
-               "gravity_set_nn_model")
+
+    def set_nn_correction(self, model):
+        """Attach a start-correction nn.Model (kind 1) for POISSON_NN_FP32_DC
+        (None detaches)."""
+        if self.L.gravity_set_nn_correction(self.h,
+                                            model.h if model is not None else None) != 0:
+            raise ValueError("the network is not a start correction")
+        self._nn_corr = model
 
     def set_criterion(self, crit):
         if self.L.gravity_set_criterion(self.h, crit) != 0:

@@ -539,39 +539,74 @@ int dc_multigrid(GravitySolver3D* s) {
     return rel <= tol ? 0 : -1;
 }
 
-// Starting potential of POISSON_NN_FP32_DC: the network's guess replaces
-// the warm start (the potential left by the previous solve) only when its
-// relative residual, with this density's boundary values, is smaller.
+// Starting potential of POISSON_NN_FP32_DC.  Candidates, in this order: the
+// warm start (the potential left by the previous solve), the warm start
+// plus the start-correction network's correction of it (computed from its
+// residual, and only while that residual is above the tolerance), and the
+// potential-guess network's guess.  Each is scored by its relative residual
+// with this density's boundary values; the smallest wins, a tie keeps the
+// earlier candidate.
 int choose_guess(GravitySolver3D* s, const double* d_rho) {
     GravDevice* d = s->dev;
     s->last_guess = -1;
     s->last_guess_rel_residual = -1.0;
     s->last_warm_rel_residual = -1.0;
+    s->last_corr_rel_residual = -1.0;
     const bool cubic = s->nx == s->ny && s->ny == s->nz && s->dx == s->dy && s->dy == s->dz;
-    if (!s->nn_model || !cubic || s->bc_type != GRAV_BC_ISOLATED) return 0;
+    if ((!s->nn_model && !s->nn_correction) || !cubic || s->bc_type != GRAV_BC_ISOLATED) return 0;
     if (!d->guess_r && (dev_alloc(&d->guess_r, d->nsplit) || dev_alloc(&d->guess_b, d->nsplit)))
         return -2;
-    if (!d->nat_tmp && dev_alloc(&d->nat_tmp, d->n)) return -2;
-    const double box[4] = {s->xmin, s->ymin, s->zmin, s->xmax - s->xmin};
-    const int rc = nn_guess_device(s->nn_model, d_rho, s->nx, d->mom, s->G, box, d->nat_tmp);
-    if (rc == -1) return 0;
-    if (rc) return -2;
-    double rw, rn;
-    if (apply_boundary(s) || rel_residual_dev(s, nullptr, nullptr, &rw)) return -2;
-    HYD_CHECK(hyd::launch_natural_to_split(d->g, d->nat_tmp, 1.0, d->guess_r, d->guess_b,
-                                           d->stream));
-    std::swap(d->phi_r, d->guess_r);
-    std::swap(d->phi_b, d->guess_b);
-    if (apply_boundary(s) || rel_residual_dev(s, nullptr, nullptr, &rn)) return -2;
-    s->last_warm_rel_residual = rw;
-    s->last_guess_rel_residual = rn;
-    if (rn < rw) {
-        s->last_guess = 1;
-    } else {
-        std::swap(d->phi_r, d->guess_r);
-        std::swap(d->phi_b, d->guess_b);
-        s->last_guess = 0;
+    if (ensure_dc_buffers(d)) return -2;
+    double best;
+    if (apply_boundary(s) || rel_residual_dev(s, d->r_r, d->r_b, &best)) return -2;
+    s->last_warm_rel_residual = best;
+    bool any = false;
+    if (s->nn_correction && best > s->tolerance) {
+        // guess = warm start + correction, the start's boundary values kept
+        const int rc = nn_correction_device(s->nn_correction, d->r_r, d->r_b, s->nx,
+                                            s->xmax - s->xmin, d->phi_r, d->phi_b, d->guess_r,
+                                            d->guess_b);
+        if (rc < -1) return -2;
+        if (rc == 0) {
+            any = true;
+            std::swap(d->phi_r, d->guess_r);
+            std::swap(d->phi_b, d->guess_b);
+            double rc2;
+            if (rel_residual_dev(s, nullptr, nullptr, &rc2)) return -2;
+            s->last_corr_rel_residual = rc2;
+            if (rc2 < best) {
+                best = rc2;
+                s->last_guess = 2;
+            } else {
+                std::swap(d->phi_r, d->guess_r);
+                std::swap(d->phi_b, d->guess_b);
+            }
+        }
     }
+    if (s->nn_model) {
+        if (!d->nat_tmp && dev_alloc(&d->nat_tmp, d->n)) return -2;
+        const double box[4] = {s->xmin, s->ymin, s->zmin, s->xmax - s->xmin};
+        const int rc = nn_guess_device(s->nn_model, d_rho, s->nx, d->mom, s->G, box, d->nat_tmp);
+        if (rc < -1) return -2;
+        if (rc == 0) {
+            any = true;
+            HYD_CHECK(hyd::launch_natural_to_split(d->g, d->nat_tmp, 1.0, d->guess_r, d->guess_b,
+                                                   d->stream));
+            std::swap(d->phi_r, d->guess_r);
+            std::swap(d->phi_b, d->guess_b);
+            double rn;
+            if (apply_boundary(s) || rel_residual_dev(s, nullptr, nullptr, &rn)) return -2;
+            s->last_guess_rel_residual = rn;
+            if (rn < best) {
+                best = rn;
+                s->last_guess = 1;
+            } else {
+                std::swap(d->phi_r, d->guess_r);
+                std::swap(d->phi_b, d->guess_b);
+            }
+        }
+    }
+    if (any && s->last_guess < 0) s->last_guess = 0;
     return 0;
 }
 
@@ -734,6 +769,7 @@ GravitySolver3D* gravity_solver_create(int nx, int ny, int nz, double xmin, doub
 void gravity_solver_destroy(GravitySolver3D* s) {
     if (!s) return;
     if (s->dev && s->dev->owns_nn) nn_model_destroy(s->nn_model);
+    if (s->dev && s->dev->owns_corr) nn_model_destroy(s->nn_correction);
     free(s->phi);
     free(s->gx);
     free(s->gy);
@@ -842,11 +878,28 @@ int gravity_set_dc_inner(GravitySolver3D* s, int inner) {
 }
 
 int gravity_set_nn_model(GravitySolver3D* s, void* model) {
+    if (model && nn_model_kind(model, nullptr) != 0) {
+        fprintf(stderr, "gravity_set_nn_model: the network is not a potential guess\n");
+        return -1;
+    }
     if (s->dev->owns_nn && s->nn_model != model) {
         nn_model_destroy(s->nn_model);
         s->dev->owns_nn = false;
     }
     s->nn_model = model;
+    return 0;
+}
+
+int gravity_set_nn_correction(GravitySolver3D* s, void* model) {
+    if (model && nn_model_kind(model, nullptr) != 1) {
+        fprintf(stderr, "gravity_set_nn_correction: the network is not a start correction\n");
+        return -1;
+    }
+    if (s->dev->owns_corr && s->nn_correction != model) {
+        nn_model_destroy(s->nn_correction);
+        s->dev->owns_corr = false;
+    }
+    s->nn_correction = model;
     return 0;
 }
 

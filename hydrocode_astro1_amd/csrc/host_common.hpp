@@ -56,6 +56,7 @@ struct GravDevice {
     float *e_r = nullptr, *e_b = nullptr, *r_r = nullptr, *r_b = nullptr;
     double *guess_r = nullptr, *guess_b = nullptr;  // learned guess (split layout)
     bool owns_nn = false;  // the network was created from an input deck
+    bool owns_corr = false;  // the same for the start-correction network
     // multigrid inner solve of the defect correction (DC_INNER_MULTIGRID):
     // pre- and post-smoothing sweeps per level and their SOR factor; V(1,1)
     // with 1.15 reached 1e-8 fastest at 256^3 (5.9 ms against 7.7 for
@@ -111,6 +112,10 @@ struct HydroDevice {
 // the grid does not fit the network.  box = x0, y0, z0, L.
 int nn_guess_device(void *model, const double *d_rho, int n, const double *d_mom, double G,
                     const double box[4], double *d_phi_nat);
+// Start plus the learned correction from its split FP32 residual (rr, rb)
+// into (gr, gb), split layout; -1 when the grid does not fit the network.
+int nn_correction_device(void *model, const float *rr, const float *rb, int n, double L,
+                         const double *pr, const double *pb, double *gr, double *gb);
 
 // Internal interface between the hydro driver and the gravity solver.
 struct GravitySolver3D;

@@ -360,6 +360,19 @@ GodunovHLLC3D* solver3d_create_from_deck(const InputDeck* deck) {
                 nn_model_destroy(m);
             }
         }
+        if (deck->nn_correction[0]) {
+            void* m = nn_model_create(0);
+            if (m && nn_model_load(m, deck->nn_correction) == 0 &&
+                gravity_set_nn_correction(s->gravity_solver, m) == 0) {
+                s->gravity_solver->dev->owns_corr = true;
+            } else {
+                fprintf(stderr,
+                        "Warning: cannot load the start-correction network %s; the Poisson "
+                        "solve does without it.\n",
+                        deck->nn_correction);
+                nn_model_destroy(m);
+            }
+        }
     }
     return s;
 }

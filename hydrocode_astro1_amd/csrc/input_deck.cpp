@@ -61,7 +61,7 @@ const Key kKeys[] = {
     KB(enable_torus), KD(torus_major_radius), KD(torus_minor_radius), KD(torus_density),
     KD(torus_pressure), KD(torus_rotation_velocity),
     KI(max_steps), KD(sor_omega), KI(sor_max_iter), KD(sor_tol), KI(multipole_lmax),
-    KS(nn_guess), KB(output_enable), KS(ic_file), KB(checkpoint),
+    KS(nn_guess), KB(output_enable), KS(ic_file), KB(checkpoint), KS(nn_correction),
 };
 
 const char* bc_name(BCType b) {

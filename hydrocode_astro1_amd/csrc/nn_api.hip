@@ -15,11 +15,19 @@
 // Inputs: normalised log10 density and the softened monopole potential
 // about the centre of mass; output: potential in units of G M / L, as the
 // monopole plus a learned correction.
+//
+// The same network has a second kind, the start correction: its input is
+// the residual r = f - L phi of a warm start averaged onto a grid of at most
+// 64^3 (features r / s and -r / s, s the root mean square of r), and its
+// output the correction e of the start (phi + e solves the problem) on that
+// grid in units of C L^2 s, C a constant stored with the weights and L the
+// box edge; e is interpolated trilinearly to the solver's grid.
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
 
+#include <cmath>
 #include <random>
 #include <vector>
 
@@ -74,6 +82,11 @@ struct Model {
     std::vector<int> coarse_n;
     double* mom_tmp = nullptr;    // [4] moments given by the host (test entry)
     hipStream_t st = nullptr;
+    int kind = 0;                 // 0 potential guess, 1 start correction
+    double cscale = 1.0;          // C of the start correction
+    double* sumsq = nullptr;      // [1] sum of squares of the correction input
+    float* rcoarse = nullptr;     // averaged residual of the start correction
+    int rcoarse_n = 0;
 };
 
 void free_acts(Acts& a) {
@@ -287,6 +300,12 @@ int features(Model& M, const T* rho, int n, const double* mom, const double box[
     return 0;
 }
 
+// features of the start correction from an averaged residual r (n^3 FP32)
+int features_corr(Model& M, const float* r, int n) {
+    HYD_CHECK(corr_features(r, n, M.sumsq, M.acts.x0, M.acts.mono, M.st));
+    return 0;
+}
+
 bool grid_ok(int n) { return n >= 16 && n % 8 == 0; }
 
 }  // namespace nnh
@@ -294,13 +313,22 @@ bool grid_ok(int n) { return n >= 16 && n % 8 == 0; }
 // Training set held on the device: FP32 density, the potential in units of
 // G M / L (natural layout) and the moments (M, centre of mass) per field.
 struct NNDataset {
-    int n = 0, count = 0;
+    int n = 0, count = 0;  // for the start correction: rho = residual, phi = target
     double box[4] = {0.0, 0.0, 0.0, 1.0};  // x0, y0, z0, L
     float *rho = nullptr, *phi = nullptr;
     double* mom = nullptr;    // [count][4]
     double* rho64 = nullptr;  // staging for the solver
     double *blob = nullptr, *mode = nullptr;
 };
+
+namespace nnh {
+// input features of dataset field idx for the model's kind
+int field_features(Model& M, NNDataset* D, int idx) {
+    const long nv = vox(D->n);
+    if (M.kind) return features_corr(M, D->rho + nv * idx, D->n);
+    return features(M, D->rho + nv * idx, D->n, D->mom + 4 * idx, D->box);
+}
+}  // namespace nnh
 
 extern "C" {
 
@@ -339,7 +367,8 @@ void* nn_model_create(unsigned long long seed) {
               hipMalloc((void**)&M->wg_part, sizeof(float) * wgrad_scratch_len()) == hipSuccess &&
               hipMalloc((void**)&M->m, sizeof(float) * off) == hipSuccess &&
               hipMalloc((void**)&M->v, sizeof(float) * off) == hipSuccess &&
-              hipMalloc((void**)&M->mom_tmp, sizeof(double) * 4) == hipSuccess;
+              hipMalloc((void**)&M->mom_tmp, sizeof(double) * 4) == hipSuccess &&
+              hipMalloc((void**)&M->sumsq, sizeof(double) * 4) == hipSuccess;
     for (int l = 0; ok && l < kNL; ++l) {
         const LayerDef& L = kLayers[l];
         ok = hipMalloc((void**)&M->wp[l], sizeof(uint16_t) * packed_len(pad4(L.cin), L.cout)) ==
@@ -363,7 +392,7 @@ void nn_model_destroy(void* h) {
     Model* M = (Model*)h;
     if (!M) return;
     free_acts(M->acts);
-    void* ps[] = {M->params, M->grads, M->m, M->v, M->mom_tmp, M->wg_part};
+    void* ps[] = {M->params, M->grads, M->m, M->v, M->mom_tmp, M->wg_part, M->sumsq, M->rcoarse};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     for (double* p : M->coarse) (void)hipFree(p);
@@ -399,9 +428,15 @@ int nn_model_save(void* h, const char* path) {
         fprintf(stderr, "nn_model_save: cannot open %s\n", path);
         return -1;
     }
+    // HYDNN002: the potential guess; HYDNN003: a network of another kind,
+    // its kind and C after the header
     const int hdr[2] = {nnh::kNL, (int)M->nparam};
-    const bool ok = fwrite("HYDNN002", 1, 8, f) == 8 && fwrite(hdr, sizeof(int), 2, f) == 2 &&
-                    fwrite(p.data(), sizeof(float), M->nparam, f) == (size_t)M->nparam;
+    bool ok = fwrite(M->kind ? "HYDNN003" : "HYDNN002", 1, 8, f) == 8 &&
+              fwrite(hdr, sizeof(int), 2, f) == 2;
+    if (ok && M->kind)
+        ok = fwrite(&M->kind, sizeof(int), 1, f) == 1 &&
+             fwrite(&M->cscale, sizeof(double), 1, f) == 1;
+    ok = ok && fwrite(p.data(), sizeof(float), M->nparam, f) == (size_t)M->nparam;
     fclose(f);
     return ok ? 0 : -1;
 }
@@ -414,12 +449,17 @@ int nn_model_load(void* h, const char* path) {
         return -1;
     }
     char magic[8];
-    int hdr[2];
+    int hdr[2], kind = 0;
+    double cscale = 1.0;
     std::vector<float> p(M->nparam);
-    const bool ok = fread(magic, 1, 8, f) == 8 && memcmp(magic, "HYDNN002", 8) == 0 &&
-                    fread(hdr, sizeof(int), 2, f) == 2 && hdr[0] == nnh::kNL &&
-                    hdr[1] == (int)M->nparam &&
-                    fread(p.data(), sizeof(float), M->nparam, f) == (size_t)M->nparam;
+    bool ok = fread(magic, 1, 8, f) == 8;
+    const bool v3 = ok && memcmp(magic, "HYDNN003", 8) == 0;
+    ok = ok && (v3 || memcmp(magic, "HYDNN002", 8) == 0) &&
+         fread(hdr, sizeof(int), 2, f) == 2 && hdr[0] == nnh::kNL && hdr[1] == (int)M->nparam;
+    if (ok && v3)
+        ok = fread(&kind, sizeof(int), 1, f) == 1 && fread(&cscale, sizeof(double), 1, f) == 1 &&
+             kind == 1 && cscale > 0.0 && std::isfinite(cscale);
+    ok = ok && fread(p.data(), sizeof(float), M->nparam, f) == (size_t)M->nparam;
     fclose(f);
     if (!ok) {
         fprintf(stderr, "nn_model_load: %s is not a weights file of this network\n", path);
@@ -428,7 +468,26 @@ int nn_model_load(void* h, const char* path) {
     HYD_CHECK(hipMemcpy(M->params, p.data(), sizeof(float) * M->nparam, hipMemcpyHostToDevice));
     if (nnh::repack(*M)) return -2;
     HYD_CHECK(hipStreamSynchronize(M->st));
+    M->kind = kind;
+    M->cscale = cscale;
     return 0;
+}
+
+int nn_model_set_kind(void* h, int kind, double cscale) {
+    nnh::Model* M = (nnh::Model*)h;
+    if ((kind != 0 && kind != 1) || !(cscale > 0.0) || !std::isfinite(cscale)) {
+        fprintf(stderr, "nn_model_set_kind: kind must be 0 or 1 and C positive\n");
+        return -1;
+    }
+    M->kind = kind;
+    M->cscale = kind ? cscale : 1.0;
+    return 0;
+}
+
+int nn_model_kind(void* h, double* cscale) {
+    nnh::Model* M = (nnh::Model*)h;
+    if (cscale) *cscale = M->cscale;
+    return M->kind;
 }
 
 int nn_model_get_params(void* h, float* out) {
@@ -473,7 +532,11 @@ int nn_model_forward_backward(void* h, const float* rho, const float* target, in
         hipMemsetAsync(M->grads, 0, sizeof(float) * M->nparam, M->st) ||
         hipMemsetAsync(M->acts.loss, 0, sizeof(float), M->st))
         rc = -2;
-    if (!rc) rc = features(*M, drho, n, M->mom_tmp, box) || forward(*M, n) ? -2 : 0;
+    if (!rc)
+        rc = (M->kind ? features_corr(*M, drho, n) : features(*M, drho, n, M->mom_tmp, box)) ||
+                     forward(*M, n)
+                 ? -2
+                 : 0;
     if (!rc && mse_loss(M->acts.out, dtg, nv, 1.0f / nv, M->acts.gout, M->acts.loss, M->st))
         rc = -2;
     if (!rc && backward(*M, n)) rc = -2;
@@ -578,6 +641,23 @@ int nn_dataset_fill(void* h, int index, const double* blob, int nblob, const dou
     return status;
 }
 
+// Field `index` from host arrays (n^3 FP32 each): the network input and its
+// target.  For the start correction the input is the averaged residual and
+// the target the correction in units of C L^2 s.
+int nn_dataset_set(void* h, int index, const float* input, const float* target) {
+    NNDataset* D = (NNDataset*)h;
+    const long nv = nnh::vox(D->n);
+    if (index < 0 || index >= D->count || !input || !target) {
+        fprintf(stderr, "nn_dataset_set: bad arguments\n");
+        return -1;
+    }
+    const double mom4[4] = {0.0, 0.0, 0.0, 0.0};
+    HYD_CHECK(hipMemcpy(D->rho + nv * index, input, sizeof(float) * nv, hipMemcpyHostToDevice));
+    HYD_CHECK(hipMemcpy(D->phi + nv * index, target, sizeof(float) * nv, hipMemcpyHostToDevice));
+    HYD_CHECK(hipMemcpy(D->mom + 4 * index, mom4, sizeof(mom4), hipMemcpyHostToDevice));
+    return 0;
+}
+
 int nn_dataset_get(void* h, int index, float* rho, float* phi, double* mom4) {
     NNDataset* D = (NNDataset*)h;
     const long nv = nnh::vox(D->n);
@@ -611,8 +691,7 @@ int nn_train(void* hm, void* hd, const int* order, int steps, int batch, float l
         HYD_CHECK(hipMemsetAsync(M->acts.loss, 0, sizeof(float), M->st));
         for (int b = 0; b < batch; ++b) {
             const int idx = order[s * batch + b];
-            if (features(*M, D->rho + nv * idx, n, D->mom + 4 * idx, D->box) || forward(*M, n))
-                return -2;
+            if (field_features(*M, D, idx) || forward(*M, n)) return -2;
             HYD_CHECK(mse_loss(M->acts.out, D->phi + nv * idx, nv, 1.0f / ((float)nv * batch),
                                M->acts.gout, M->acts.loss, M->st));
             if (backward(*M, n)) return -2;
@@ -643,7 +722,7 @@ int nn_evaluate(void* hm, void* hd, int first, int count, float* mse) {
     if (alloc_acts(M->acts, n)) return -2;
     HYD_CHECK(hipMemsetAsync(M->acts.loss, 0, sizeof(float), M->st));
     for (int i = first; i < first + count; ++i) {
-        if (features(*M, D->rho + nv * i, n, D->mom + 4 * i, D->box) || forward(*M, n)) return -2;
+        if (field_features(*M, D, i) || forward(*M, n)) return -2;
         HYD_CHECK(mse_loss(M->acts.out, D->phi + nv * i, nv, 1.0f / ((float)nv * count),
                            M->acts.gout, M->acts.loss, M->st));
     }
@@ -750,5 +829,32 @@ int nn_guess_device(void* hm, const double* d_rho, int n, const double* d_mom, d
     if (alloc_acts(M->acts, nc)) return -2;
     if (features(*M, cur, nc, d_mom, box) || forward(*M, nc)) return -2;
     HYD_CHECK(prolong_to_f64(M->acts.out, nc, n, d_mom, G, box[3], d_phi_nat, M->st));
+    return 0;
+}
+
+// Corrected start of a cubic n^3 solve with box edge L: the split FP32
+// residual (rr, rb) of the start (pr, pb) is averaged onto nc^3 (nc = n
+// halved until at most 64), the start-correction network runs there, and
+// guess = start + C L^2 s P(out) on the interior, the start on the boundary
+// (split layout, FP64).  Returns -1 when the grid does not fit the network.
+int nn_correction_device(void* hm, const float* rr, const float* rb, int n, double L,
+                         const double* pr, const double* pb, double* gr, double* gb) {
+    using namespace nnh;
+    Model* M = (Model*)hm;
+    if (M->kind != 1 || n % 2 != 0) return -1;
+    int nc = n;
+    while (nc > 64 && nc % 2 == 0) nc /= 2;
+    if (nc > 64 || !grid_ok(nc)) return -1;
+    if (M->rcoarse_n != nc) {
+        if (M->rcoarse) (void)hipFree(M->rcoarse);
+        M->rcoarse = nullptr;
+        M->rcoarse_n = 0;
+        HYD_CHECK(hipMalloc((void**)&M->rcoarse, sizeof(float) * vox(nc)));
+        M->rcoarse_n = nc;
+    }
+    if (alloc_acts(M->acts, nc)) return -2;
+    HYD_CHECK(corr_restrict(rr, rb, n, n / nc, M->rcoarse, M->st));
+    if (features_corr(*M, M->rcoarse, nc) || forward(*M, nc)) return -2;
+    HYD_CHECK(corr_apply(M->acts.out, nc, n, M->sumsq, M->cscale * L * L, pr, pb, gr, gb, M->st));
     return 0;
 }

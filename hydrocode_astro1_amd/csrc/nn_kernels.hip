@@ -713,6 +713,112 @@ __global__ void prolong_kernel(const float* __restrict__ c, int nc, int n,
     }
 }
 
+// ------------------------------------------------- start correction --
+// The correction network reads the warm start's residual averaged over
+// f^3 blocks (nc = n / f points per direction) and predicts the correction
+// there; the correction is interpolated back and added to the start.
+
+// r (FP32, red-black split layout of the n^3 grid, n/2 entries per row and
+// colour) averaged over f^3 blocks into a natural nc^3 array; each block is
+// summed in a fixed order.
+__global__ void corr_restrict_kernel(const float* __restrict__ rr, const float* __restrict__ rb,
+                                     int n, int f, float* __restrict__ out) {
+    const int nc = n / f, nzh = n / 2;
+    const long total = (long)nc * nc * nc;
+    const float w = 1.0f / (float)(f * f * f);
+    for (long e = (long)blockIdx.x * blockDim.x + threadIdx.x; e < total;
+         e += (long)gridDim.x * blockDim.x) {
+        const int K = (int)(e % nc), J = (int)((e / nc) % nc), I = (int)(e / ((long)nc * nc));
+        float s = 0.0f;
+        for (int a = 0; a < f; ++a)
+            for (int b = 0; b < f; ++b)
+                for (int c = 0; c < f; ++c) {
+                    const int i = f * I + a, j = f * J + b, k = f * K + c;
+                    const float* src = ((i + j + k) & 1) ? rb : rr;
+                    s += src[((long)i * n + j) * nzh + (k >> 1)];
+                }
+        out[e] = s * w;
+    }
+}
+
+// sum of x^2 over count values by one workgroup, in a fixed order
+__global__ void __launch_bounds__(1024) corr_sumsq_kernel(const float* __restrict__ x, long count,
+                                                          double* __restrict__ out) {
+    __shared__ double part[1024];
+    double s = 0.0;
+    for (long v = threadIdx.x; v < count; v += 1024) {
+        const double a = (double)x[v];
+        s += a * a;
+    }
+    part[threadIdx.x] = s;
+    __syncthreads();
+    for (int o = 512; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = part[0];
+}
+
+// Features of the correction network (cp = 4 layout): channel 0 = r / s,
+// channel 1 = -r / s, s the root mean square of r (r all zero: zeros); no
+// monopole term.
+__global__ void corr_features_kernel(const float* __restrict__ r, long nvox,
+                                     const double* __restrict__ sumsq, uint16_t* __restrict__ x,
+                                     float* __restrict__ mono) {
+    const double ms = sumsq[0] / (double)nvox;
+    const float inv = ms > 0.0 ? (float)(1.0 / sqrt(ms)) : 0.0f;
+    for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < nvox;
+         v += (long)gridDim.x * blockDim.x) {
+        const float a = r[v] * inv;
+        x[4 * v] = f2bf(a);
+        x[4 * v + 1] = f2bf(-a);
+        x[4 * v + 2] = 0;
+        x[4 * v + 3] = 0;
+        mono[v] = 0.0f;
+    }
+}
+
+// guess = phi + scale * s * P(e) at the interior cells of the split n^3
+// grid and phi at its boundary cells: e the network output on the nc^3
+// grid, P trilinear interpolation (edges clamped, as prolong_kernel), s the
+// root mean square of the restricted residual.  One thread per split entry
+// (i, j, m), both colours.
+__global__ void corr_apply_kernel(const float* __restrict__ e, int nc, int n,
+                                  const double* __restrict__ sumsq, double scale,
+                                  const double* __restrict__ pr, const double* __restrict__ pb,
+                                  double* __restrict__ gr, double* __restrict__ gb) {
+    const int nzh = n / 2, f = n / nc;
+    const int m = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
+    const int i = blockIdx.z;
+    if (m >= nzh || j >= n) return;
+    const double sc = scale * sqrt(sumsq[0] / ((double)nc * nc * nc));
+    const long base = ((long)i * n + j) * nzh + m;
+    int i0, i1, j0, j1;
+    float wi, wj;
+    lin_weights(i, f, nc, i0, i1, wi);
+    lin_weights(j, f, nc, j0, j1, wj);
+#pragma unroll
+    for (int colour = 0; colour < 2; ++colour) {
+        const int k = 2 * m + ((i + j + colour) & 1);
+        const double p = (colour ? pb : pr)[base];
+        double v = p;
+        if (i >= 1 && i <= n - 2 && j >= 1 && j <= n - 2 && k >= 1 && k <= n - 2) {
+            int k0, k1;
+            float wk;
+            lin_weights(k, f, nc, k0, k1, wk);
+            auto at = [&](int a, int b, int d) { return e[((long)a * nc + b) * nc + d]; };
+            const float c00 = at(i0, j0, k0) * (1 - wk) + at(i0, j0, k1) * wk;
+            const float c01 = at(i0, j1, k0) * (1 - wk) + at(i0, j1, k1) * wk;
+            const float c10 = at(i1, j0, k0) * (1 - wk) + at(i1, j0, k1) * wk;
+            const float c11 = at(i1, j1, k0) * (1 - wk) + at(i1, j1, k1) * wk;
+            const float c0 = c00 * (1 - wj) + c01 * wj, c1 = c10 * (1 - wj) + c11 * wj;
+            v = p + sc * (double)(c0 * (1 - wi) + c1 * wi);
+        }
+        (colour ? gb : gr)[base] = v;
+    }
+}
+
 // ------------------------------------------------------------ launchers --
 static unsigned grid_n(long n) {
     long b = (n + 255) / 256;
@@ -880,6 +986,29 @@ hipError_t scale_potential(const double* phi, long count, const double* mom, dou
                            float* out, hipStream_t st) {
     hipLaunchKernelGGL(scale_f64_to_f32_kernel, dim3(grid_n(count)), dim3(256), 0, st, phi, count,
                        mom, G, L, out);
+    return hipGetLastError();
+}
+
+hipError_t corr_restrict(const float* rr, const float* rb, int n, int f, float* out,
+                         hipStream_t st) {
+    const long total = (long)(n / f) * (n / f) * (n / f);
+    hipLaunchKernelGGL(corr_restrict_kernel, dim3(grid_n(total)), dim3(256), 0, st, rr, rb, n, f,
+                       out);
+    return hipGetLastError();
+}
+hipError_t corr_features(const float* r, int nc, double* sumsq, uint16_t* x, float* mono,
+                         hipStream_t st) {
+    const long nv = (long)nc * nc * nc;
+    hipLaunchKernelGGL(corr_sumsq_kernel, dim3(1), dim3(1024), 0, st, r, nv, sumsq);
+    hipLaunchKernelGGL(corr_features_kernel, dim3(grid_n(nv)), dim3(256), 0, st, r, nv, sumsq, x,
+                       mono);
+    return hipGetLastError();
+}
+hipError_t corr_apply(const float* e, int nc, int n, const double* sumsq, double scale,
+                      const double* pr, const double* pb, double* gr, double* gb, hipStream_t st) {
+    hipLaunchKernelGGL(corr_apply_kernel,
+                       dim3((unsigned)((n / 2 + 63) / 64), (unsigned)((n + 3) / 4), (unsigned)n),
+                       dim3(256), 0, st, e, nc, n, sumsq, scale, pr, pb, gr, gb);
     return hipGetLastError();
 }
 

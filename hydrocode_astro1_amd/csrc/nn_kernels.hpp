@@ -107,5 +107,17 @@ hipError_t blob_field(int n, const double* blob, int nblob, const double* mode, 
 hipError_t scale_potential(const double* phi, long count, const double* mom, double G, double L,
                            float* out, hipStream_t st);
 
+// Correction of a warm start (the correction network): the split FP32
+// residual of an n^3 grid averaged over f^3 blocks into nc^3 = (n/f)^3;
+// the features (r / s, -r / s) with s the root mean square (sumsq[0] = sum
+// of r^2, fixed order; mono zeroed); and guess = phi + scale * s * P(e) on
+// the interior (phi on the boundary), P trilinear from nc^3 to n^3.
+hipError_t corr_restrict(const float* rr, const float* rb, int n, int f, float* out,
+                         hipStream_t st);
+hipError_t corr_features(const float* r, int nc, double* sumsq, uint16_t* x, float* mono,
+                         hipStream_t st);
+hipError_t corr_apply(const float* e, int nc, int n, const double* sumsq, double scale,
+                      const double* pr, const double* pb, double* gr, double* gb, hipStream_t st);
+
 }  // namespace nn
 }  // namespace hyd

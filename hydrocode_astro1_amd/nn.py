@@ -16,6 +16,18 @@ lognormal k^-2 turbulence factor; each potential is this library's own
 isolated solve to a relative residual of 1e-8.
 
     python -m hydrocode_astro1_amd.nn --fields 2000 --epochs 4 --out weights.bin
+
+The same network has a second kind, the start correction (kind 1).  Its
+input is the residual r = f - L phi of a warm start (the previous step's
+potential with the new density's boundary values) averaged onto 64^3, its
+output the correction e of the start, in units of C L^2 s (s the root mean
+square of the averaged r, L the box edge, C a constant stored with the
+weights).  Training pairs come from explosion runs (the n=3 polytrope of
+ics.explosion_n3, seeds 1-8 at 128^3 and 21-22 at 256^3) whose every step is
+solved to a relative residual of 1e-11; the held-out seed 12345 is the
+stellar-explosion workload.
+
+    python -m hydrocode_astro1_amd.nn --kind correction --out corr.bin
 """
 import argparse
 import ctypes as C
@@ -35,6 +47,9 @@ _ip = C.POINTER(C.c_int)
 
 DEFAULT_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
                                "nn_guess_128.bin")
+CORRECTION_WEIGHTS = os.path.join(os.path.dirname(os.path.abspath(__file__)), "data",
+                                  "nn_correction_64.bin")
+KIND_GUESS, KIND_CORRECTION = 0, 1
 # (cin, cout, nearest-upsampled input) of the 12 convolutions, in order
 LAYERS = [(2, 8, 0), (8, 8, 0), (8, 12, 0), (12, 12, 0), (12, 16, 0), (16, 16, 0),
           (16, 24, 0), (24, 24, 0), (24, 16, 1), (16, 12, 1), (12, 8, 1), (8, 1, 0)]
@@ -88,6 +103,17 @@ class Model:
     def save(self, path):
         if self.L.nn_model_save(self.h, os.fsencode(path)) != 0:
             raise RuntimeError(f"cannot save network weights to {path}")
+
+    def set_kind(self, kind, cscale=1.0):
+        """KIND_GUESS, or KIND_CORRECTION with its output scale C."""
+        if self.L.nn_model_set_kind(self.h, int(kind), float(cscale)) != 0:
+            raise ValueError(f"bad network kind {kind} or scale {cscale}")
+
+    def kind(self):
+        """(kind, C)."""
+        c = C.c_double(0.0)
+        k = self.L.nn_model_kind(self.h, C.byref(c))
+        return k, c.value
 
     def get_params(self):
         p = np.zeros(self.nparam, dtype=np.float32)
@@ -156,6 +182,13 @@ class Dataset:
         return _check(self.L.nn_dataset_fill(self.h, index, _d(blob), blob.size // 6, _d(mode),
                                              mode.size // 5, float(floor_rho), float(sigma),
                                              grav.h), "nn_dataset_fill")
+
+    def set(self, index, inp, target):
+        """Field `index` from host arrays: the network input and its target."""
+        inp = np.ascontiguousarray(inp, dtype=np.float32)
+        target = np.ascontiguousarray(target, dtype=np.float32)
+        assert inp.shape == target.shape == (self.n,) * 3
+        _check(self.L.nn_dataset_set(self.h, index, _f(inp), _f(target)), "nn_dataset_set")
 
     def get(self, index):
         n = self.n
@@ -256,21 +289,148 @@ def train(model, ds, epochs=4, batch=4, lr=2e-3, val=0, seed=0, log=print, decay
     return losses, vmse
 
 
+# ---- start correction ------------------------------------------------------
+def average(a, f):
+    """Mean over f^3 blocks (n^3 -> (n/f)^3)."""
+    m = a.shape[0] // f
+    return a.reshape(m, f, m, f, m, f).mean(axis=(1, 3, 5))
+
+
+def zero_boundary(e):
+    e[0] = e[-1] = 0.0
+    e[:, 0] = e[:, -1] = 0.0
+    e[:, :, 0] = e[:, :, -1] = 0.0
+    return e
+
+
+def laplacian(e, h):
+    """7-point Laplacian on the interior cells (boundary cells 0)."""
+    r = np.zeros_like(e)
+    c = e[1:-1, 1:-1, 1:-1]
+    r[1:-1, 1:-1, 1:-1] = ((e[2:, 1:-1, 1:-1] + e[:-2, 1:-1, 1:-1] - 2.0 * c) +
+                           (e[1:-1, 2:, 1:-1] + e[1:-1, :-2, 1:-1] - 2.0 * c) +
+                           (e[1:-1, 1:-1, 2:] + e[1:-1, 1:-1, :-2] - 2.0 * c)) / (h * h)
+    return r
+
+
+def correction_pairs(n, seed, steps, nc=64, tol=1e-11, log=print):
+    """(averaged residual, averaged correction) pairs of `steps` consecutive
+    self-gravity steps of the explosion at n^3 (unit box, G = 1, outflow
+    boundaries): each step's potential is solved to the relative residual
+    `tol`, e = phi_new - phi_old with the boundary cells zeroed (the start
+    takes the new boundary values), r = L e = f - L phi_old on the interior."""
+    from . import BC_OUTFLOW, Solver, ics
+    U = ics.explosion_n3(n, seed=seed)
+    s = Solver(n, n, n, G=1.0, gamma=5.0 / 3.0, cfl=0.4, bc=(BC_OUTFLOW,) * 6)
+    g = s.gravity
+    g.set_lmax(6)
+    g.set_criterion(SOR_CRIT_REL_RESID)
+    g.set_mode(POISSON_FP32_DC)
+    g.set_dc_inner(DC_INNER_MULTIGRID)
+    g.set_params(1.5, 400, tol)
+    s.set_state(U)
+    del U
+    f = n // nc
+    h = 1.0 / n
+    prev = None
+    R, E = [], []
+    t0 = time.perf_counter()
+    for k in range(steps + 1):
+        s.step()
+        lib().gravity_sync_to_host(g.h)
+        phi = g.phi().copy()
+        if prev is not None:
+            e = zero_boundary(phi - prev)
+            R.append(average(laplacian(e, h), f).astype(np.float32))
+            E.append(average(e, f).astype(np.float32))
+        prev = phi
+        if log and (k + 1) % 40 == 0:
+            log(f"[nn] {n}^3 seed {seed}: step {k + 1}/{steps + 1} "
+                f"({time.perf_counter() - t0:.1f}s)")
+    s.close()
+    return R, E
+
+
+def correction_scale(R, E, box_len=1.0):
+    """C: the median over pairs of rms(e) / (rms(r) L^2)."""
+    sr = np.array([np.sqrt(np.mean(np.square(r, dtype=np.float64))) for r in R])
+    se = np.array([np.sqrt(np.mean(np.square(e, dtype=np.float64))) for e in E])
+    ok = sr > 0
+    return float(np.median(se[ok] / (sr[ok] * box_len * box_len)))
+
+
+def correction_dataset(R, E, cscale, box_len=1.0):
+    """Device dataset of the pairs with pairs of a zero residual dropped:
+    input r, target e / (C L^2 s)."""
+    keep = [i for i in range(len(R)) if np.any(R[i])]
+    ds = Dataset(R[0].shape[0], len(keep))
+    for j, i in enumerate(keep):
+        s = math.sqrt(float(np.mean(np.square(R[i], dtype=np.float64))))
+        ds.set(j, R[i], E[i] / (cscale * box_len * box_len * s))
+    return ds
+
+
+def train_correction(runs, epochs=40, batch=8, lr=2e-3, decay=0.92, val_frac=0.1, seed=0,
+                     log=print):
+    """Pairs of the runs [(n, seed, steps)], then the start-correction
+    network trained on them; returns (model, validation MSE)."""
+    R, E = [], []
+    for n, sd, steps in runs:
+        r, e = correction_pairs(n, sd, steps, log=log)
+        R += r
+        E += e
+        if log:
+            log(f"[nn] pairs so far {len(R)}")
+    C_ = correction_scale(R, E)
+    perm = np.random.default_rng(seed).permutation(len(R))
+    R = [R[i] for i in perm]
+    E = [E[i] for i in perm]
+    ds = correction_dataset(R, E, C_)
+    val = max(1, int(round(val_frac * ds.count)))
+    model = Model(seed=seed)
+    model.set_kind(KIND_CORRECTION, C_)
+    if log:
+        log(f"[nn] {ds.count} pairs, C = {C_:.4e}")
+    _, vmse = train(model, ds, epochs, batch, lr, val=val, seed=seed, log=log, decay=decay)
+    ds.close()
+    return model, vmse
+
+
 def main(argv=None):
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--n", type=int, default=128)
     ap.add_argument("--fields", type=int, default=2000)
     ap.add_argument("--val", type=int, default=50)
-    ap.add_argument("--epochs", type=int, default=4)
-    ap.add_argument("--batch", type=int, default=4)
+    ap.add_argument("--epochs", type=int, default=None, help="4 (guess), 40 (correction)")
+    ap.add_argument("--batch", type=int, default=None, help="4 (guess), 8 (correction)")
     ap.add_argument("--lr", type=float, default=2e-3)
-    ap.add_argument("--decay", type=float, default=0.5, help="learning-rate factor per epoch")
+    ap.add_argument("--decay", type=float, default=None,
+                    help="learning-rate factor per epoch: 0.5 (guess), 0.92 (correction)")
     ap.add_argument("--omega", type=float, default=1.9, help="SOR factor of the sor inner solve")
     ap.add_argument("--inner", choices=("auto", "sor", "multigrid"), default="auto",
                     help="inner solve of the training targets' defect correction")
     ap.add_argument("--seed", type=int, default=0)
-    ap.add_argument("--out", default=DEFAULT_WEIGHTS)
+    ap.add_argument("--out", default=None)
+    ap.add_argument("--kind", choices=("guess", "correction"), default="guess")
+    ap.add_argument("--runs", default="128:1:160,128:2:160,128:3:160,128:4:160,128:5:160,"
+                    "128:6:160,128:7:160,128:8:160,256:21:130,256:22:130",
+                    help="start correction: explosion runs n:seed:steps")
     a = ap.parse_args(argv)
+    corr = a.kind == "correction"
+    a.epochs = a.epochs or (40 if corr else 4)
+    a.batch = a.batch or (8 if corr else 4)
+    a.decay = a.decay or (0.92 if corr else 0.5)
+    if corr:
+        runs = [tuple(int(v) for v in r.split(":")) for r in a.runs.split(",") if r]
+        model, vmse = train_correction(runs, epochs=a.epochs, batch=a.batch, lr=a.lr,
+                                       decay=a.decay, seed=a.seed)
+        out = a.out or CORRECTION_WEIGHTS
+        os.makedirs(os.path.dirname(os.path.abspath(out)), exist_ok=True)
+        model.save(out)
+        print(f"[nn] start correction: validation MSE {vmse}, C = {model.kind()[1]:.4e}, "
+              f"saved {out}", file=sys.stderr)
+        return
+    a.out = a.out or DEFAULT_WEIGHTS
     inner = {"auto": None, "sor": DC_INNER_SOR, "multigrid": DC_INNER_MULTIGRID}[a.inner]
     ds = build_dataset(a.n, a.fields, omega=a.omega, inner=inner)
     model = Model(seed=a.seed)

@@ -64,13 +64,17 @@ typedef struct GravitySolver3D {
     double last_solve_ms;     /* device time of the last potential solve     */
     void *nn_model;      /* learned-guess network (nn_model_create), or NULL */
     int last_guess;      /* POISSON_NN_FP32_DC: 1 network guess used, 0 warm
-                            start kept, -1 no guess possible              */
+                            start kept, -1 no guess possible (2 below)    */
     double last_guess_rel_residual; /* residual of the network guess         */
     double last_warm_rel_residual;  /* residual of the warm start            */
     int dc_inner;        /* DcInner from input_deck.h: inner solver of the FP32
                             modes; with DC_INNER_MULTIGRID, last_iterations
                             counts V-cycles                               */
     int moment_order;    /* MomentOrder from input_deck.h                    */
+    void *nn_correction; /* start-correction network (nn_model_set_kind 1),
+                            or NULL; with it last_guess 2 = warm start plus
+                            the network's correction used                 */
+    double last_corr_rel_residual; /* residual of the corrected warm start */
 } GravitySolver3D;
 
 /* gravity_solver_3d.h:64-68 */

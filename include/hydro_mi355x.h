@@ -185,7 +185,14 @@ long nn_model_num_params(void *model);
 long nn_model_param_len(void *model);
 long nn_model_param_offset(void *model, int layer, int bias);
 int nn_model_save(void *model, const char *path);
+/* Loads weights and the network's kind and C (HYDNN002 files: kind 0). */
 int nn_model_load(void *model, const char *path);
+/* Kind 0: the potential guess (inputs from the density).  Kind 1: the start
+ * correction; input the residual r = f - L phi of a start averaged onto the
+ * network's grid (features r / s, -r / s, s its root mean square), output the
+ * correction in units of C L^2 s (L the box edge, C = cscale > 0). */
+int nn_model_set_kind(void *model, int kind, double cscale);
+int nn_model_kind(void *model, double *cscale);
 int nn_model_get_params(void *model, float *out);
 int nn_model_set_params(void *model, const float *in);
 /* One forward + backward pass on host data (kernel self-check). */
@@ -202,6 +209,9 @@ void nn_dataset_destroy(void *dataset);
 int nn_dataset_fill(void *dataset, int index, const double *blob, int nblob, const double *mode,
                     int nmode, double floor_rho, double sigma, GravitySolver3D *grav);
 int nn_dataset_get(void *dataset, int index, float *rho, float *phi, double *mom4);
+/* Field `index` from host arrays: the network's input (density for kind 0,
+ * averaged residual for kind 1) and its target (n^3 FP32 each). */
+int nn_dataset_set(void *dataset, int index, const float *input, const float *target);
 /* Adam over mini-batches of dataset fields listed in order[steps * batch]. */
 int nn_train(void *model, void *dataset, const int *order, int steps, int batch, float lr,
              float *loss_log);
@@ -214,6 +224,12 @@ double nn_forward_flops(int n);
 /* Attach a network to a solver for POISSON_NN_FP32_DC (NULL detaches; the
  * caller keeps ownership). */
 int gravity_set_nn_model(GravitySolver3D *solver, void *model);
+/* Attach a start-correction network (kind 1) for POISSON_NN_FP32_DC: the
+ * warm start plus its correction becomes a candidate start beside the warm
+ * start and the potential guess (NULL detaches; the caller keeps ownership).
+ * The grid is averaged by 2 until at most 64^3, so n must be 64 x 2^k or a
+ * multiple of 8 from 16 to 64.  Rejects a network of the other kind. */
+int gravity_set_nn_correction(GravitySolver3D *solver, void *model);
 
 #ifdef __cplusplus
 }

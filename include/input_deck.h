@@ -163,6 +163,8 @@ typedef struct {
     int moment_order;         /* MomentOrder, default AUTO                */
     bool checkpoint;          /* also write HYDCK001 checkpoints at output
                                  times (U, potential, g), default false   */
+    char nn_correction[256];  /* weights file for the start correction of
+                                 POISSON_NN_FP32_DC, ""                   */
 } InputDeck;
 
 /* input_deck.h:127-131 */

@@ -209,24 +209,25 @@ def test_deck_added_keys(tmp_path):
         f.write("nx = 32\nmax_steps = 200\nsor_tol = 1e-8\nmultipole_lmax = 6\n"
                 "poisson_mode = fp32dc\nsor_criterion = rel_resid\noutput_enable = false\n"
                 "ic_file = state.bin\ndc_inner = multigrid\nmoment_order = sequential\n"
-                "checkpoint = true\n")
+                "checkpoint = true\nnn_correction = corr.bin\n")
     class Deck(C.Structure):
         _fields_ = _PREFIX_FIELDS + [
             ("max_steps", C.c_int), ("sor_omega", C.c_double), ("sor_max_iter", C.c_int),
             ("sor_tol", C.c_double), ("sor_criterion", C.c_int), ("multipole_lmax", C.c_int),
             ("poisson_mode", C.c_int), ("nn_guess", C.c_char * 256), ("output_enable", C.c_bool),
             ("ic_file", C.c_char * 256), ("dc_inner", C.c_int), ("moment_order", C.c_int),
-            ("checkpoint", C.c_bool)]
+            ("checkpoint", C.c_bool), ("nn_correction", C.c_char * 256)]
 
     lib = H.lib()
     p = lib.input_deck_create(path.encode())
     d = Deck.from_address(p)
     got = (d.nx, d.max_steps, d.sor_omega, d.sor_max_iter, d.sor_tol, d.sor_criterion,
            d.multipole_lmax, d.poisson_mode, d.output_enable, d.ic_file, d.dc_inner, d.moment_order,
-           d.checkpoint)
+           d.checkpoint, d.nn_correction)
     lib.input_deck_destroy(p)
     assert got == (32, 200, 1.5, 10000, 1e-8, H.SOR_CRIT_REL_RESID, 6, H.POISSON_FP32_DC, False,
-                   b"state.bin", H.DC_INNER_MULTIGRID, H.MOMENT_ORDER_SEQUENTIAL, True)
+                   b"state.bin", H.DC_INNER_MULTIGRID, H.MOMENT_ORDER_SEQUENTIAL, True,
+                   b"corr.bin")
 
 
 def test_initial_state_file_roundtrip(tmp_path):

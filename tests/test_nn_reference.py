@@ -101,3 +101,28 @@ def test_gradient_rounding_is_identity_forward_bf16_backward():
     x.grad = None
     (R.gbf16(x, enabled=False) * c).sum().backward()
     assert torch.equal(x.grad, c)
+
+
+def test_correction_pair_helpers():
+    """The start correction's host helpers: the 7-point Laplacian is exact on
+    a quadratic (6 on the interior, 0 on the boundary), the f^3 average keeps
+    the mean, and C is the median ratio rms(e) / (rms(r) L^2)."""
+    from hydrocode_astro1_amd import nn
+    n = 16
+    h = 1.0 / n
+    X = (np.arange(n) + 0.5) * h
+    x, y, z = np.meshgrid(X, X, X, indexing="ij")
+    lap = nn.laplacian(x * x + y * y + z * z, h)
+    assert np.allclose(lap[1:-1, 1:-1, 1:-1], 6.0, rtol=1e-9)
+    assert np.all(lap[0] == 0) and np.all(lap[:, :, -1] == 0)
+    a = np.random.default_rng(1).standard_normal((n, n, n))
+    avg = nn.average(a, 4)
+    assert avg.shape == (4, 4, 4)
+    assert avg.mean() == pytest.approx(a.mean(), abs=1e-12)
+    assert avg[1, 2, 3] == pytest.approx(a[4:8, 8:12, 12:16].mean())
+    e = nn.zero_boundary(a.copy())
+    assert np.all(e[:, 0] == 0) and np.all(e[-1] == 0) and np.all(e[1:-1, 1:-1, 1:-1] == a[1:-1, 1:-1, 1:-1])
+    R_ = [a, 2 * a, 0 * a, a]
+    E_ = [3 * a, 6.5 * a, a, 2 * a]
+    assert nn.correction_scale(R_, E_) == pytest.approx(3.0)
+    assert nn.correction_scale(R_, E_, box_len=2.0) == pytest.approx(0.75)
