@@ -430,7 +430,8 @@ def selfgravity_block(H, ics, n, steps=10, arms=None):
         def f(gr):
             info = setup(gr)
             gr.set_dc_inner(H.DC_INNER_SOR)
-            info["iterations_are"] = "SOR sweeps"
+            gr.set_params(1.5, 400000, 1e-8)
+            info["iterations_are"] = "SOR sweeps (omega 1.5)"
             return info
         return f
 

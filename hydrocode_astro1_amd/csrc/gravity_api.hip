@@ -311,6 +311,19 @@ int ensure_dc_buffers(GravDevice* d) {
     return 0;
 }
 
+// Boundary values and residual r (FP32, split) of the starting potential,
+// unless the start choice left them in place.
+int start_residual(GravitySolver3D* s, double* rel) {
+    GravDevice* d = s->dev;
+    if (d->start_residual_ready) {
+        d->start_residual_ready = false;
+        *rel = d->start_rel;
+        return 0;
+    }
+    if (apply_boundary(s)) return -2;
+    return rel_residual_dev(s, d->r_r, d->r_b, rel);
+}
+
 // Defect correction: outer FP64 residual r = f - L phi, inner FP32
 // red-black sweeps on L e = r with e = 0 on the boundary, phi += e in FP64.
 // Stops on the relative residual ||r|| / ||f|| <= tolerance.
@@ -318,9 +331,8 @@ int sor_fp32_dc(GravitySolver3D* s) {
     GravDevice* d = s->dev;
     if (ensure_dc_buffers(d)) return -2;
     const SorCoef c = coefficients(s);
-    if (apply_boundary(s)) return -2;
     double rel;
-    if (rel_residual_dev(s, d->r_r, d->r_b, &rel)) return -2;
+    if (start_residual(s, &rel)) return -2;
     int sweeps = 0, k = 64;
     const double tol = s->tolerance;
     for (int outer = 0; outer < 64 && rel > tol && sweeps < s->max_iterations; ++outer) {
@@ -509,9 +521,8 @@ int dc_multigrid(GravitySolver3D* s) {
     GravDevice* d = s->dev;
     if (ensure_dc_buffers(d) || ensure_mg(s)) return -2;
     const SorCoef c = coefficients(s);
-    if (apply_boundary(s)) return -2;
     double rel;
-    if (rel_residual_dev(s, d->r_r, d->r_b, &rel)) return -2;
+    if (start_residual(s, &rel)) return -2;
     int cycles = 0, k = 3;
     const double tol = s->tolerance;
     for (int outer = 0; outer < 64 && rel > tol && cycles < s->max_iterations; ++outer) {
@@ -561,6 +572,7 @@ int choose_guess(GravitySolver3D* s, const double* d_rho) {
     if (apply_boundary(s) || rel_residual_dev(s, d->r_r, d->r_b, &best)) return -2;
     s->last_warm_rel_residual = best;
     bool any = false;
+    bool ready = true;  // r_r / r_b hold the residual of the start in phi
     if (s->nn_correction && best > s->tolerance) {
         // guess = warm start + correction, the start's boundary values kept
         const int rc = nn_correction_device(s->nn_correction, d->r_r, d->r_b, s->nx,
@@ -572,7 +584,7 @@ int choose_guess(GravitySolver3D* s, const double* d_rho) {
             std::swap(d->phi_r, d->guess_r);
             std::swap(d->phi_b, d->guess_b);
             double rc2;
-            if (rel_residual_dev(s, nullptr, nullptr, &rc2)) return -2;
+            if (rel_residual_dev(s, d->r_r, d->r_b, &rc2)) return -2;
             s->last_corr_rel_residual = rc2;
             if (rc2 < best) {
                 best = rc2;
@@ -580,6 +592,7 @@ int choose_guess(GravitySolver3D* s, const double* d_rho) {
             } else {
                 std::swap(d->phi_r, d->guess_r);
                 std::swap(d->phi_b, d->guess_b);
+                ready = false;  // r now holds the rejected start's residual
             }
         }
     }
@@ -600,6 +613,7 @@ int choose_guess(GravitySolver3D* s, const double* d_rho) {
             if (rn < best) {
                 best = rn;
                 s->last_guess = 1;
+                ready = false;
             } else {
                 std::swap(d->phi_r, d->guess_r);
                 std::swap(d->phi_b, d->guess_b);
@@ -607,11 +621,14 @@ int choose_guess(GravitySolver3D* s, const double* d_rho) {
         }
     }
     if (any && s->last_guess < 0) s->last_guess = 0;
+    d->start_residual_ready = ready;
+    d->start_rel = best;
     return 0;
 }
 
 int potential_device(GravitySolver3D* s, const double* d_rho) {
     GravDevice* d = s->dev;
+    d->start_residual_ready = false;
     HYD_CHECK(hipEventRecord(d->ev0, d->stream));
     const double four_pi_G = 4.0 * kPi * s->G;  // gravity_solver_3d.c:527
     if (d->two_d) {

@@ -57,6 +57,11 @@ struct GravDevice {
     double *guess_r = nullptr, *guess_b = nullptr;  // learned guess (split layout)
     bool owns_nn = false;  // the network was created from an input deck
     bool owns_corr = false;  // the same for the start-correction network
+    // set by the start choice of POISSON_NN_FP32_DC when r_r / r_b already
+    // hold the chosen start's residual (boundary applied) and start_rel its
+    // relative residual; the defect correction then skips recomputing them
+    bool start_residual_ready = false;
+    double start_rel = 0.0;
     // multigrid inner solve of the defect correction (DC_INNER_MULTIGRID):
     // pre- and post-smoothing sweeps per level and their SOR factor; V(1,1)
     // with 1.15 reached 1e-8 fastest at 256^3 (5.9 ms against 7.7 for

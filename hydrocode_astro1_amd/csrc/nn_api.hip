@@ -21,7 +21,8 @@
 // 64^3 (features r / s and -r / s, s the root mean square of r), and its
 // output the correction e of the start (phi + e solves the problem) on that
 // grid in units of C L^2 s, C a constant stored with the weights and L the
-// box edge; e is interpolated trilinearly to the solver's grid.
+// box edge; e is interpolated to the solver's grid by cubic convolution
+// (Catmull-Rom), odd about the boundary.
 #include <math.h>
 #include <stdio.h>
 #include <stdlib.h>
@@ -87,6 +88,8 @@ struct Model {
     double* sumsq = nullptr;      // [1] sum of squares of the correction input
     float* rcoarse = nullptr;     // averaged residual of the start correction
     int rcoarse_n = 0;
+    float *interp1 = nullptr, *interp2 = nullptr;  // interpolation stages of the correction
+    int interp_n = 0;
 };
 
 void free_acts(Acts& a) {
@@ -392,7 +395,8 @@ void nn_model_destroy(void* h) {
     Model* M = (Model*)h;
     if (!M) return;
     free_acts(M->acts);
-    void* ps[] = {M->params, M->grads, M->m, M->v, M->mom_tmp, M->wg_part, M->sumsq, M->rcoarse};
+    void* ps[] = {M->params, M->grads,  M->m,       M->v,      M->mom_tmp,
+                  M->wg_part, M->sumsq, M->rcoarse, M->interp1, M->interp2};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     for (double* p : M->coarse) (void)hipFree(p);
@@ -836,7 +840,8 @@ int nn_guess_device(void* hm, const double* d_rho, int n, const double* d_mom, d
 // residual (rr, rb) of the start (pr, pb) is averaged onto nc^3 (nc = n
 // halved until at most 64), the start-correction network runs there, and
 // guess = start + C L^2 s P(out) on the interior, the start on the boundary
-// (split layout, FP64).  Returns -1 when the grid does not fit the network.
+// (split layout, FP64), P cubic (Catmull-Rom) interpolation.  Returns -1
+// when the grid does not fit the network.
 int nn_correction_device(void* hm, const float* rr, const float* rb, int n, double L,
                          const double* pr, const double* pb, double* gr, double* gb) {
     using namespace nnh;
@@ -852,9 +857,19 @@ int nn_correction_device(void* hm, const float* rr, const float* rb, int n, doub
         HYD_CHECK(hipMalloc((void**)&M->rcoarse, sizeof(float) * vox(nc)));
         M->rcoarse_n = nc;
     }
+    if (M->interp_n != n) {
+        if (M->interp1) (void)hipFree(M->interp1);
+        if (M->interp2) (void)hipFree(M->interp2);
+        M->interp1 = M->interp2 = nullptr;
+        M->interp_n = 0;
+        HYD_CHECK(hipMalloc((void**)&M->interp1, sizeof(float) * nc * nc * n));
+        HYD_CHECK(hipMalloc((void**)&M->interp2, sizeof(float) * nc * (long)n * n));
+        M->interp_n = n;
+    }
     if (alloc_acts(M->acts, nc)) return -2;
     HYD_CHECK(corr_restrict(rr, rb, n, n / nc, M->rcoarse, M->st));
     if (features_corr(*M, M->rcoarse, nc) || forward(*M, nc)) return -2;
-    HYD_CHECK(corr_apply(M->acts.out, nc, n, M->sumsq, M->cscale * L * L, pr, pb, gr, gb, M->st));
+    HYD_CHECK(corr_apply(M->acts.out, nc, n, M->interp1, M->interp2, M->sumsq,
+                         M->cscale * L * L, pr, pb, gr, gb, M->st));
     return 0;
 }

@@ -778,12 +778,79 @@ __global__ void corr_features_kernel(const float* __restrict__ r, long nvox,
     }
 }
 
+// Catmull-Rom taps of fine cell i (f fine cells per coarse cell, nc coarse
+// cells, cell centres): the coarse cells a - 1 .. a + 2 around the coarse
+// coordinate x = (i + 0.5) / f - 0.5 with the cubic convolution weights of
+// t = x - a; a tap outside the grid takes the mirrored cell with its weight
+// negated (an odd reflection: the correction vanishes at the boundary).
+// Trilinear interpolation leaves a Laplacian spike at every coarse cell
+// centre (the slope jumps there), which makes the interpolated correction's
+// residual larger than the one it removes once f reaches 4; this cubic is
+// continuously differentiable.
+__device__ __forceinline__ void cr_taps(int i, int f, int nc, int idx[4], float w[4]) {
+    const float x = ((float)i + 0.5f) / (float)f - 0.5f;
+    const float af = floorf(x);
+    const float t = x - af, t2 = t * t, t3 = t2 * t;
+    w[0] = 0.5f * (-t3 + 2.0f * t2 - t);
+    w[1] = 0.5f * (3.0f * t3 - 5.0f * t2 + 2.0f);
+    w[2] = 0.5f * (-3.0f * t3 + 4.0f * t2 + t);
+    w[3] = 0.5f * (t3 - t2);
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        int c = (int)af - 1 + q;
+        if (c < 0) {
+            c = -c - 1;
+            w[q] = -w[q];
+        } else if (c >= nc) {
+            c = 2 * nc - c - 1;
+            w[q] = -w[q];
+        }
+        idx[q] = c;
+    }
+}
+
+// Separable interpolation of e (nc^3, natural) to the fine grid, first
+// along z: t1[I][J][k] (nc x nc x n), then along y: t2[I][j][k] (nc x n x n).
+__global__ void corr_interp_z_kernel(const float* __restrict__ e, int nc, int n,
+                                     float* __restrict__ t1) {
+    const long total = (long)nc * nc * n;
+    const int f = n / nc;
+    for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < total;
+         v += (long)gridDim.x * blockDim.x) {
+        const int k = (int)(v % n);
+        const long row = v / n;  // I * nc + J
+        int idx[4];
+        float w[4];
+        cr_taps(k, f, nc, idx, w);
+        const float* src = e + row * nc;
+        t1[v] = w[0] * src[idx[0]] + w[1] * src[idx[1]] + w[2] * src[idx[2]] + w[3] * src[idx[3]];
+    }
+}
+
+__global__ void corr_interp_y_kernel(const float* __restrict__ t1, int nc, int n,
+                                     float* __restrict__ t2) {
+    const long total = (long)nc * n * n;
+    const int f = n / nc;
+    for (long v = (long)blockIdx.x * blockDim.x + threadIdx.x; v < total;
+         v += (long)gridDim.x * blockDim.x) {
+        const int k = (int)(v % n);
+        const int j = (int)((v / n) % n);
+        const int I = (int)(v / ((long)n * n));
+        int idx[4];
+        float w[4];
+        cr_taps(j, f, nc, idx, w);
+        const float* src = t1 + (long)I * nc * n + k;
+        t2[v] = w[0] * src[(long)idx[0] * n] + w[1] * src[(long)idx[1] * n] +
+                w[2] * src[(long)idx[2] * n] + w[3] * src[(long)idx[3] * n];
+    }
+}
+
 // guess = phi + scale * s * P(e) at the interior cells of the split n^3
-// grid and phi at its boundary cells: e the network output on the nc^3
-// grid, P trilinear interpolation (edges clamped, as prolong_kernel), s the
-// root mean square of the restricted residual.  One thread per split entry
-// (i, j, m), both colours.
-__global__ void corr_apply_kernel(const float* __restrict__ e, int nc, int n,
+// grid and phi at its boundary cells: P the separable Catmull-Rom
+// interpolation above, finished here along x from t2; s the root mean
+// square of the averaged residual.  One thread per split entry (i, j, m),
+// both colours.
+__global__ void corr_apply_kernel(const float* __restrict__ t2, int nc, int n,
                                   const double* __restrict__ sumsq, double scale,
                                   const double* __restrict__ pr, const double* __restrict__ pb,
                                   double* __restrict__ gr, double* __restrict__ gb) {
@@ -794,26 +861,21 @@ __global__ void corr_apply_kernel(const float* __restrict__ e, int nc, int n,
     if (m >= nzh || j >= n) return;
     const double sc = scale * sqrt(sumsq[0] / ((double)nc * nc * nc));
     const long base = ((long)i * n + j) * nzh + m;
-    int i0, i1, j0, j1;
-    float wi, wj;
-    lin_weights(i, f, nc, i0, i1, wi);
-    lin_weights(j, f, nc, j0, j1, wj);
+    int idx[4];
+    float w[4];
+    cr_taps(i, f, nc, idx, w);
+    const bool inner_ij = i >= 1 && i <= n - 2 && j >= 1 && j <= n - 2;
 #pragma unroll
     for (int colour = 0; colour < 2; ++colour) {
         const int k = 2 * m + ((i + j + colour) & 1);
         const double p = (colour ? pb : pr)[base];
         double v = p;
-        if (i >= 1 && i <= n - 2 && j >= 1 && j <= n - 2 && k >= 1 && k <= n - 2) {
-            int k0, k1;
-            float wk;
-            lin_weights(k, f, nc, k0, k1, wk);
-            auto at = [&](int a, int b, int d) { return e[((long)a * nc + b) * nc + d]; };
-            const float c00 = at(i0, j0, k0) * (1 - wk) + at(i0, j0, k1) * wk;
-            const float c01 = at(i0, j1, k0) * (1 - wk) + at(i0, j1, k1) * wk;
-            const float c10 = at(i1, j0, k0) * (1 - wk) + at(i1, j0, k1) * wk;
-            const float c11 = at(i1, j1, k0) * (1 - wk) + at(i1, j1, k1) * wk;
-            const float c0 = c00 * (1 - wj) + c01 * wj, c1 = c10 * (1 - wj) + c11 * wj;
-            v = p + sc * (double)(c0 * (1 - wi) + c1 * wi);
+        if (inner_ij && k >= 1 && k <= n - 2) {
+            const float* src = t2 + (long)j * n + k;
+            const long pl = (long)n * n;
+            const float c = w[0] * src[idx[0] * pl] + w[1] * src[idx[1] * pl] +
+                            w[2] * src[idx[2] * pl] + w[3] * src[idx[3] * pl];
+            v = p + sc * (double)c;
         }
         (colour ? gb : gr)[base] = v;
     }
@@ -1004,11 +1066,16 @@ hipError_t corr_features(const float* r, int nc, double* sumsq, uint16_t* x, flo
                        mono);
     return hipGetLastError();
 }
-hipError_t corr_apply(const float* e, int nc, int n, const double* sumsq, double scale,
-                      const double* pr, const double* pb, double* gr, double* gb, hipStream_t st) {
+hipError_t corr_apply(const float* e, int nc, int n, float* t1, float* t2, const double* sumsq,
+                      double scale, const double* pr, const double* pb, double* gr, double* gb,
+                      hipStream_t st) {
+    hipLaunchKernelGGL(corr_interp_z_kernel, dim3(grid_n((long)nc * nc * n)), dim3(256), 0, st, e,
+                       nc, n, t1);
+    hipLaunchKernelGGL(corr_interp_y_kernel, dim3(grid_n((long)nc * n * n)), dim3(256), 0, st, t1,
+                       nc, n, t2);
     hipLaunchKernelGGL(corr_apply_kernel,
                        dim3((unsigned)((n / 2 + 63) / 64), (unsigned)((n + 3) / 4), (unsigned)n),
-                       dim3(256), 0, st, e, nc, n, sumsq, scale, pr, pb, gr, gb);
+                       dim3(256), 0, st, t2, nc, n, sumsq, scale, pr, pb, gr, gb);
     return hipGetLastError();
 }
 
