@@ -237,9 +237,10 @@ def poisson_block(H, ics, n):
     Warm start (the case of every step after the first): phi_0 = the
     converged potential of the initial state, one hydro step with gravity,
     then each arm starts from phi_0 on the new density: plain FP64 SOR, FP32
-    defect correction, and the network guess (kept only when its residual
-    beats phi_0) + FP32 defect correction.  Cold start (the first step):
-    plain FP64 SOR from zero vs network guess + FP32 defect correction."""
+    defect correction, and the learned starts + FP32 defect correction: the
+    potential guess and phi_0 plus the start correction's correction of it,
+    each kept only when its residual beats phi_0's.  Cold start (the first
+    step): plain FP64 SOR from zero vs the same arms from zero."""
     import numpy as np
     from hydrocode_astro1_amd import nn
 
@@ -269,6 +270,10 @@ def poisson_block(H, ics, n):
         out["network"].update(network_timing(H, model))
     else:
         out["network"] = {"weights": None, "note": "no trained weights; network arms skipped"}
+    if model is not None and os.path.exists(nn.CORRECTION_WEIGHTS):
+        corr = nn.Model(path=nn.CORRECTION_WEIGHTS)
+        gr.set_nn_correction(corr)
+        out["network"]["correction_weights"] = os.path.relpath(nn.CORRECTION_WEIGHTS, ROOT)
 
     def arm(mode, phi_start, rho, inner=H.DC_INNER_SOR):
         gr.set_mode(mode)
@@ -279,8 +284,12 @@ def poisson_block(H, ics, n):
         r = {"status": st, "ms": gr.s.last_solve_ms, "rel_residual": gr.s.last_rel_residual}
         r["v_cycles" if inner == H.DC_INNER_MULTIGRID else "iterations"] = it
         if mode == H.POISSON_NN_FP32_DC:
-            r["guess_used"] = bool(gr.s.last_guess == 1)
+            # start used: 0 the given one, 1 the potential guess, 2 the given
+            # one plus the start correction
+            r["start_used"] = gr.s.last_guess
+            r["guess_used"] = bool(gr.s.last_guess in (1, 2))
             r["guess_rel_residual"] = gr.s.last_guess_rel_residual
+            r["corrected_rel_residual"] = gr.s.last_corr_rel_residual
             r["start_rel_residual"] = gr.s.last_warm_rel_residual
         return r, gr.phi()
 
@@ -329,6 +338,7 @@ def poisson_block(H, ics, n):
     out["speedup_cold_vs_plain"] = {k: cold["ms"] / v["ms"] for k, v in coldres.items()
                                     if k != "plain_fp64_sor" and v["ms"] > 0}
     gr.set_nn_model(None)
+    gr.set_nn_correction(None)
     s.close()
     return out
 

@@ -86,6 +86,7 @@ struct Model {
     int kind = 0;                 // 0 potential guess, 1 start correction
     double cscale = 1.0;          // C of the start correction
     double* sumsq = nullptr;      // [1] sum of squares of the correction input
+    double* sumsq_part = nullptr; // [256] its partial sums
     float* rcoarse = nullptr;     // averaged residual of the start correction
     int rcoarse_n = 0;
     float *interp1 = nullptr, *interp2 = nullptr;  // interpolation stages of the correction
@@ -305,7 +306,7 @@ int features(Model& M, const T* rho, int n, const double* mom, const double box[
 
 // features of the start correction from an averaged residual r (n^3 FP32)
 int features_corr(Model& M, const float* r, int n) {
-    HYD_CHECK(corr_features(r, n, M.sumsq, M.acts.x0, M.acts.mono, M.st));
+    HYD_CHECK(corr_features(r, n, M.sumsq, M.sumsq_part, M.acts.x0, M.acts.mono, M.st));
     return 0;
 }
 
@@ -371,7 +372,8 @@ void* nn_model_create(unsigned long long seed) {
               hipMalloc((void**)&M->m, sizeof(float) * off) == hipSuccess &&
               hipMalloc((void**)&M->v, sizeof(float) * off) == hipSuccess &&
               hipMalloc((void**)&M->mom_tmp, sizeof(double) * 4) == hipSuccess &&
-              hipMalloc((void**)&M->sumsq, sizeof(double) * 4) == hipSuccess;
+              hipMalloc((void**)&M->sumsq, sizeof(double) * 4) == hipSuccess &&
+              hipMalloc((void**)&M->sumsq_part, sizeof(double) * 256) == hipSuccess;
     for (int l = 0; ok && l < kNL; ++l) {
         const LayerDef& L = kLayers[l];
         ok = hipMalloc((void**)&M->wp[l], sizeof(uint16_t) * packed_len(pad4(L.cin), L.cout)) ==
@@ -396,7 +398,7 @@ void nn_model_destroy(void* h) {
     if (!M) return;
     free_acts(M->acts);
     void* ps[] = {M->params, M->grads,  M->m,       M->v,      M->mom_tmp,
-                  M->wg_part, M->sumsq, M->rcoarse, M->interp1, M->interp2};
+                  M->wg_part, M->sumsq, M->rcoarse, M->interp1, M->interp2, M->sumsq_part};
     for (void* p : ps)
         if (p) (void)hipFree(p);
     for (double* p : M->coarse) (void)hipFree(p);

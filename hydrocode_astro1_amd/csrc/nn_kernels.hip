@@ -741,22 +741,32 @@ __global__ void corr_restrict_kernel(const float* __restrict__ rr, const float* 
     }
 }
 
-// sum of x^2 over count values by one workgroup, in a fixed order
-__global__ void __launch_bounds__(1024) corr_sumsq_kernel(const float* __restrict__ x, long count,
-                                                          double* __restrict__ out) {
-    __shared__ double part[1024];
+// Sum of x^2 over count values in a fixed order: CORR_SUM_BLOCKS
+// workgroups of 256 each sum a strided share and reduce it by a tree
+// (first kernel, x != nullptr), then one workgroup adds the partial sums by
+// the same tree (second kernel, x == nullptr).
+constexpr int CORR_SUM_BLOCKS = 256;
+__global__ void __launch_bounds__(256) corr_sumsq_kernel(const float* __restrict__ x, long count,
+                                                         double* __restrict__ part,
+                                                         double* __restrict__ out) {
+    __shared__ double red[256];
     double s = 0.0;
-    for (long v = threadIdx.x; v < count; v += 1024) {
-        const double a = (double)x[v];
-        s += a * a;
+    if (x) {
+        const long stride = (long)CORR_SUM_BLOCKS * 256;
+        for (long v = (long)blockIdx.x * 256 + threadIdx.x; v < count; v += stride) {
+            const double a = (double)x[v];
+            s += a * a;
+        }
+    } else {
+        s = part[threadIdx.x];
     }
-    part[threadIdx.x] = s;
+    red[threadIdx.x] = s;
     __syncthreads();
-    for (int o = 512; o > 0; o >>= 1) {
-        if ((int)threadIdx.x < o) part[threadIdx.x] += part[threadIdx.x + o];
+    for (int o = 128; o > 0; o >>= 1) {
+        if ((int)threadIdx.x < o) red[threadIdx.x] += red[threadIdx.x + o];
         __syncthreads();
     }
-    if (threadIdx.x == 0) out[0] = part[0];
+    if (threadIdx.x == 0) (x ? part[blockIdx.x] : out[0]) = red[0];
 }
 
 // Features of the correction network (cp = 4 layout): channel 0 = r / s,
@@ -1058,10 +1068,13 @@ hipError_t corr_restrict(const float* rr, const float* rb, int n, int f, float* 
                        out);
     return hipGetLastError();
 }
-hipError_t corr_features(const float* r, int nc, double* sumsq, uint16_t* x, float* mono,
-                         hipStream_t st) {
+hipError_t corr_features(const float* r, int nc, double* sumsq, double* part, uint16_t* x,
+                         float* mono, hipStream_t st) {
     const long nv = (long)nc * nc * nc;
-    hipLaunchKernelGGL(corr_sumsq_kernel, dim3(1), dim3(1024), 0, st, r, nv, sumsq);
+    hipLaunchKernelGGL(corr_sumsq_kernel, dim3(CORR_SUM_BLOCKS), dim3(256), 0, st, r, nv, part,
+                       sumsq);
+    hipLaunchKernelGGL(corr_sumsq_kernel, dim3(1), dim3(256), 0, st, (const float*)nullptr, nv,
+                       part, sumsq);
     hipLaunchKernelGGL(corr_features_kernel, dim3(grid_n(nv)), dim3(256), 0, st, r, nv, sumsq, x,
                        mono);
     return hipGetLastError();

@@ -266,3 +266,16 @@ def test_trained_correction_on_a_held_out_run():
     warm = slice(4, None)
     assert np.mean(used[warm] == 2) >= 0.75, used
     assert corr[warm].sum() < 0.95 * base[warm].sum(), (corr, base)
+
+
+@pytest.mark.skipif(not os.path.exists(nn.CORRECTION_WEIGHTS),
+                    reason="no trained start-correction weights")
+def test_trained_correction_at_256_with_multigrid():
+    """The committed start correction on the 256^3 stellar explosion (seed
+    12345, held out of training) with multigrid inner solves: the corrected
+    start wins the warm steps and the run needs fewer V-cycles."""
+    base, _ = explosion_sweeps(256, 12345, 12, None, H.DC_INNER_MULTIGRID)
+    corr, used = explosion_sweeps(256, 12345, 12, nn.CORRECTION_WEIGHTS, H.DC_INNER_MULTIGRID)
+    warm = slice(1, None)
+    assert np.mean(used[warm] == 2) >= 0.9, used
+    assert corr[warm].sum() < base[warm].sum(), (corr, base)
