@@ -552,11 +552,13 @@ int dc_multigrid(GravitySolver3D* s) {
 
 // Starting potential of POISSON_NN_FP32_DC.  Candidates, in this order: the
 // warm start (the potential left by the previous solve), the warm start
-// plus the start-correction network's correction of it (computed from its
+// plus the start-correction network's correction e of it (computed from its
 // residual, and only while that residual is above the tolerance), and the
 // potential-guess network's guess.  Each is scored by its relative residual
 // with this density's boundary values; the smallest wins, a tie keeps the
-// earlier candidate.
+// earlier candidate.  The corrected start is kept as phi and e: e is the
+// FP32 unknown of the defect correction's inner solve, which starts from it
+// (and phi += e follows that solve), while r stays phi's residual.
 int choose_guess(GravitySolver3D* s, const double* d_rho) {
     GravDevice* d = s->dev;
     s->last_guess = -1;
@@ -565,38 +567,41 @@ int choose_guess(GravitySolver3D* s, const double* d_rho) {
     s->last_corr_rel_residual = -1.0;
     const bool cubic = s->nx == s->ny && s->ny == s->nz && s->dx == s->dy && s->dy == s->dz;
     if ((!s->nn_model && !s->nn_correction) || !cubic || s->bc_type != GRAV_BC_ISOLATED) return 0;
-    if (!d->guess_r && (dev_alloc(&d->guess_r, d->nsplit) || dev_alloc(&d->guess_b, d->nsplit)))
-        return -2;
     if (ensure_dc_buffers(d)) return -2;
     double best;
     if (apply_boundary(s) || rel_residual_dev(s, d->r_r, d->r_b, &best)) return -2;
+    const double ff = d->acc_host[1];  // sum of f^2 over the interior
     s->last_warm_rel_residual = best;
     bool any = false;
-    bool ready = true;  // r_r / r_b hold the residual of the start in phi
+    bool ready = true;       // r_r / r_b hold the residual of the start in phi
+    bool e_start = false;    // e holds the correction of the chosen start
+    const size_t ebytes = sizeof(float) * (size_t)d->nsplit;
     if (s->nn_correction && best > s->tolerance) {
-        // guess = warm start + correction, the start's boundary values kept
         const int rc = nn_correction_device(s->nn_correction, d->r_r, d->r_b, s->nx,
-                                            s->xmax - s->xmin, d->phi_r, d->phi_b, d->guess_r,
-                                            d->guess_b);
+                                            s->xmax - s->xmin, d->e_r, d->e_b);
         if (rc < -1) return -2;
         if (rc == 0) {
             any = true;
-            std::swap(d->phi_r, d->guess_r);
-            std::swap(d->phi_b, d->guess_b);
-            double rc2;
-            if (rel_residual_dev(s, d->r_r, d->r_b, &rc2)) return -2;
+            HYD_CHECK(hyd::launch_start_residual_sq(d->g, coefficients(s), d->e_r, d->e_b, d->r_r,
+                                                    d->r_b, d->res_part, d->acc, d->stream));
+            HYD_CHECK(hipMemcpyAsync(d->acc_host, d->acc, sizeof(double), hipMemcpyDeviceToHost,
+                                     d->stream));
+            HYD_CHECK(hipStreamSynchronize(d->stream));
+            const double rc2 = ff > 0.0 ? sqrt(d->acc_host[0] / ff) : sqrt(d->acc_host[0]);
             s->last_corr_rel_residual = rc2;
             if (rc2 < best) {
                 best = rc2;
                 s->last_guess = 2;
+                e_start = true;
             } else {
-                std::swap(d->phi_r, d->guess_r);
-                std::swap(d->phi_b, d->guess_b);
-                ready = false;  // r now holds the rejected start's residual
+                HYD_CHECK(hipMemsetAsync(d->e_r, 0, ebytes, d->stream));
+                HYD_CHECK(hipMemsetAsync(d->e_b, 0, ebytes, d->stream));
             }
         }
     }
     if (s->nn_model) {
+        if (!d->guess_r && (dev_alloc(&d->guess_r, d->nsplit) || dev_alloc(&d->guess_b, d->nsplit)))
+            return -2;
         if (!d->nat_tmp && dev_alloc(&d->nat_tmp, d->n)) return -2;
         const double box[4] = {s->xmin, s->ymin, s->zmin, s->xmax - s->xmin};
         const int rc = nn_guess_device(s->nn_model, d_rho, s->nx, d->mom, s->G, box, d->nat_tmp);
@@ -614,6 +619,11 @@ int choose_guess(GravitySolver3D* s, const double* d_rho) {
                 best = rn;
                 s->last_guess = 1;
                 ready = false;
+                if (e_start) {  // the corrected start lost
+                    HYD_CHECK(hipMemsetAsync(d->e_r, 0, ebytes, d->stream));
+                    HYD_CHECK(hipMemsetAsync(d->e_b, 0, ebytes, d->stream));
+                    e_start = false;
+                }
             } else {
                 std::swap(d->phi_r, d->guess_r);
                 std::swap(d->phi_b, d->guess_b);
@@ -621,6 +631,11 @@ int choose_guess(GravitySolver3D* s, const double* d_rho) {
         }
     }
     if (any && s->last_guess < 0) s->last_guess = 0;
+    if (e_start && (best <= s->tolerance || s->max_iterations < 1)) {
+        // no inner solve will run: phi += e now, and the solve checks phi
+        HYD_CHECK(hyd::launch_correct(d->g, d->phi_r, d->phi_b, d->e_r, d->e_b, d->stream));
+        ready = false;
+    }
     d->start_residual_ready = ready;
     d->start_rel = best;
     return 0;

@@ -758,6 +758,56 @@ __global__ void __launch_bounds__(256) residual_finish_kernel(const double* __re
     }
 }
 
+// Sum over the interior cells of (r - L e)^2: the residual of the start
+// phi + e when r is phi's residual (L e in FP32 as the inner sweeps apply
+// it, squares and sums in FP64), per block into part[2 b] (part[2 b + 1] =
+// 0) in the geometry of residual_kernel, so residual_finish_kernel adds the
+// blocks in the same fixed order.
+__global__ void __launch_bounds__(256) start_residual_sq_kernel(GridDims g, float dx2i,
+                                                                float dy2i, float dz2i,
+                                                                float factor,
+                                                                const float* __restrict__ er,
+                                                                const float* __restrict__ eb,
+                                                                const float* __restrict__ rr,
+                                                                const float* __restrict__ rb,
+                                                                double* __restrict__ part) {
+    const int m = blockIdx.x * 64 + (threadIdx.x & 63);
+    const int j = blockIdx.y * SOR_ROWS + (threadIdx.x >> 6);
+    const int i0 = blockIdx.z * RES_PLANES;
+    const int i1 = i0 + RES_PLANES < g.nx ? i0 + RES_PLANES : g.nx;
+    double s = 0.0;
+    if (m < g.nzh && j >= 1 && j <= g.ny - 2) {
+        const long sxh = (long)g.ny * g.nzh;
+        for (int i = i0 > 1 ? i0 : 1; i < i1 && i <= g.nx - 2; ++i) {
+            const long base = (long)i * sxh + (long)j * g.nzh;
+#pragma unroll
+            for (int colour = 0; colour < 2; ++colour) {
+                const int k = 2 * m + ((i + j + colour) & 1);
+                if (k < 1 || k > g.nz - 2) continue;
+                const float* mine = colour ? eb : er;
+                const float* other = colour ? er : eb;
+                const float lap = (other[base - sxh + m] + other[base + sxh + m]) * dx2i +
+                                  (other[base - g.nzh + m] + other[base + g.nzh + m]) * dy2i +
+                                  (other[base + ((k - 1) >> 1)] + other[base + ((k + 1) >> 1)]) *
+                                      dz2i -
+                                  factor * mine[base + m];
+                const double q = (double)((colour ? rb : rr)[base + m] - lap);
+                s += q * q;
+            }
+        }
+    }
+    __shared__ double sp[4];
+#pragma unroll
+    for (int o = 32; o > 0; o >>= 1) s += __shfl_xor(s, o, 64);
+    if ((threadIdx.x & 63) == 0) sp[threadIdx.x >> 6] = s;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        const long b = ((long)blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+        part[2 * b] = (sp[0] + sp[1]) + (sp[2] + sp[3]);
+        part[2 * b + 1] = 0.0;
+    }
+}
+
 // One colour of red-black SOR on L e = r in FP32 (same indexing as
 // sor_sweep_kernel); gs = (lap - r) * (1 / factor).
 __global__ void __launch_bounds__(256) sor_sweep_f32_kernel(GridDims g, float dx2i, float dy2i,
@@ -1044,6 +1094,17 @@ hipError_t launch_residual(const GridDims& g, const SorCoef& c, const double* ph
     const dim3 grid = residual_grid(g);
     hipLaunchKernelGGL(residual_kernel, grid, dim3(256), 0, st, g, c, phi_r, phi_b, rhs_r, rhs_b,
                        r_r, r_b, part);
+    hipLaunchKernelGGL(residual_finish_kernel, dim3(1), dim3(256), 0, st, part,
+                       (long)grid.x * grid.y * grid.z, acc);
+    return hipGetLastError();
+}
+
+hipError_t launch_start_residual_sq(const GridDims& g, const SorCoef& c, const float* e_r,
+                                   const float* e_b, const float* r_r, const float* r_b,
+                                   double* part, double* acc, hipStream_t st) {
+    const dim3 grid = residual_grid(g);
+    hipLaunchKernelGGL(start_residual_sq_kernel, grid, dim3(256), 0, st, g, (float)c.dx2i,
+                       (float)c.dy2i, (float)c.dz2i, (float)c.factor, e_r, e_b, r_r, r_b, part);
     hipLaunchKernelGGL(residual_finish_kernel, dim3(1), dim3(256), 0, st, part,
                        (long)grid.x * grid.y * grid.z, acc);
     return hipGetLastError();

@@ -137,6 +137,11 @@ long residual_scratch_len(const GridDims& g);
 hipError_t launch_residual(const GridDims& g, const SorCoef& c, const double* phi_r,
                            const double* phi_b, const double* rhs_r, const double* rhs_b,
                            float* r_r, float* r_b, double* part, double* acc, hipStream_t st);
+// acc[0] = sum over the interior of (r - L e)^2 (L e in FP32), acc[1] = 0;
+// part as for launch_residual.
+hipError_t launch_start_residual_sq(const GridDims& g, const SorCoef& c, const float* e_r,
+                                   const float* e_b, const float* r_r, const float* r_b,
+                                   double* part, double* acc, hipStream_t st);
 // FP32 red-black sweep of L e = r, e = 0 on the boundary.
 hipError_t launch_sor_sweep_f32(const GridDims& g, const SorCoef& c, int colour, float* e_r,
                                 float* e_b, const float* r_r, const float* r_b, hipStream_t st);

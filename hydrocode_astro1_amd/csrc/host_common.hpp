@@ -117,10 +117,10 @@ struct HydroDevice {
 // the grid does not fit the network.  box = x0, y0, z0, L.
 int nn_guess_device(void *model, const double *d_rho, int n, const double *d_mom, double G,
                     const double box[4], double *d_phi_nat);
-// Start plus the learned correction from its split FP32 residual (rr, rb)
-// into (gr, gb), split layout; -1 when the grid does not fit the network.
+// The learned correction of a start from its split FP32 residual (rr, rb)
+// into (er, eb), split FP32; -1 when the grid does not fit the network.
 int nn_correction_device(void *model, const float *rr, const float *rb, int n, double L,
-                         const double *pr, const double *pb, double *gr, double *gb);
+                         float *er, float *eb);
 
 // Internal interface between the hydro driver and the gravity solver.
 struct GravitySolver3D;

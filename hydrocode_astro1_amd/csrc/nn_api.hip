@@ -838,14 +838,14 @@ int nn_guess_device(void* hm, const double* d_rho, int n, const double* d_mom, d
     return 0;
 }
 
-// Corrected start of a cubic n^3 solve with box edge L: the split FP32
-// residual (rr, rb) of the start (pr, pb) is averaged onto nc^3 (nc = n
-// halved until at most 64), the start-correction network runs there, and
-// guess = start + C L^2 s P(out) on the interior, the start on the boundary
-// (split layout, FP64), P cubic (Catmull-Rom) interpolation.  Returns -1
-// when the grid does not fit the network.
-int nn_correction_device(void* hm, const float* rr, const float* rb, int n, double L,
-                         const double* pr, const double* pb, double* gr, double* gb) {
+// Correction of the start of a cubic n^3 solve with box edge L: the split
+// FP32 residual (rr, rb) of the start is averaged onto nc^3 (nc = n halved
+// until at most 64), the start-correction network runs there, and
+// e = C L^2 s P(out) on the interior, 0 on the boundary (split layout,
+// FP32), P cubic (Catmull-Rom) interpolation.  Returns -1 when the grid does
+// not fit the network.
+int nn_correction_device(void* hm, const float* rr, const float* rb, int n, double L, float* er,
+                         float* eb) {
     using namespace nnh;
     Model* M = (Model*)hm;
     if (M->kind != 1 || n % 2 != 0) return -1;
@@ -872,6 +872,6 @@ int nn_correction_device(void* hm, const float* rr, const float* rb, int n, doub
     HYD_CHECK(corr_restrict(rr, rb, n, n / nc, M->rcoarse, M->st));
     if (features_corr(*M, M->rcoarse, nc) || forward(*M, nc)) return -2;
     HYD_CHECK(corr_apply(M->acts.out, nc, n, M->interp1, M->interp2, M->sumsq,
-                         M->cscale * L * L, pr, pb, gr, gb, M->st));
+                         M->cscale * L * L, er, eb, M->st));
     return 0;
 }

@@ -855,15 +855,14 @@ __global__ void corr_interp_y_kernel(const float* __restrict__ t1, int nc, int n
     }
 }
 
-// guess = phi + scale * s * P(e) at the interior cells of the split n^3
-// grid and phi at its boundary cells: P the separable Catmull-Rom
-// interpolation above, finished here along x from t2; s the root mean
-// square of the averaged residual.  One thread per split entry (i, j, m),
-// both colours.
+// The correction scale * s * P(e) at the interior cells of the split n^3
+// grid and 0 at its boundary cells, in FP32 (the inner solve's unknown): P
+// the separable Catmull-Rom interpolation above, finished here along x from
+// t2; s the root mean square of the averaged residual.  One thread per
+// split entry (i, j, m), both colours.
 __global__ void corr_apply_kernel(const float* __restrict__ t2, int nc, int n,
                                   const double* __restrict__ sumsq, double scale,
-                                  const double* __restrict__ pr, const double* __restrict__ pb,
-                                  double* __restrict__ gr, double* __restrict__ gb) {
+                                  float* __restrict__ er, float* __restrict__ eb) {
     const int nzh = n / 2, f = n / nc;
     const int m = blockIdx.x * 64 + (threadIdx.x & 63);
     const int j = blockIdx.y * 4 + (threadIdx.x >> 6);
@@ -878,16 +877,15 @@ __global__ void corr_apply_kernel(const float* __restrict__ t2, int nc, int n,
 #pragma unroll
     for (int colour = 0; colour < 2; ++colour) {
         const int k = 2 * m + ((i + j + colour) & 1);
-        const double p = (colour ? pb : pr)[base];
-        double v = p;
+        float v = 0.0f;
         if (inner_ij && k >= 1 && k <= n - 2) {
             const float* src = t2 + (long)j * n + k;
             const long pl = (long)n * n;
             const float c = w[0] * src[idx[0] * pl] + w[1] * src[idx[1] * pl] +
                             w[2] * src[idx[2] * pl] + w[3] * src[idx[3] * pl];
-            v = p + sc * (double)c;
+            v = (float)(sc * (double)c);
         }
-        (colour ? gb : gr)[base] = v;
+        (colour ? eb : er)[base] = v;
     }
 }
 
@@ -1080,15 +1078,14 @@ hipError_t corr_features(const float* r, int nc, double* sumsq, double* part, ui
     return hipGetLastError();
 }
 hipError_t corr_apply(const float* e, int nc, int n, float* t1, float* t2, const double* sumsq,
-                      double scale, const double* pr, const double* pb, double* gr, double* gb,
-                      hipStream_t st) {
+                      double scale, float* er, float* eb, hipStream_t st) {
     hipLaunchKernelGGL(corr_interp_z_kernel, dim3(grid_n((long)nc * nc * n)), dim3(256), 0, st, e,
                        nc, n, t1);
     hipLaunchKernelGGL(corr_interp_y_kernel, dim3(grid_n((long)nc * n * n)), dim3(256), 0, st, t1,
                        nc, n, t2);
     hipLaunchKernelGGL(corr_apply_kernel,
                        dim3((unsigned)((n / 2 + 63) / 64), (unsigned)((n + 3) / 4), (unsigned)n),
-                       dim3(256), 0, st, t2, nc, n, sumsq, scale, pr, pb, gr, gb);
+                       dim3(256), 0, st, t2, nc, n, sumsq, scale, er, eb);
     return hipGetLastError();
 }
 

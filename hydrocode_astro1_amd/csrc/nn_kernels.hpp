@@ -110,17 +110,17 @@ hipError_t scale_potential(const double* phi, long count, const double* mom, dou
 // Correction of a warm start (the correction network): the split FP32
 // residual of an n^3 grid averaged over f^3 blocks into nc^3 = (n/f)^3;
 // the features (r / s, -r / s) with s the root mean square (sumsq[0] = sum
-// of r^2 in a fixed order through part, 256 doubles; mono zeroed); and guess = phi + scale * s * P(e) on
-// the interior (phi on the boundary), P the separable Catmull-Rom
-// interpolation from nc^3 to n^3 with odd reflection at the edges, through
-// t1 (nc * nc * n floats) and t2 (nc * n * n floats).
+// of r^2 in a fixed order through part, 256 doubles; mono zeroed); and the
+// correction scale * s * P(e) on the interior, 0 on the boundary (FP32,
+// split layout, into er / eb), P the separable Catmull-Rom interpolation
+// from nc^3 to n^3 with odd reflection at the edges, through t1
+// (nc * nc * n floats) and t2 (nc * n * n floats).
 hipError_t corr_restrict(const float* rr, const float* rb, int n, int f, float* out,
                          hipStream_t st);
 hipError_t corr_features(const float* r, int nc, double* sumsq, double* part, uint16_t* x,
                          float* mono, hipStream_t st);
 hipError_t corr_apply(const float* e, int nc, int n, float* t1, float* t2, const double* sumsq,
-                      double scale, const double* pr, const double* pb, double* gr, double* gb,
-                      hipStream_t st);
+                      double scale, float* er, float* eb, hipStream_t st);
 
 }  // namespace nn
 }  // namespace hyd

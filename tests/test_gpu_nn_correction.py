@@ -172,7 +172,9 @@ def test_corrected_start_matches_numpy(net):
     if kind == "constant":
         p = np.zeros(m.nparam, dtype=np.float32)
         p[H.lib().nn_model_param_offset(m.h, 11, 1)] = float(arg)
-        tol = 1e-5
+        # the solver applies the correction in FP32 (the inner solve's
+        # unknown) and its Laplacian in FP32
+        tol = 3e-5
     else:
         p = m.get_params()
         rng = np.random.default_rng(int(arg))
