@@ -464,7 +464,8 @@ def snapshot_study(H, ics, n, steps=200, every=20, big=(0, 100, 200)):
     the density rho_s and the potential of step s-1 (zero for s = 0) are
     taken, and each arm solves rho_s to a 1e-8 relative residual from that
     potential: plain FP64 SOR, FP32 defect correction with SOR sweeps, with
-    multigrid, and the learned guess + multigrid.  Errors are relative L2
+    multigrid, the learned guess + multigrid, and the start correction with
+    SOR sweeps and with multigrid.  Errors are relative L2
     against the plain FP64 SOR potential of the same solve.  At the steps in
     `big` the same snapshot is prolonged 2x2x2 piecewise constant (mass
     preserving) to 512^3 and solved by the three arms without the guess (the
@@ -476,6 +477,8 @@ def snapshot_study(H, ics, n, steps=200, every=20, big=(0, 100, 200)):
         print(f"[study] {msg}", file=sys.stderr, flush=True)
 
     model = nn.Model(path=nn.DEFAULT_WEIGHTS) if os.path.exists(nn.DEFAULT_WEIGHTS) else None
+    corr = (nn.Model(path=nn.CORRECTION_WEIGHTS) if os.path.exists(nn.CORRECTION_WEIGHTS)
+            else None)
     arms = [("plain_fp64_sor", H.POISSON_FP64_SOR, H.DC_INNER_SOR),
             ("fp32_dc", H.POISSON_FP32_DC, H.DC_INNER_SOR),
             ("fp32_dc_multigrid", H.POISSON_FP32_DC, H.DC_INNER_MULTIGRID)]
@@ -488,8 +491,15 @@ def snapshot_study(H, ics, n, steps=200, every=20, big=(0, 100, 200)):
         if with_nn and model is not None:
             g.set_nn_model(model)
             use.append(("nn_fp32_dc_multigrid", H.POISSON_NN_FP32_DC, H.DC_INNER_MULTIGRID))
+        if with_nn and corr is not None:
+            use += [("nn_correction_fp32_dc", H.POISSON_NN_FP32_DC, H.DC_INNER_SOR),
+                    ("nn_correction_fp32_dc_multigrid", H.POISSON_NN_FP32_DC,
+                     H.DC_INNER_MULTIGRID)]
         res, ref = {}, None
         for name, mode, inner in use:
+            # the potential guess alone, or the start correction alone
+            g.set_nn_model(model if name.startswith("nn_fp32") else None)
+            g.set_nn_correction(corr if name.startswith("nn_correction") else None)
             g.set_mode(mode)
             g.set_dc_inner(inner)
             g.set_params(1.5, 2000000, 1e-8)
@@ -499,7 +509,8 @@ def snapshot_study(H, ics, n, steps=200, every=20, big=(0, 100, 200)):
             r = {"status": st, "ms": g.s.last_solve_ms, "iterations": g.diagnostics()[0],
                  "rel_residual": g.s.last_rel_residual}
             if mode == H.POISSON_NN_FP32_DC:
-                r["guess_used"] = bool(g.s.last_guess == 1)
+                r["guess_used"] = bool(g.s.last_guess in (1, 2))
+                r["start_used"] = g.s.last_guess
             if ref is None:
                 ref = phi
             else:
@@ -507,6 +518,7 @@ def snapshot_study(H, ics, n, steps=200, every=20, big=(0, 100, 200)):
             res[name] = r
             log(f"{m}^3 {name}: {r['ms']:.1f} ms, {r['iterations']} iterations")
         g.set_nn_model(None)
+        g.set_nn_correction(None)
         g.close()
         plain = res["plain_fp64_sor"]["ms"]
         res["speedup_vs_plain"] = {k: plain / v["ms"] for k, v in res.items()
@@ -557,6 +569,8 @@ def main():
     ap.add_argument("--multigrid-probe", type=int, default=0, metavar="REPS",
                     help="only run REPS cold FP32 defect-correction solves with multigrid inner "
                          "solves (relative residual 1e-8) on the 256^3 explosion density, then exit")
+    ap.add_argument("--snapshot-no-512", action="store_true",
+                    help="the snapshot study without its 512^3 prolongations")
     ap.add_argument("--snapshot-study", type=int, default=0, metavar="STEPS",
                     help="only run the Poisson snapshot study on STEPS steps of the "
                          "self-gravitating explosion (snapshots every 20 steps, 512^3 "
@@ -612,7 +626,8 @@ def run(args):
     if args.snapshot_study > 0:
         k = args.snapshot_study
         return {"snapshot_study": snapshot_study(H, ics, n, steps=k, every=max(1, k // 10),
-                                                 big=(0, k // 2, k))}
+                                                 big=() if args.snapshot_no_512 else
+                                                 (0, k // 2, k))}
     if args.nn_probe > 0:
         from hydrocode_astro1_amd import nn
         return {"nn_probe": network_timing(H, nn.Model(path=nn.DEFAULT_WEIGHTS), 128,
