@@ -62,6 +62,10 @@ struct GravDevice {
     // relative residual; the defect correction then skips recomputing them
     bool start_residual_ready = false;
     double start_rel = 0.0;
+    // relative residual of the potential guess the last time it was tried
+    // (-1: never); the guess is skipped while the best start so far is under
+    // a quarter of it
+    double guess_last_rel = -1.0;
     // multigrid inner solve of the defect correction (DC_INNER_MULTIGRID):
     // pre- and post-smoothing sweeps per level and their SOR factor; V(1,1)
     // with 1.15 reached 1e-8 fastest at 256^3 (5.9 ms against 7.7 for

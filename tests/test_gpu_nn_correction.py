@@ -281,3 +281,34 @@ def test_trained_correction_at_256_with_multigrid():
     warm = slice(1, None)
     assert np.mean(used[warm] == 2) >= 0.9, used
     assert corr[warm].sum() < base[warm].sum(), (corr, base)
+
+
+def test_potential_guess_skipped_while_it_cannot_win():
+    """The potential guess is tried on a cold start and again after a
+    network is attached, and skipped while the best start has under a
+    quarter of the residual the guess had when last tried."""
+    n = 64
+    X = (np.arange(n) + 0.5) / n
+    x, y, z = np.meshgrid(X, X, X, indexing="ij")
+    rho = 1e-3 + np.exp(-((x - 0.5) ** 2 + (y - 0.45) ** 2 + (z - 0.5) ** 2) / 0.02)
+    g = H.Gravity(n, n, n, G=1.0)
+    g.set_criterion(H.SOR_CRIT_REL_RESID)
+    g.set_params(1.5, 400, 1e-8)
+    g.set_mode(H.POISSON_NN_FP32_DC)
+    g.set_dc_inner(H.DC_INNER_MULTIGRID)
+    m = nn.Model(seed=4)
+    g.set_nn_model(m)
+    assert g.solve_potential(rho) == 0
+    tried = g.s.last_guess_rel_residual
+    assert tried > 0 and g.s.last_guess in (0, 1)
+    # the converged potential as the start: far under a quarter of the guess's
+    assert g.solve_potential(rho) == 0
+    assert g.s.last_guess_rel_residual == -1.0 and g.s.last_guess == 0
+    assert 4.0 * g.s.last_warm_rel_residual < tried
+    assert g.diagnostics()[0] == 0
+    g.set_nn_model(m)  # attaching a network makes the next solve try it
+    assert g.solve_potential(rho) == 0
+    # the guess's input statistics are summed with atomics, so the same
+    # density gives the same guess to about 1e-5 (measured 4.5e-5 here)
+    assert g.s.last_guess_rel_residual == pytest.approx(tried, rel=1e-3)
+    assert g.s.last_guess == 0
